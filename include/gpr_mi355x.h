@@ -1,0 +1,152 @@
+/* C ABI of libgpr_mi355x.so: the ADER-WENO time step of the GPR system on one
+ * AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in replacement for the pybind11 module GPRpy built from
+ * code/cpp/bindings/bindings.cpp (haranjackson/phd).  Each entry point names
+ * the reference function it replaces.  Conventions:
+ *   - The caller owns every array; results are written in place, in the
+ *     reference's C-order layouts:
+ *       u   [nx(,ny)][V]               cells
+ *       ub  [(nx+2N)(,ny+2N)][V]       ghost-extended by N per side
+ *       wh  [(nx+2)(,ny+2)][N^ndim][V]
+ *       qh  [(nx+2)(,ny+2)][N^(ndim+1)][V]
+ *       mask[(nx+2)(,ny+2)] uint8
+ *   - Functions without _dev take host pointers, copy to the device, run on
+ *     the context's stream and synchronise before returning.  *_dev functions
+ *     take device pointers and only enqueue work (the caller synchronises).
+ *   - Return 0 on success, a negative GPR_E_* code otherwise; the message is
+ *     in gpr_last_error() (per thread).
+ *   - N (Gauss-Legendre nodes per dimension, 2 or 3), ndim (1 or 2) and
+ *     thermal (V = 14 or 17) are fixed per context; the reference fixes them
+ *     at compile time (code/cpp/src/options.h:5-35).
+ */
+#ifndef GPR_MI355X_H
+#define GPR_MI355X_H
+
+#include <stdint.h>
+
+#include "gpr_par.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPR_OK 0
+#define GPR_E_ARG (-1)          /* bad argument / shape */
+#define GPR_E_UNSUPPORTED (-2)  /* option outside this path (e.g. FLUX != 0) */
+#define GPR_E_HIP (-3)          /* HIP runtime error */
+#define GPR_E_EOS (-4)          /* EOS code not recognised (mg.cpp:27) */
+#define GPR_E_PREDICTOR (-5)    /* a cell's predictor converged in neither phase */
+#define GPR_E_NONFINITE (-6)    /* state became non-finite (iterator.cpp:161-165) */
+
+typedef struct gpr_ctx gpr_ctx;
+
+/* Solver settings beyond the reference's compile-time constants. */
+typedef struct gpr_solver_opts {
+  int32_t dg_it;      /* Picard cap, reference DG_IT = 50 (options.h:31) */
+  double dg_tol;      /* Picard tolerance, reference DG_TOL = 6e-6 (options.h:32) */
+  int32_t stiff_it;   /* cap of the preconditioned fallback iteration */
+  double stiff_tol;   /* its stopping tolerance (same rule as dg_tol) */
+  int32_t bailout;    /* 0: Picard runs to dg_it as in the reference; k > 0:
+                         leave it once its stopping statistic grew k times in a row */
+  int32_t mode;       /* 0: Picard, then the fallback for cells it leaves;
+                         1: the preconditioned iteration for every cell */
+  double contorted_tol; /* reset threshold of iterator(), its contorted_tol argument */
+} gpr_solver_opts;
+
+const char *gpr_last_error(void);
+int gpr_version(void);
+
+/* Context: device, stream, constants and work buffers. */
+int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal);
+int gpr_ctx_destroy(gpr_ctx *ctx);
+int gpr_ctx_set_opts(gpr_ctx *ctx, const gpr_solver_opts *opts);
+int gpr_ctx_get_opts(const gpr_ctx *ctx, gpr_solver_opts *opts);
+/* hipStream_t of the context (as void*), for callers that share device work */
+void *gpr_ctx_stream(gpr_ctx *ctx);
+
+/* options.N(), options.NV(), ... (bindings.cpp:83-88) */
+int gpr_options(const gpr_ctx *ctx, int32_t *N, int32_t *V, int32_t *ndim, int32_t *thermal);
+
+/* Basis constants (etc/globals.cpp:16-50; bindings.cpp:217-221 and
+ * solvers.weno.coefficient_matrices / oscillation_indicator, :236-237).
+ * Arrays of N, N, 2N and N*N (row-major) doubles. */
+int gpr_constants(int N, double *nodes, double *wghts, double *endvals, double *dervals,
+                  double *mLinv, double *mRinv, double *mCLinv, double *sig, double *dg_end,
+                  double *dg_der, double *dg_mat);
+
+/* --- pointwise physics over `count` states (system.*, bindings.cpp:197-206) */
+/* flux (equations.cpp:11-48): ret[count][V] = F_d(Q) (written, not accumulated) */
+int gpr_flux(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP);
+/* source (equations.cpp:50-67) */
+int gpr_source(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, const gpr_par *MP);
+/* Bdot (equations.cpp:149-167): ret = B_d(Q) x */
+int gpr_bdot(gpr_ctx *ctx, double *ret, const double *Q, const double *x, int64_t count, int d,
+             const gpr_par *MP);
+/* max_abs_eigs (eig.cpp:66-77): ret[count] */
+int gpr_max_abs_eigs(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP);
+/* thermo_acoustic_tensor (eig.cpp:55-64): ret[count][n][n], n = 3 (4 if thermal) */
+int gpr_thermo_acoustic_tensor(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, int d,
+                               const gpr_par *MP);
+/* solvers.fv.D_RUS (fluxes.cpp:66-75) and solvers.fv.Bint (fluxes.cpp:9-20) */
+int gpr_d_rus(gpr_ctx *ctx, double *ret, const double *qL, const double *qR, int64_t count, int d,
+              const gpr_par *MP);
+int gpr_bint(gpr_ctx *ctx, double *ret, const double *qL, const double *qR, int64_t count, int d,
+             const gpr_par *MP);
+
+/* --- grid (boundaries / extend_mask, grid.cpp:147-230, bindings.cpp:66-67) */
+int gpr_boundaries(gpr_ctx *ctx, const double *u, double *ub, const int32_t *nX, const int32_t *bcs);
+int gpr_extend_mask(gpr_ctx *ctx, const uint8_t *mask, uint8_t *maskb, const int32_t *nX);
+
+/* --- the three stages (bindings.cpp:235-253; steppers.cpp:10-22) */
+/* solvers.weno.weno_launcher(wh, ub, nX, MP) (weno.h:6) */
+int gpr_weno_launcher(gpr_ctx *ctx, double *wh, const double *ub, const int32_t *nX, const gpr_par *MP);
+/* solvers.dg.predictor(qh, wh, dt, dX, STIFF, STIFF_IG, MP, mask) (dg.h:7-8);
+ * ncell = mask length (ghost-extended cells), as the reference takes it from
+ * the mask.  iters (optional, [ncell]) receives the Picard count, 1000 + the
+ * fallback count, or -1 for a failed cell.  STIFF selects mode 1. */
+int gpr_predictor(gpr_ctx *ctx, double *qh, const double *wh, double dt, const double *dX,
+                  int64_t ncell, int stiff, int stiff_ig, const gpr_par *MP, const uint8_t *mask,
+                  int32_t *iters);
+/* solvers.fv.fv_launcher(u, rec, nX, dt, dX, SOURCES, TIME, FLUX, MP, mask)
+ * (fv.h:17-18); SOURCES = TIME = 1 and FLUX = 0 (Rusanov) are supported. */
+int gpr_fv_launcher(gpr_ctx *ctx, double *u, const double *qh, const int32_t *nX, double dt,
+                    const double *dX, int sources, int time, int flux, const gpr_par *MP,
+                    const uint8_t *mask);
+/* solvers.timestep (iterator.cpp:54-83) for one material: *dt_out */
+int gpr_timestep(gpr_ctx *ctx, double *dt_out, const double *u, const int32_t *nX, const double *dX,
+                 double CFL, double t, double tf, int64_t count, const gpr_par *MP);
+/* solvers.ader_stepper(u, ub, nX, dt, dX, STIFF, FLUX, MP, mask) (steppers.h:7-8) */
+int gpr_ader_stepper(gpr_ctx *ctx, double *u, const double *ub, const int32_t *nX, double dt,
+                     const double *dX, int stiff, int flux, const gpr_par *MP, const uint8_t *mask);
+/* solvers.renorm_distortion / contorted / reset_distortion (utils.cpp) */
+int gpr_renorm_distortion(gpr_ctx *ctx, double *u, int64_t ncell, const gpr_par *MP);
+int gpr_contorted(gpr_ctx *ctx, int32_t *result, const double *u, int64_t ncell, double tol);
+int gpr_reset_distortion(gpr_ctx *ctx, double *u, int64_t ncell, const gpr_par *MP);
+
+/* solvers.iterator (iterator.cpp:85-170) for one material (LSET = 0),
+ * SPLIT = 0, FLUX = 0.  u is advanced in place to tf.  snapshots
+ * ([100][ncell*V], may be null) receives u at each 1% of tf as the reference's
+ * returned list; *nsnap the number filled.  On a non-finite state the loop
+ * stops as at iterator.cpp:161-165: the last two snapshots hold the previous
+ * and the non-finite state, u holds the non-finite state, GPR_E_NONFINITE. */
+int gpr_iterator(gpr_ctx *ctx, double *u, double tf, const int32_t *nX, const double *dX, double CFL,
+                 const int32_t *bcs, int split, int half_step, int stiff, int flux, const gpr_par *MPs,
+                 int nmat, double contorted_tol, double *snapshots, int32_t *nsnap, int64_t *steps);
+
+/* --- device-resident time loop (device pointers) ---------------------------
+ * Prepare a run on a device-resident u (d_u, [cells][V] doubles): builds the
+ * mask, sizes the work buffers and computes the first dt on the device. */
+int gpr_dev_begin(gpr_ctx *ctx, double *d_u, const int32_t *nX, const double *dX, double CFL,
+                  const int32_t *bcs, double t0, double tf, int64_t count0, const gpr_par *MP);
+/* enqueue `nsteps` full steps (boundaries, WENO, predictor, FV + post, dt) */
+int gpr_dev_steps(gpr_ctx *ctx, int64_t nsteps);
+/* read back t, count, the last dt and the status flags (synchronises) */
+int gpr_dev_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t *pred_fail,
+                   int32_t *nonfinite);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPR_MI355X_H */

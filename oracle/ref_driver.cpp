@@ -1,0 +1,376 @@
+// C entry points into the reference solver, for building golden fixtures and
+// for timing the reference's CPU path.
+//
+// This file is linked against the reference's own sources, compiled where they
+// lie under code/cpp/src (see oracle/Makefile).  It only converts plain
+// pointers to the Eigen types the reference takes and calls the reference's
+// functions; it holds no numerics of its own.  Oracle infrastructure: nothing
+// in the product links or loads it.
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "etc/globals.h"
+#include "etc/grid.h"
+#include "solvers/dg/dg.h"
+#include "solvers/fv/fluxes.h"
+#include "solvers/fv/fv.h"
+#include "solvers/iterator.h"
+#include "solvers/steppers.h"
+#include "solvers/utils.h"
+#include "solvers/weno/weno.h"
+#include "system/eig.h"
+#include "system/equations.h"
+#include "system/objects.h"
+#include "system/variables/state.h"
+
+#include "../include/gpr_par.h"
+
+namespace {
+
+// gpr_par -> Par, with every field the pybind11 path leaves at zero also zero
+// (pybind11 value-initialises Par, bindings.cpp:115 / pybind11 detail/init.h).
+Par to_par(const gpr_par *p) {
+  Par P = Par();
+  P.EOS = p->EOS;
+  P.ρ0 = p->rho0;
+  P.Tref = p->Tref;
+  P.cv = p->cv;
+  P.pINF = p->pINF;
+  P.Γ0 = p->Gamma0;
+  P.c02 = p->c02;
+  P.s = p->s;
+  P.α = p->alpha;
+  P.β = p->beta;
+  P.γ = p->gamma;
+  P.A = p->A;
+  P.B = p->B;
+  P.R1 = p->R1;
+  P.R2 = p->R2;
+  P.b02 = p->b02;
+  P.μ = p->mu;
+  P.τ0 = p->tau0;
+  P.σY = p->sigmaY;
+  P.n = p->n;
+  P.cα2 = p->calpha2;
+  P.κ = p->kappa;
+  P.T0 = p->T0;
+  P.δp = Vec3(p->dp[0], p->dp[1], p->dp[2]);
+  P.SOLID = p->SOLID != 0;
+  P.POWER_LAW = p->POWER_LAW != 0;
+  P.MULTI = false;
+  P.BINGHAM = p->BINGHAM != 0;
+  P.REACTION = p->REACTION;
+  return P;
+}
+
+using VMap = Eigen::Map<Vec>;
+using BMap = Eigen::Map<bVec>;
+using IMap = Eigen::Map<iVec>;
+
+iVec make_nX(const int32_t *nX, int ndim) {
+  iVec r(ndim);
+  for (int i = 0; i < ndim; i++)
+    r(i) = nX[i];
+  return r;
+}
+
+Vec make_dX(const double *dX, int ndim) {
+  Vec r(ndim);
+  for (int i = 0; i < ndim; i++)
+    r(i) = dX[i];
+  return r;
+}
+
+long prod_ext(const int32_t *nX, int ndim, int ext) {
+  long r = 1;
+  for (int i = 0; i < ndim; i++)
+    r *= nX[i] + 2 * ext;
+  return r;
+}
+
+long ipow(long b, int e) {
+  long r = 1;
+  for (int i = 0; i < e; i++)
+    r *= b;
+  return r;
+}
+
+} // namespace
+
+extern "C" {
+
+int ref_N() { return N; }
+int ref_V() { return V; }
+
+// Basis and WENO constants built at load time (etc/globals.cpp:16-50).
+void ref_constants(double *nodes, double *wghts, double *endvals,
+                   double *dervals, double *mLinv_, double *mRinv_,
+                   double *mCLinv_, double *sig, double *dg_end,
+                   double *dg_der, double *dg_mat) {
+  for (int i = 0; i < N; i++) {
+    nodes[i] = NODES(i);
+    wghts[i] = WGHTS(i);
+    endvals[i] = ENDVALS(0, i);
+    endvals[N + i] = ENDVALS(1, i);
+    for (int j = 0; j < N; j++) {
+      dervals[i * N + j] = DERVALS(i, j);
+      mLinv_[i * N + j] = mLinv(i, j);
+      mRinv_[i * N + j] = mRinv(i, j);
+      mCLinv_[i * N + j] = mCLinv(i, j);
+      sig[i * N + j] = SIG(i, j);
+      dg_end[i * N + j] = DG_END(i, j);
+      dg_der[i * N + j] = DG_DER(i, j);
+      dg_mat[i * N + j] = DG_MAT(i, j);
+    }
+  }
+}
+
+// --- pointwise physics (system/equations.cpp, system/eig.cpp) --------------
+void ref_flux(double *ret, const double *Q, int d, const gpr_par *p) {
+  Par P = to_par(p);
+  VecV r = VecV::Zero();
+  VecV q = Eigen::Map<const VecV>(Q);
+  flux(r, q, d, P);
+  Eigen::Map<VecV>{ret} = r;
+}
+
+void ref_source(double *ret, const double *Q, const gpr_par *p) {
+  Par P = to_par(p);
+  VecV r;
+  VecV q = Eigen::Map<const VecV>(Q);
+  source(r, q, P);
+  Eigen::Map<VecV>{ret} = r;
+}
+
+void ref_bdot(double *ret, const double *Q, const double *x, int d,
+              const gpr_par *p) {
+  Par P = to_par(p);
+  VecV r = VecV::Zero();
+  VecV q = Eigen::Map<const VecV>(Q);
+  VecV xx = Eigen::Map<const VecV>(x);
+  Bdot(r, q, xx, d, P);
+  Eigen::Map<VecV>{ret} = r;
+}
+
+double ref_pressure(const double *Q, const gpr_par *p) {
+  Par P = to_par(p);
+  VecV q = Eigen::Map<const VecV>(Q);
+  return pressure(q, P);
+}
+
+double ref_max_abs_eigs(const double *Q, int d, const gpr_par *p) {
+  Par P = to_par(p);
+  VecV q = Eigen::Map<const VecV>(Q);
+  return max_abs_eigs(q, d, P);
+}
+
+// Thermo-acoustic tensor Ξ (eig.cpp:55-64); returns its order (3 or 4).
+int ref_thermo_acoustic_tensor(double *ret, const double *Q, int d,
+                               const gpr_par *p) {
+  Par P = to_par(p);
+  VecV q = Eigen::Map<const VecV>(Q);
+  Mat X = thermo_acoustic_tensor(q, d, P);
+  int n = X.rows();
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      ret[i * n + j] = X(i, j);
+  return n;
+}
+
+void ref_d_rus(double *ret, const double *qL, const double *qR, int d,
+               const gpr_par *p) {
+  Par P = to_par(p);
+  VecV l = Eigen::Map<const VecV>(qL), r = Eigen::Map<const VecV>(qR);
+  Eigen::Map<VecV>{ret} = D_RUS(l, r, d, P);
+}
+
+void ref_bint(double *ret, const double *qL, const double *qR, int d,
+              const gpr_par *p) {
+  Par P = to_par(p);
+  VecV l = Eigen::Map<const VecV>(qL), r = Eigen::Map<const VecV>(qR);
+  Eigen::Map<VecV>{ret} = Bint(l, r, d, P);
+}
+
+// --- grid (etc/grid.cpp) -----------------------------------------------------
+void ref_boundaries(const double *u, double *ub, const int32_t *nX, int ndim,
+                    const int32_t *bcs) {
+  long nc = prod_ext(nX, ndim, 0), nb = prod_ext(nX, ndim, N);
+  Vec uu = Eigen::Map<const Vec>(u, nc * V);
+  VMap ubm(ub, nb * V);
+  iVec n = make_nX(nX, ndim);
+  iVec b(2 * ndim);
+  for (int i = 0; i < 2 * ndim; i++)
+    b(i) = bcs[i];
+  boundaries(uu, ubm, n, b);
+}
+
+void ref_extend_mask(const uint8_t *mask, uint8_t *maskb, const int32_t *nX,
+                     int ndim) {
+  long nc = prod_ext(nX, ndim, 0), nb = prod_ext(nX, ndim, 1);
+  bVec m(nc), mb(nb);
+  for (long i = 0; i < nc; i++)
+    m(i) = mask[i] != 0;
+  iVec n = make_nX(nX, ndim);
+  extend_mask(m, mb, n);
+  for (long i = 0; i < nb; i++)
+    maskb[i] = mb(i) ? 1 : 0;
+}
+
+// --- the three stages of one ADER-WENO step (solvers/steppers.cpp:10-22) ----
+void ref_weno_launcher(double *wh, const double *ub, const int32_t *nX,
+                       int ndim, const gpr_par *p) {
+  Par P = to_par(p);
+  long nb = prod_ext(nX, ndim, N), nw = prod_ext(nX, ndim, 1) * ipow(N, ndim);
+  Vec ubv = Eigen::Map<const Vec>(ub, nb * V);
+  VMap whm(wh, nw * V);
+  iVec n = make_nX(nX, ndim);
+  weno_launcher(whm, ubv, n, P);
+}
+
+void ref_predictor(double *qh, const double *wh, double dt, const double *dX,
+                   const int32_t *nX, int ndim, int stiff, const gpr_par *p,
+                   const uint8_t *mask) {
+  Par P = to_par(p);
+  long ne = prod_ext(nX, ndim, 1);
+  long nw = ne * ipow(N, ndim), nq = ne * ipow(N, ndim + 1);
+  Vec whv = Eigen::Map<const Vec>(wh, nw * V);
+  VMap qhm(qh, nq * V);
+  Vec d = make_dX(dX, ndim);
+  bVec m(ne);
+  for (long i = 0; i < ne; i++)
+    m(i) = mask[i] != 0;
+  predictor(qhm, whv, dt, d, stiff != 0, false, P, m);
+}
+
+void ref_fv_launcher(double *u, const double *qh, const int32_t *nX, int ndim,
+                     double dt, const double *dX, const gpr_par *p,
+                     const uint8_t *mask) {
+  Par P = to_par(p);
+  long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
+  long nq = ne * ipow(N, ndim + 1);
+  VMap um(u, nc * V);
+  Vec qv = Eigen::Map<const Vec>(qh, nq * V);
+  Vec d = make_dX(dX, ndim);
+  iVec n = make_nX(nX, ndim);
+  bVec m(ne);
+  for (long i = 0; i < ne; i++)
+    m(i) = mask[i] != 0;
+  fv_launcher(um, qv, n, dt, d, true, true, 0, P, m);
+}
+
+double ref_timestep(const double *u, const int32_t *nX, int ndim,
+                    const double *dX, double CFL, double t, double tf,
+                    int count, const gpr_par *p) {
+  long nc = prod_ext(nX, ndim, 0);
+  std::vector<Vec> grids(1, Eigen::Map<const Vec>(u, nc * V));
+  std::vector<bVec> masks(1, bVec::Ones(nc));
+  std::vector<Par> MPs(1, to_par(p));
+  aVec d(ndim);
+  for (int i = 0; i < ndim; i++)
+    d(i) = dX[i];
+  return timestep(grids, masks, d, CFL, t, tf, count, MPs, 1);
+}
+
+void ref_renorm_distortion(double *u, long ncell, const gpr_par *p) {
+  std::vector<Par> MPs(1, to_par(p));
+  VMap um(u, ncell * V);
+  renorm_distortion(um, MPs);
+}
+
+int ref_contorted(const double *u, long ncell, double tol) {
+  Vec uu = Eigen::Map<const Vec>(u, ncell * V);
+  return contorted(uu, tol) ? 1 : 0;
+}
+
+void ref_reset_distortion(double *u, long ncell, const gpr_par *p) {
+  std::vector<Par> MPs(1, to_par(p));
+  VMap um(u, ncell * V);
+  reset_distortion(um, MPs);
+}
+
+// Whole ADER step on the x-slab OpenMP split (steppers.cpp:24-48).
+void ref_ader_stepper_para(double *u, const int32_t *nX, int ndim, double dt,
+                           const double *dX, const int32_t *bcs,
+                           const gpr_par *p) {
+  Par P = to_par(p);
+  long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
+  long nb = prod_ext(nX, ndim, N);
+  VMap um(u, nc * V);
+  Vec ub(nb * V);
+  iVec n = make_nX(nX, ndim);
+  iVec b(2 * ndim);
+  for (int i = 0; i < 2 * ndim; i++)
+    b(i) = bcs[i];
+  boundaries(um, ub, n, b);
+  bVec m = bVec::Ones(nc), mb(ne);
+  extend_mask(m, mb, n);
+  Vec d = make_dX(dX, ndim);
+  ader_stepper_para(um, ub, n, dt, d, false, 0, P, mb);
+}
+
+// Runs up to `nsteps` steps of the reference time loop (iterator.cpp:104-166,
+// LSET = 0, SPLIT = false, FLUX = 0, STIFF as given) from (t, count), writing
+// each step's dt into dts[] and its wall time into secs[] (may be null).  The
+// extended mask is built once: it is constant for a single all-true material
+// mask, and its 2D builder is O(nx ny (nx + ny)) (grid.cpp:213-226).
+// Returns the number of steps taken; -1 if u became non-finite.
+long ref_run(double *u, const int32_t *nX, int ndim, const double *dX,
+             double CFL, const int32_t *bcs, double *t_io, long *count_io,
+             double tf, long nsteps, double contorted_tol, int stiff,
+             const gpr_par *p, double *dts, double *secs) {
+  Par P = to_par(p);
+  std::vector<Par> MPs(1, P);
+  long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
+  long nb = prod_ext(nX, ndim, N);
+  VMap um(u, nc * V);
+  Vec ub(nb * V);
+  iVec n = make_nX(nX, ndim);
+  iVec b(2 * ndim);
+  for (int i = 0; i < 2 * ndim; i++)
+    b(i) = bcs[i];
+  aVec da(ndim);
+  for (int i = 0; i < ndim; i++)
+    da(i) = dX[i];
+  Vec dv = make_dX(dX, ndim);
+  std::vector<Vec> grids(1);
+  std::vector<bVec> masks(1, bVec::Ones(nc));
+  bVec mb(ne);
+  extend_mask(masks[0], mb, n);
+
+  double t = *t_io;
+  long count = *count_io;
+  long k = 0;
+  while (t < tf && k < nsteps) {
+    auto t0 = std::chrono::steady_clock::now();
+    grids[0] = um;
+    double dt = timestep(grids, masks, da, CFL, t, tf, count, MPs, 1);
+    boundaries(grids[0], ub, n, b);
+    ader_stepper_para(grids[0], ub, n, dt, dv, stiff != 0, 0, MPs[0], mb);
+    um = grids[0];
+    t += dt;
+    count += 1;
+    renorm_distortion(um, MPs);
+    if (contorted(um, contorted_tol))
+      reset_distortion(um, MPs);
+    auto t1 = std::chrono::steady_clock::now();
+    if (dts)
+      dts[k] = dt;
+    if (secs)
+      secs[k] = std::chrono::duration<double>(t1 - t0).count();
+    k++;
+    if (um.array().isNaN().any()) {
+      *t_io = t;
+      *count_io = count;
+      return -1;
+    }
+  }
+  *t_io = t;
+  *count_io = count;
+  return k;
+}
+
+} // extern "C"

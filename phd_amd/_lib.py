@@ -1,0 +1,167 @@
+"""ctypes binding of libgpr_mi355x.so (include/gpr_mi355x.h).
+
+The library is the only compute path: if it is missing or no HIP device is
+visible, every call raises.  There is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import Par
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpr_mi355x.so")
+
+GPR_OK = 0
+ERRORS = {-1: "GPR_E_ARG", -2: "GPR_E_UNSUPPORTED", -3: "GPR_E_HIP", -4: "GPR_E_EOS",
+          -5: "GPR_E_PREDICTOR", -6: "GPR_E_NONFINITE"}
+
+# exported entry points, as declared in include/gpr_mi355x.h
+SYMBOLS = [
+    "gpr_last_error", "gpr_version", "gpr_ctx_create", "gpr_ctx_destroy", "gpr_ctx_set_opts",
+    "gpr_ctx_get_opts", "gpr_ctx_stream", "gpr_options", "gpr_constants", "gpr_flux",
+    "gpr_source", "gpr_bdot", "gpr_max_abs_eigs", "gpr_thermo_acoustic_tensor", "gpr_d_rus",
+    "gpr_bint", "gpr_boundaries", "gpr_extend_mask", "gpr_weno_launcher", "gpr_predictor",
+    "gpr_fv_launcher", "gpr_timestep", "gpr_ader_stepper", "gpr_renorm_distortion",
+    "gpr_contorted", "gpr_reset_distortion", "gpr_iterator", "gpr_dev_begin", "gpr_dev_steps",
+    "gpr_dev_status",
+]
+
+
+class GprError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("dg_it", ctypes.c_int32), ("dg_tol", ctypes.c_double),
+                ("stiff_it", ctypes.c_int32), ("stiff_tol", ctypes.c_double),
+                ("bailout", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("contorted_tol", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_Dp = ctypes.POINTER(ctypes.c_double)
+_Ip = ctypes.POINTER(ctypes.c_int32)
+_Up = ctypes.POINTER(ctypes.c_uint8)
+_Lp = ctypes.POINTER(ctypes.c_int64)
+_d = ctypes.c_double
+_i = ctypes.c_int
+_l = ctypes.c_int64
+_M = ctypes.POINTER(Par)
+
+# argument types, in the order of include/gpr_mi355x.h
+ARGTYPES = {
+    "gpr_ctx_create": [ctypes.POINTER(ctypes.c_void_p), _i, _i, _i],
+    "gpr_ctx_destroy": [_P],
+    "gpr_ctx_set_opts": [_P, ctypes.POINTER(SolverOpts)],
+    "gpr_ctx_get_opts": [_P, ctypes.POINTER(SolverOpts)],
+    "gpr_ctx_stream": [_P],
+    "gpr_options": [_P, _Ip, _Ip, _Ip, _Ip],
+    "gpr_constants": [_i] + [_Dp] * 11,
+    "gpr_flux": [_P, _Dp, _Dp, _l, _i, _M],
+    "gpr_source": [_P, _Dp, _Dp, _l, _M],
+    "gpr_bdot": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
+    "gpr_max_abs_eigs": [_P, _Dp, _Dp, _l, _i, _M],
+    "gpr_thermo_acoustic_tensor": [_P, _Dp, _Dp, _l, _i, _M],
+    "gpr_d_rus": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
+    "gpr_bint": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
+    "gpr_boundaries": [_P, _Dp, _Dp, _Ip, _Ip],
+    "gpr_extend_mask": [_P, _Up, _Up, _Ip],
+    "gpr_weno_launcher": [_P, _Dp, _Dp, _Ip, _M],
+    "gpr_predictor": [_P, _Dp, _Dp, _d, _Dp, _l, _i, _i, _M, _Up, _Ip],
+    "gpr_fv_launcher": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _i, _M, _Up],
+    "gpr_timestep": [_P, _Dp, _Dp, _Ip, _Dp, _d, _d, _d, _l, _M],
+    "gpr_ader_stepper": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
+    "gpr_renorm_distortion": [_P, _Dp, _l, _M],
+    "gpr_contorted": [_P, _Ip, _Dp, _l, _d],
+    "gpr_reset_distortion": [_P, _Dp, _l, _M],
+    "gpr_iterator": [_P, _Dp, _d, _Ip, _Dp, _d, _Ip, _i, _i, _i, _i, _M, _i, _d, _Dp, _Ip, _Lp],
+    "gpr_dev_begin": [_P, _P, _Ip, _Dp, _d, _Ip, _d, _d, _l, _M],
+    "gpr_dev_steps": [_P, _l],
+    "gpr_dev_status": [_P, _Dp, _Lp, _Dp, _Ip, _Ip],
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m phd_amd.build` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.gpr_last_error.restype = ctypes.c_char_p
+        L.gpr_ctx_stream.restype = ctypes.c_void_p
+        for name in SYMBOLS:
+            if name not in ("gpr_last_error", "gpr_ctx_stream"):
+                getattr(L, name).restype = ctypes.c_int
+        for name, args in ARGTYPES.items():
+            getattr(L, name).argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != GPR_OK:
+        raise GprError(rc, load().gpr_last_error().decode())
+    return rc
+
+
+D = ctypes.POINTER(ctypes.c_double)
+I32 = ctypes.POINTER(ctypes.c_int32)
+U8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def dptr(a):
+    return a.ctypes.data_as(D)
+
+
+def iptr(a):
+    return a.ctypes.data_as(I32)
+
+
+def u8ptr(a):
+    return a.ctypes.data_as(U8)
+
+
+def par_ref(MP):
+    if not isinstance(MP, Par):
+        raise TypeError("MP must be a phd_amd.params.Par")
+    return ctypes.pointer(MP)
+
+
+class Context:
+    """One gpr_ctx: (N, ndim, thermal) fixed, its own HIP stream and buffers."""
+
+    def __init__(self, N=3, ndim=2, thermal=False):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.gpr_ctx_create(ctypes.byref(h), int(N), int(ndim), int(bool(thermal))))
+        self.h = h
+        self.N, self.ndim, self.thermal = int(N), int(ndim), bool(thermal)
+        self.V = 17 if thermal else 14
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.gpr_ctx_destroy(h)
+            self.h = None
+
+    @property
+    def opts(self):
+        o = SolverOpts()
+        check(load().gpr_ctx_get_opts(self.h, ctypes.byref(o)))
+        return o
+
+    def set_opts(self, **kw):
+        o = self.opts
+        for k, v in kw.items():
+            setattr(o, k, v)
+        check(load().gpr_ctx_set_opts(self.h, ctypes.byref(o)))

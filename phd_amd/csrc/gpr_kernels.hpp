@@ -1,0 +1,895 @@
+// HIP kernels of one ADER-WENO step for the GPR system on gfx950.
+//
+// Pipeline (one step, all on one stream, dt kept on the device):
+//   k_boundaries  u -> ub (ghost layers, etc/grid.cpp:147-170)
+//   k_weno_*      ub -> wh (solvers/weno/weno.cpp:75-157)
+//   k_predictor   wh -> qh (solvers/dg/dg1.cpp, dg2.cpp): one space-time node
+//                 per lane, the cell's nodes in one wavefront, contractions
+//                 through LDS, Picard with the reference stopping rule and a
+//                 stiff-source preconditioned fallback
+//   k_fv_faces    qh -> face Rusanov + path-conservative jumps (fv.cpp:162-268)
+//   k_fv_cells    cell sources/B terms + face gather -> u (fv.cpp:56-109),
+//                 fused with the post-step distortion renormalisation, the
+//                 contortion flag and the next step's CFL maximum
+//   k_reset / k_finalize_dt   (iterator.cpp:54-83,145-151, utils.cpp:44-85)
+#pragma once
+
+#include "gpr_device.hpp"
+
+namespace gpr {
+
+constexpr int WAVE = 64;
+
+// Device-resident time-loop state.
+struct StepState {
+  double t, tf, cfl, dt;
+  long long count;
+  unsigned long long maxspeed;  // bit pattern of a non-negative double (max over cells)
+  unsigned long long maxspeed2; // same, recomputed after a global distortion reset
+  int contorted;
+  int nonfinite;
+  int pred_fail;
+  int done;
+  double contorted_tol;
+  int pushcount;  // snapshots taken
+  int pad_;
+};
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *addr, double v) {
+  if (!(v >= 0.0)) return;  // NaN (and negatives) never enter the maximum, as in std::max
+  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T u = __shfl_xor(v, o);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------- boundaries
+// One thread per (ghost-extended cell, variable).  Boundary codes 0
+// transmissive, 1 periodic, 2 slip, 3 stick, 5 symmetric (grid.cpp:5-10).
+__device__ __forceinline__ int bc_src(int I, int n, int N, int code, bool &refl) {
+  refl = false;
+  if (I >= N && I < n + N) return I - N;
+  bool left = I < N;
+  int i = left ? I : I - n - N;  // 0..N-1
+  switch (code) {
+  case 0: return left ? 0 : n - 1;
+  case 1: return left ? n - N + i : i;
+  default: refl = true; return left ? N - 1 - i : n - 1 - i;
+  }
+}
+__device__ __forceinline__ double bc_sign(int v, int code, int d) {
+  if (code == 2) return v == 2 + d ? -1.0 : 1.0;
+  if (code == 3) return (v >= 2 && v <= 4) ? -1.0 : 1.0;
+  if (code == 5) {
+    if (v == 2 + d) return -1.0;
+    if (v >= 5 && v < 14) {
+      int k = v - 5, r = k / 3, c = k % 3;
+      return ((r == d) != (c == d)) ? -1.0 : 1.0;  // row d and column d flipped: the (d,d) entry twice
+    }
+  }
+  return 1.0;
+}
+
+template <int V, int ND>
+__global__ void k_boundaries(const double *__restrict__ u, double *__restrict__ ub, int nx, int ny, int N,
+                             int4 bc) {
+  long nby = ND == 2 ? ny + 2 * N : 1;
+  long total = (long)(nx + 2 * N) * nby * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    int v = id % V;
+    long c = id / V;
+    int J = ND == 2 ? c % nby : 0;
+    int I = ND == 2 ? c / nby : c;
+    bool rx, ry = false;
+    int codex = I < N ? bc.x : bc.y;
+    int i = bc_src(I, nx, N, codex, rx);
+    double sgn = rx ? bc_sign(v, codex, 0) : 1.0;
+    int j = 0;
+    if (ND == 2) {
+      int codey = J < N ? bc.z : bc.w;
+      j = bc_src(J, ny, N, codey, ry);
+      if (ry) sgn *= bc_sign(v, codey, 1);
+    }
+    long src = ND == 2 ? ((long)i * ny + j) * V + v : (long)i * V + v;
+    ub[id] = sgn * u[src];
+  }
+}
+
+// ---------------------------------------------------------------------- WENO
+// Reconstruction of one variable over one stencil line (coeffs, weno.cpp:17-73)
+// in difference form about the central value c: w_s = c + M_s^-1 (d - c), whose
+// oscillation indicator and weighted mean are those of the direct form exactly
+// (the basis sums to one, so M_s^-1 maps constants to themselves and SIG
+// annihilates them) but do not cancel when |c| >> |d - c|.
+template <typename R, int N>
+__device__ __forceinline__ void weno_point(const R *d, const Basis &K, int ncell, int gap, int ind, R *out) {
+  constexpr int FN2 = (N - 1) / 2;
+  R c = d[N - 1];
+  R dd[2 * N - 1];
+#pragma unroll
+  for (int r = 0; r < 2 * N - 1; r++) dd[r] = d[r] - c;
+  R ws[3][N];
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    R sl = 0, sr = 0, sc = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      sl += R(K.mLinv[a][k]) * dd[k];
+      sr += R(K.mRinv[a][k]) * dd[N - 1 + k];
+      sc += R(K.mCLinv[a][k]) * dd[FN2 + k];
+    }
+    ws[0][a] = sl;
+    ws[1][a] = sr;
+    ws[2][a] = sc;
+  }
+  constexpr int NSTEN = N > 2 ? 3 : 2;
+  R o[3];
+#pragma unroll
+  for (int s = 0; s < NSTEN; s++) {
+    R qv = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+      for (int j = 0; j < N; j++) qv += ws[s][i] * R(K.sig[i][j]) * ws[s][j];
+    R t = qv + R(1e-14);
+    R t2 = t * t, t4 = t2 * t2;
+    o[s] = R(s == 2 ? 1e5 : 1.0) / (t4 * t4);
+  }
+  bool useL = ind >= gap, useR = ind <= ncell - gap - 1, useC = N > 2 && ind + FN2 >= gap;
+  R num[N], den = 0;
+#pragma unroll
+  for (int a = 0; a < N; a++) num[a] = 0;
+  if (useL) {
+#pragma unroll
+    for (int a = 0; a < N; a++) num[a] += ws[0][a] * o[0];
+    den += o[0];
+  }
+  if (useR) {
+#pragma unroll
+    for (int a = 0; a < N; a++) num[a] += ws[1][a] * o[1];
+    den += o[1];
+  }
+  if (useC) {
+#pragma unroll
+    for (int a = 0; a < N; a++) num[a] += ws[2][a] * o[2];
+    den += o[2];
+  }
+#pragma unroll
+  for (int a = 0; a < N; a++) out[a] = c + num[a] / den;
+}
+
+// 1D: wh[(nx+2)][N][V] from ub[(nx+2N)][V] (weno_launcher ndim 1, weno.cpp:145-147)
+template <typename R, int N, int V>
+__global__ void k_weno_1d(const double *__restrict__ ub, R *__restrict__ wh, int nx, Basis K) {
+  int NX = nx + 2;
+  long total = (long)NX * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    int v = id % V, i = id / V;
+    R d[2 * N - 1], o[N];
+#pragma unroll
+    for (int r = 0; r < 2 * N - 1; r++) d[r] = R(ub[(long)(i + r) * V + v]);
+    weno_point<R, N>(d, K, NX, 0, i, o);
+#pragma unroll
+    for (int a = 0; a < N; a++) wh[((long)i * N + a) * V + v] = o[a];
+  }
+}
+
+// 2D x-pass: ux[(nx+2)][(ny+2N)][N][V] (weno.cpp:93-104, NO_CORNERS gap N on
+// the first and last N columns of the ghost-extended grid)
+template <typename R, int N, int V>
+__global__ void k_weno_x(const double *__restrict__ ub, R *__restrict__ ux, int nx, int ny, Basis K) {
+  int NX = nx + 2, ncol = ny + 2 * N;
+  long total = (long)NX * ncol * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    int v = id % V;
+    long c = id / V;
+    int j = c % ncol, i = c / ncol;
+    int gap = (j < N || j >= ncol - N) ? N : 0;
+    R d[2 * N - 1], o[N];
+#pragma unroll
+    for (int r = 0; r < 2 * N - 1; r++) d[r] = R(ub[((long)(i + r) * ncol + j) * V + v]);
+    weno_point<R, N>(d, K, NX, gap, i, o);
+#pragma unroll
+    for (int a = 0; a < N; a++) ux[((c)*N + a) * V + v] = o[a];
+  }
+}
+
+// 2D y-pass: wh[(nx+2)(ny+2)][N][N][V] (weno.cpp:109-121, gap 1 on the first
+// and last ghost rows)
+template <typename R, int N, int V>
+__global__ void k_weno_y(const R *__restrict__ ux, R *__restrict__ wh, int nx, int ny, Basis K) {
+  int NX = nx + 2, NY = ny + 2, ncol = ny + 2 * N;
+  long total = (long)NX * NY * N * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    int v = id % V;
+    long c = id / V;
+    int a = c % N;
+    c /= N;
+    int j = c % NY, i = c / NY;
+    int gap = (i == 0 || i == NX - 1) ? 1 : 0;
+    R d[2 * N - 1], o[N];
+#pragma unroll
+    for (int r = 0; r < 2 * N - 1; r++) d[r] = ux[(((long)i * ncol + j + r) * N + a) * V + v];
+    weno_point<R, N>(d, K, NY, gap, j, o);
+#pragma unroll
+    for (int b = 0; b < N; b++) wh[((((long)i * NY + j) * N + a) * N + b) * V + v] = o[b];
+  }
+}
+
+// ----------------------------------------------------------------- predictor
+struct PredArgs {
+  double *qh;              // [ncell][NN][V]
+  const double *wh;        // [ncell][NS][V]
+  const uint8_t *mask;     // [ncell]
+  int32_t *iters;          // optional per-cell iteration count (fallback: 1000 + its)
+  int *fail;               // device counter of cells that converged in neither phase
+  const StepState *st;     // if non-null, dt is read from st->dt
+  double dt, dx, dy;
+  long ncell;
+  int dg_it, stiff_it, bailout, mode;  // mode 0: Picard then fallback; 1: fallback only
+  double dg_tol, stiff_tol;
+  Basis K;
+  Mat m;
+};
+
+// Preconditioner of one spatial node, frozen at its WENO state ws:
+// Ainv = (DG_MAT + dt kappa W)^-1, Jinv = (DG_MAT + dt rJ W)^-1 and the polar
+// rotation Rot of A, where the distortion source has Jacobian -kappa Pi at a
+// relaxed state (Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3)).
+template <int N, bool TH>
+__device__ void stiff_precond(const double *ws, const PredArgs &a, double dt, double *out27) {
+  NodeState<double, TH> s;
+  node_state<double, TH>(ws, a.m, s);
+  double dA = det3(ws + 5);
+  double s13 = cbrt(dA);
+  double kap = 2.0 * theta1inv<double, TH>(ws, s, a.m) * s.cs2 * s13 * s13;
+  if (!(kap >= 0.0) || !(kap < 1e300)) kap = 0.0;
+  double rJ = 0.0;
+  if (TH) {
+    rJ = a.m.p.calpha2 * s.T / (a.m.p.kappa * s.r);
+    if (!(rJ >= 0.0) || !(rJ < 1e300)) rJ = 0.0;
+  }
+  for (int which = 0; which < 2; which++) {
+    double rate = which == 0 ? kap : rJ;
+    double M[N][2 * N];
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < 2 * N; j++)
+        M[i][j] = j < N ? a.K.dgmat[i][j] + (i == j ? dt * rate * a.K.w[i] : 0.0) : (j - N == i ? 1.0 : 0.0);
+    for (int c = 0; c < N; c++) {
+      int piv = c;
+      for (int r = c + 1; r < N; r++)
+        if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+      if (piv != c)
+        for (int j = 0; j < 2 * N; j++) { double tt = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = tt; }
+      double dg = M[c][c];
+      for (int j = 0; j < 2 * N; j++) M[c][j] /= dg;
+      for (int r = 0; r < N; r++) {
+        if (r == c) continue;
+        double f = M[r][c];
+        for (int j = 0; j < 2 * N; j++) M[r][j] -= f * M[c][j];
+      }
+    }
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++) out27[which * 9 + i * N + j] = M[i][N + j];
+  }
+  double X[9];
+  for (int i = 0; i < 9; i++) X[i] = ws[5 + i];
+  if (dA > 0) {
+    for (int it = 0; it < 40; it++) {
+      double d = det3(X), inv[9];
+      inv[0] = (X[4] * X[8] - X[5] * X[7]) / d;
+      inv[1] = (X[2] * X[7] - X[1] * X[8]) / d;
+      inv[2] = (X[1] * X[5] - X[2] * X[4]) / d;
+      inv[3] = (X[5] * X[6] - X[3] * X[8]) / d;
+      inv[4] = (X[0] * X[8] - X[2] * X[6]) / d;
+      inv[5] = (X[2] * X[3] - X[0] * X[5]) / d;
+      inv[6] = (X[3] * X[7] - X[4] * X[6]) / d;
+      inv[7] = (X[1] * X[6] - X[0] * X[7]) / d;
+      inv[8] = (X[0] * X[4] - X[1] * X[3]) / d;
+      double ch = 0;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double nx = 0.5 * (X[i * 3 + j] + inv[j * 3 + i]);
+          ch = fmax(ch, fabs(nx - X[i * 3 + j]));
+          X[i * 3 + j] = nx;
+        }
+      if (ch < 1e-15) break;
+    }
+  } else {
+    for (int i = 0; i < 9; i++) X[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
+  for (int i = 0; i < 9; i++) out27[18 + i] = X[i];
+}
+
+// Pi(X) = Rot dev(sym(Rot^T X))
+template <typename R> __device__ __forceinline__ void proj_dev_sym(const double *Rot, const R *X, R *out) {
+  R Y[9], S[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      Y[i * 3 + j] = R(Rot[0 * 3 + i]) * X[0 * 3 + j] + R(Rot[1 * 3 + i]) * X[1 * 3 + j] + R(Rot[2 * 3 + i]) * X[2 * 3 + j];
+  R tr = (Y[0] + Y[4] + Y[8]) / R(3);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) S[i * 3 + j] = R(0.5) * (Y[i * 3 + j] + Y[j * 3 + i]) - (i == j ? tr : R(0));
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      out[i * 3 + j] = R(Rot[i * 3 + 0]) * S[0 * 3 + j] + R(Rot[i * 3 + 1]) * S[1 * 3 + j] + R(Rot[i * 3 + 2]) * S[2 * 3 + j];
+}
+
+// One wavefront of 64 lanes holds CPW cells, one space-time node per lane.
+template <int N, int ND> struct PredShape {
+  static constexpr int NS = ND == 1 ? N : N * N;  // spatial nodes per cell
+  static constexpr int NN = NS * N;                // space-time nodes per cell
+  static constexpr int CPW = WAVE / NN;            // cells per wavefront
+};
+
+template <int N, bool TH, int ND>
+__global__ __launch_bounds__(64) void k_predictor(PredArgs a) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int NS = PredShape<N, ND>::NS;
+  constexpr int NN = PredShape<N, ND>::NN;
+  constexpr int CPW = PredShape<N, ND>::CPW;
+  using R = double;
+  __shared__ R sQ[CPW * NN * V];
+  __shared__ R sF[CPW * NN * V];
+  __shared__ R sG[ND == 2 ? CPW * NN * V : CPW * NN * 9];
+  __shared__ double sPre[CPW * NS * 27];
+  __shared__ R sRat[WAVE];
+
+  const Basis &K = a.K;
+  const int lane = threadIdx.x;
+  const int slot = lane / NN;
+  const int node = lane - slot * NN;
+  const int t = node / NS;
+  const int s = node - t * NS;
+  const int ia = ND == 1 ? s : s / N;
+  const int ib = ND == 1 ? 0 : s - (s / N) * N;
+  const long cell = (long)blockIdx.x * CPW + slot;
+  const bool lane_ok = slot < CPW;
+  const bool valid = lane_ok && cell < a.ncell;
+  const bool active = valid && a.mask[cell];
+  const int sl = slot < CPW ? slot : 0;  // idle lanes alias slot 0 for LDS addresses they never write
+  R *myQ = sQ + (sl * NN) * V;
+  R *myF = sF + (sl * NN) * V;
+  R *myG = sG + (sl * NN) * (ND == 2 ? V : 9);
+  double *myPre = sPre + (sl * NS + s) * 27;
+
+  const double dt = a.st ? a.st->dt : a.dt;
+  const R Ws = ND == 1 ? R(K.w[ia]) : R(K.w[ia]) * R(K.w[ib]);
+  const R Wt = R(K.w[t]);
+  const R e0t = R(K.endv[0][t]);
+
+  R w[V], q0[V], q1[V];
+  if (active) {
+#pragma unroll
+    for (int v = 0; v < V; v++) w[v] = a.wh[(cell * NS + s) * V + v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) w[v] = v == 0 ? R(1) : (v == 5 || v == 9 || v == 13 ? R(1) : R(0));
+  }
+#pragma unroll
+  for (int v = 0; v < V; v++) q0[v] = w[v];
+
+  // phase: 0 Picard, 1 preconditioned, 2 converged, 3 idle / given up
+  int phase = active ? (a.mode == 1 ? 1 : 0) : 3;
+  int it = 0, pit = 0, grow = 0;
+  R prev = R(1e308);
+  const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * NN);
+  bool need_pre = phase == 1;
+
+  while (__any(phase == 0 || phase == 1)) {
+    if (need_pre && t == 0 && lane_ok) stiff_precond<N, TH>(w, a, dt, myPre);
+    need_pre = false;
+    // (1) share the iterate (lanes past the last whole cell write nothing)
+    if (lane_ok)
+#pragma unroll
+      for (int v = 0; v < V; v++) myQ[node * V + v] = q0[v];
+    __syncthreads();
+
+    // (2) spatial derivatives, local source and non-conservative terms, fluxes
+    R rhs[V];
+    {
+      R dq[V];
+      NodeState<R, TH> st;
+      node_state<R, TH>(q0, a.m, st);
+      source<R, TH>(q0, st, a.m, rhs);
+      const R inv_dx = R(1) / R(a.dx);
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R acc = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) acc += R(K.der[ia][j]) * myQ[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
+        dq[v] = acc * inv_dx;
+      }
+      bdot_sub(st.v, dq, 0, R(1), rhs);
+      if (ND == 2) {
+        const R inv_dy = R(1) / R(a.dy);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          R acc = 0;
+#pragma unroll
+          for (int j = 0; j < N; j++) acc += R(K.der[ib][j]) * myQ[(t * NS + ia * N + j) * V + v];
+          dq[v] = acc * inv_dy;
+        }
+        bdot_sub(st.v, dq, 1, R(1), rhs);
+      }
+      const R wts = Wt * Ws;
+#pragma unroll
+      for (int v = 0; v < V; v++) rhs[v] *= wts;
+      R F[V];
+      flux<R, TH>(q0, st, a.m, 0, F);
+      const R wb = ND == 2 ? R(K.w[ib]) : R(1);
+      if (lane_ok)
+#pragma unroll
+        for (int v = 0; v < V; v++) myF[node * V + v] = F[v] * wb;
+      if (ND == 2) {
+        flux<R, TH>(q0, st, a.m, 1, F);
+        if (lane_ok)
+#pragma unroll
+          for (int v = 0; v < V; v++) myG[node * V + v] = F[v];
+      }
+    }
+    __syncthreads();
+    // (3) flux contractions (rhs2, dg2.cpp:173-201), then dt and the initial term
+    {
+      const R cx = Wt / R(a.dx);
+      const R cy = ND == 2 ? Wt * R(K.w[ia]) / R(a.dy) : R(0);
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R sx = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) sx += R(K.dgder[ia][j]) * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
+        R val = rhs[v] - cx * sx;
+        if (ND == 2) {
+          R sy = 0;
+#pragma unroll
+          for (int j = 0; j < N; j++) sy += R(K.dgder[ib][j]) * myG[(t * NS + ia * N + j) * V + v];
+          val -= cy * sy;
+        }
+        rhs[v] = val * R(dt) + e0t * Ws * w[v];
+      }
+    }
+    __syncthreads();
+    // (4) time solve: Picard q1 = (DG_MAT (x) W_s)^-1 rhs, or the preconditioned update
+    if (phase == 1) {
+      R g[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R acc = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += R(K.dgmat[t][k]) * myQ[(k * NS + s) * V + v];
+        g[v] = acc - rhs[v] / Ws;
+      }
+      R P[9];
+      proj_dev_sym<R>(myPre + 18, g + 5, P);
+      R *pg = myG + node * (ND == 2 ? V : 9) + (ND == 2 ? 5 : 0);
+      if (lane_ok) {
+#pragma unroll
+        for (int v = 0; v < V; v++) myF[node * V + v] = g[v];
+#pragma unroll
+        for (int k = 0; k < 9; k++) pg[k] = P[k];
+      }
+    } else if (lane_ok) {
+#pragma unroll
+      for (int v = 0; v < V; v++) myF[node * V + v] = rhs[v];
+    }
+    __syncthreads();
+    if (phase == 1) {
+      const double *Ainv = myPre, *Jinv = myPre + 9;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R d = 0;
+        if (v >= 5 && v < 14) {
+#pragma unroll
+          for (int k = 0; k < N; k++) {
+            R gk = myF[(k * NS + s) * V + v];
+            R pk = myG[(k * NS + s) * (ND == 2 ? V : 9) + (ND == 2 ? v : v - 5)];
+            d += R(K.dgmatinv[t][k]) * (gk - pk) + R(Ainv[t * N + k]) * pk;
+          }
+        } else if (v >= 14) {
+#pragma unroll
+          for (int k = 0; k < N; k++) d += R(Jinv[t * N + k]) * myF[(k * NS + s) * V + v];
+        } else {
+#pragma unroll
+          for (int k = 0; k < N; k++) d += R(K.dgmatinv[t][k]) * myF[(k * NS + s) * V + v];
+        }
+        q1[v] = q0[v] - d;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        R acc = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += R(K.dgmatinv[t][k]) * myF[(k * NS + s) * V + v];
+        q1[v] = acc / Ws;
+      }
+    }
+    // (5) stopping rule over the cell's whole block (dg2.cpp:278-297)
+    const R tol = phase == 1 ? R(a.stiff_tol) : R(a.dg_tol);
+    bool more = false, nan = false;
+    R ratio = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      R dd = fabs(q1[v] - q0[v]), lim = tol * (R(1) + fabs(q0[v]));
+      more |= dd > lim;
+      nan |= q1[v] != q1[v];
+      ratio = fmax(ratio, dd / lim);
+    }
+    const unsigned long long bm = __ballot(more), bn = __ballot(nan);
+    const bool cmore = (bm & cellbits) != 0, cnan = (bn & cellbits) != 0;
+    if (a.bailout > 0) {
+      sRat[lane] = ratio;
+      __syncthreads();
+      R cr = 0;
+      for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
+      ratio = cr;
+      __syncthreads();
+    }
+    if (phase == 0) {
+      it++;
+      bool to_fallback = false;
+      if (cmore) {
+#pragma unroll
+        for (int v = 0; v < V; v++) q0[v] = q1[v];
+        grow = ratio > prev ? grow + 1 : 0;
+        prev = ratio;
+        if (it >= a.dg_it || (a.bailout > 0 && grow >= a.bailout)) to_fallback = true;
+      } else if (cnan) {
+        to_fallback = true;
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = q1[v];
+        if (a.iters && node == 0) a.iters[cell] = it;
+        phase = 2;
+      }
+      if (to_fallback) {
+        phase = 1;
+        pit = 0;
+        need_pre = true;
+#pragma unroll
+        for (int v = 0; v < V; v++) q0[v] = w[v];
+      }
+    } else if (phase == 1) {
+      pit++;
+#pragma unroll
+      for (int v = 0; v < V; v++) q0[v] = q1[v];
+      if (!cmore || pit >= a.stiff_it) {
+#pragma unroll
+        for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = q1[v];
+        bool ok = !cmore && !cnan;
+        if (node == 0) {
+          if (a.iters) a.iters[cell] = ok ? 1000 + pit : -1;
+          if (!ok) atomicAdd(a.fail, 1);
+        }
+        phase = ok ? 2 : 3;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------ FV
+// Rusanov flux D_RUS (fluxes.cpp:66-75) and the path integral Bint
+// (fluxes.cpp:9-20) between two states, accumulated as
+//   lo += k (b + f),  hi += k (b - f).
+template <bool TH>
+__device__ __forceinline__ void rusanov(const double *qL, const double *qR, int d, const Mat &m, double *f) {
+  constexpr int V = Dims<TH>::V;
+  double sL = max_abs_eigs<TH>(qL, m, d), sR = max_abs_eigs<TH>(qR, m, d);
+  double smax = (sL < sR) ? sR : sL;  // std::max(max1, max2): a NaN on the right is dropped
+  double F[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) f[v] = smax * (qL[v] - qR[v]);
+  {
+    NodeState<double, TH> st;
+    node_state<double, TH>(qL, m, st);
+    flux<double, TH>(qL, st, m, d, F);
+#pragma unroll
+    for (int v = 0; v < V; v++) f[v] += F[v];
+    node_state<double, TH>(qR, m, st);
+    flux<double, TH>(qR, st, m, d, F);
+#pragma unroll
+    for (int v = 0; v < V; v++) f[v] += F[v];
+  }
+}
+
+template <int N, bool TH>
+__device__ __forceinline__ void bint(const double *qL, const double *qR, int d, const Basis &K, double *b) {
+  constexpr int V = Dims<TH>::V;
+#pragma unroll
+  for (int v = 0; v < V; v++) b[v] = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    double xi = K.x[i];
+    double r = qL[0] + xi * (qR[0] - qL[0]);
+    double vv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) vv[c] = (qL[2 + c] + xi * (qR[2 + c] - qL[2 + c])) / r;
+    double dq[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) dq[v] = qR[v] - qL[v];
+    bdot_sub(vv, dq, d, -K.w[i], b);
+  }
+}
+
+template <int N, bool TH>
+__device__ __forceinline__ void riemann_accum(const double *qL, const double *qR, int d, const Mat &m,
+                                              const Basis &K, double k, double *lo, double *hi) {
+  constexpr int V = Dims<TH>::V;
+  double f[V], b[V];
+  rusanov<TH>(qL, qR, d, m, f);
+  bint<N, TH>(qL, qR, d, K, b);
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    lo[v] += k * (b[v] + f[v]);
+    hi[v] += k * (b[v] - f[v]);
+  }
+}
+
+struct FvArgs {
+  double *u;             // [cells][V]
+  const double *qh;      // [(nx+2)(ny+2)][NN][V]
+  const uint8_t *mask;   // [(nx+2)(ny+2)]
+  double *fx;            // x faces [(nx+1)][ny][2][V]
+  double *fy;            // y faces [nx][(ny+1)][2][V]
+  StepState *st;         // if non-null: dt from st->dt, post-step reductions into st
+  double dt, dx, dy;
+  int nx, ny;
+  int post;              // fuse renormalisation, contortion flag and CFL maximum
+  Basis K;
+  Mat m;
+};
+
+// One thread per face; interfs1/interfs2 (fv.cpp:111-160, 162-268).
+template <int N, bool TH, int ND>
+__global__ __launch_bounds__(128) void k_fv_faces(FvArgs a) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int NS = ND == 1 ? N : N * N;
+  constexpr int NN = NS * N;
+  const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
+  const long nfx = (long)(nx + 1) * ny;
+  const long nfy = ND == 2 ? (long)nx * (ny + 1) : 0;
+  const double dt = a.st ? a.st->dt : a.dt;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < nfx + nfy; id += (long)gridDim.x * blockDim.x) {
+    const int d = id < nfx ? 0 : 1;
+    long e0, e1;
+    double *out;
+    if (d == 0) {
+      int i = id / ny, j = id % ny;  // face between extended (i, j+1) and (i+1, j+1)
+      e0 = ND == 2 ? (long)i * (ny + 2) + (j + 1) : i;
+      e1 = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
+      out = a.fx + id * 2 * V;
+    } else {
+      long k = id - nfx;
+      int i = k / (ny + 1), j = k % (ny + 1);  // between extended (i+1, j) and (i+1, j+1)
+      e0 = (long)(i + 1) * (ny + 2) + j;
+      e1 = e0 + 1;
+      out = a.fy + k * 2 * V;
+    }
+    double lo[V], hi[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) { lo[v] = 0.0; hi[v] = 0.0; }
+    if (a.mask[e0] && a.mask[e1]) {
+      const double h = d == 0 ? a.dx : a.dy;
+      for (int t = 0; t < N; t++) {
+        const double kt = dt * a.K.w[t] / (2.0 * h);
+        const int nsf = ND == 2 ? N : 1;
+        for (int sf = 0; sf < nsf; sf++) {
+          double qL[V], qR[V];
+          const double *pl = a.qh + (e0 * NN + t * NS) * V;
+          const double *pr = a.qh + (e1 * NN + t * NS) * V;
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            double l = 0, r = 0;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+              int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
+              l += a.K.endv[1][k] * pl[nl * V + v];
+              r += a.K.endv[0][k] * pr[nl * V + v];
+            }
+            qL[v] = l;
+            qR[v] = r;
+          }
+          const double kk = ND == 2 ? kt * a.K.w[sf] : kt;
+          riemann_accum<N, TH>(qL, qR, d, a.m, a.K, kk, lo, hi);
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      out[v] = lo[v];
+      out[V + v] = hi[v];
+    }
+  }
+}
+
+// One thread per real cell: centres (fv.cpp:11-109), minus the face terms,
+// then the optional post-step work.
+template <int N, bool TH, int ND>
+__global__ __launch_bounds__(128) void k_fv_cells(FvArgs a) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int NS = ND == 1 ? N : N * N;
+  constexpr int NN = NS * N;
+  const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
+  const long ncell = (long)nx * ny;
+  const double dt = a.st ? a.st->dt : a.dt;
+  double local_max = 0.0;
+  int contorted = 0, nonfinite = 0;
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+    const int i = ND == 2 ? c / ny : c, j = ND == 2 ? c % ny : 0;
+    const long e = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) du[v] = 0.0;
+    if (a.mask[e]) {
+      for (int t = 0; t < N; t++) {
+        const double *qc = a.qh + (e * NN + t * NS) * V;
+        const double wt = dt * a.K.w[t];
+        for (int sn = 0; sn < NS; sn++) {
+          const int ia = ND == 1 ? sn : sn / N, ib = ND == 1 ? 0 : sn % N;
+          double q[V], S[V], dq[V];
+#pragma unroll
+          for (int v = 0; v < V; v++) q[v] = qc[sn * V + v];
+          NodeState<double, TH> st;
+          node_state<double, TH>(q, a.m, st);
+          source<double, TH>(q, st, a.m, S);
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < N; k++) acc += a.K.der[ia][k] * qc[(ND == 1 ? k : k * N + ib) * V + v];
+            dq[v] = acc / a.dx;
+          }
+          bdot_sub(st.v, dq, 0, 1.0, S);
+          if (ND == 2) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              double acc = 0;
+#pragma unroll
+              for (int k = 0; k < N; k++) acc += a.K.der[ib][k] * qc[(ia * N + k) * V + v];
+              dq[v] = acc / a.dy;
+            }
+            bdot_sub(st.v, dq, 1, 1.0, S);
+          }
+          const double ws = ND == 2 ? wt * a.K.w[ia] * a.K.w[ib] : wt * a.K.w[ia];
+#pragma unroll
+          for (int v = 0; v < V; v++) du[v] += ws * S[v];
+        }
+      }
+    }
+    // faces: this cell is "hi" of its left face and "lo" of its right face
+    {
+      const double *fl = a.fx + ((long)i * ny + j) * 2 * V;        // face i: hi side
+      const double *fr = a.fx + ((long)(i + 1) * ny + j) * 2 * V;  // face i+1: lo side
+#pragma unroll
+      for (int v = 0; v < V; v++) du[v] -= fl[V + v] + fr[v];
+      if (ND == 2) {
+        const double *fb = a.fy + ((long)i * (ny + 1) + j) * 2 * V;
+        const double *ft = a.fy + ((long)i * (ny + 1) + j + 1) * 2 * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) du[v] -= fb[V + v] + ft[v];
+      }
+    }
+    double q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = a.u[c * V + v] + du[v];
+    if (a.post) {
+      // renorm_distortion (utils.cpp:6-23)
+      if (q[0] > 0) {
+        double cf = cbrt(q[0] / (a.m.p.rho0 * det3(q + 5)));
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[5 + k] *= cf;
+      }
+      // contorted (utils.cpp:65-85): mean squared singular value over |det A|^(2/3)
+      double f = 0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) f += q[5 + k] * q[5 + k];
+      double d13 = cbrt(fabs(det3(q + 5)));
+      if (f / (d13 * d13) / 3.0 - 1.0 > a.st->contorted_tol) contorted = 1;
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        if (!(q[v] - q[v] == 0.0)) nonfinite = 1;
+      double sp = 0;
+      for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, a.m, d) / (d == 0 ? a.dx : a.dy);
+      if (sp > local_max) local_max = sp;
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) a.u[c * V + v] = q[v];
+  }
+  if (a.post) {
+    double m = wave_max(local_max);
+    int cf = __any(contorted), nf = __any(nonfinite);
+    if ((threadIdx.x & 63) == 0) {
+      atomic_max_pos(&a.st->maxspeed, m);
+      if (cf) atomicOr(&a.st->contorted, 1);
+      if (nf) atomicOr(&a.st->nonfinite, 1);
+    }
+  }
+}
+
+// CFL maximum of u (timestep, iterator.cpp:58-72) into *dst.
+template <bool TH, int ND>
+__global__ __launch_bounds__(256) void k_cfl(const double *u, long ncell, double dx, double dy, Mat m,
+                                            unsigned long long *dst) {
+  constexpr int V = Dims<TH>::V;
+  double local_max = 0.0;
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+    double q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = u[c * V + v];
+    double sp = 0;
+    for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
+    if (sp > local_max) local_max = sp;
+  }
+  double mx = wave_max(local_max);
+  if ((threadIdx.x & 63) == 0) atomic_max_pos(dst, mx);
+}
+
+// reset_distortion on every cell when the contortion flag is set (utils.cpp:44-63,
+// iterator.cpp:150-151); the CFL maximum is then recomputed into maxspeed2.
+template <bool TH, int ND>
+__global__ __launch_bounds__(256) void k_reset(double *u, long ncell, double dx, double dy, Mat m, StepState *st) {
+  constexpr int V = Dims<TH>::V;
+  if (!st->contorted) return;
+  double local_max = 0.0;
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+    double q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = u[c * V + v];
+    double r = q[0];
+    double cf = cbrt(r / m.p.rho0);
+    NodeState<double, TH> s;
+    node_state<double, TH>(q, m, s);
+    double Gm = Gamma_MG(r, m.p);
+    double C0 = (1 - m.p.beta / Gm) * s.cs2;
+    q[1] -= r * (C0 / 4 * s.devGsq);
+#pragma unroll
+    for (int k = 0; k < 9; k++) q[5 + k] = 0.0;
+    q[5] = cf;
+    q[9] = cf;
+    q[13] = cf;
+#pragma unroll
+    for (int v = 0; v < V; v++) u[c * V + v] = q[v];
+    double sp = 0;
+    for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
+    if (sp > local_max) local_max = sp;
+  }
+  double mx = wave_max(local_max);
+  if ((threadIdx.x & 63) == 0) atomic_max_pos(&st->maxspeed2, mx);
+}
+
+// dt for the coming step from the CFL maximum (timestep, iterator.cpp:74-82).
+// first: the maximum was computed by k_cfl into maxspeed; otherwise by the
+// previous step's post work (maxspeed, or maxspeed2 after a reset).
+__global__ void k_finalize_dt(StepState *st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long bits = st->contorted ? st->maxspeed2 : st->maxspeed;
+  double MAX = __longlong_as_double((long long)bits);
+  double dt = st->cfl / MAX;
+  if (st->count <= 5) dt *= 0.2;
+  if (st->t + dt > st->tf) dt = st->tf - st->t;
+  st->dt = dt;
+  st->maxspeed = 0ull;
+  st->maxspeed2 = 0ull;
+  st->contorted = 0;
+  st->done = st->t >= st->tf ? 1 : 0;
+}
+
+// t += dt, count += 1 (iterator.cpp:142-143)
+__global__ void k_advance(StepState *st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->t += st->dt;
+  st->count += 1;
+}
+
+}  // namespace gpr

@@ -1,0 +1,1073 @@
+// libgpr_mi355x.so: host side of the ADER-WENO GPR step on gfx950 and its C ABI
+// (include/gpr_mi355x.h).  Kernels are in gpr_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gpr_mi355x.h"
+#include "gpr_kernels.hpp"
+
+using namespace gpr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(GPR_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// ------------------------------------------------------------ basis constants
+// Lagrange basis on the Gauss-Legendre nodes of [0, 1] (basis.cpp:7-72), the
+// WENO stencil matrices (weno_matrices.cpp:9-29), the oscillation indicator
+// (weno_matrices.cpp:31-42) and the DG time matrices (dg_matrices.cpp:30-53).
+// Polynomials are monomial coefficient arrays; every integral is of degree
+// <= 2N - 1 and is taken exactly with the same N-point Gauss rule.
+using Poly = std::array<double, 8>;
+
+double peval(const Poly &p, double x) {
+  double r = 0;
+  for (int i = 7; i >= 0; i--) r = r * x + p[i];
+  return r;
+}
+Poly pderiv(const Poly &p) {
+  Poly r{};
+  for (int i = 1; i < 8; i++) r[i - 1] = i * p[i];
+  return r;
+}
+
+int invert(int n, const double *a, double *out) {
+  double m[4][8];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < 2 * n; j++) m[i][j] = j < n ? a[i * n + j] : (j - n == i ? 1.0 : 0.0);
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+    if (m[piv][c] == 0.0) return -1;
+    for (int j = 0; j < 2 * n; j++) std::swap(m[c][j], m[piv][j]);
+    double d = m[c][c];
+    for (int j = 0; j < 2 * n; j++) m[c][j] /= d;
+    for (int r = 0; r < n; r++)
+      if (r != c) {
+        double f = m[r][c];
+        for (int j = 0; j < 2 * n; j++) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) out[i * n + j] = m[i][n + j];
+  return 0;
+}
+
+Basis make_basis(int N) {
+  Basis K;
+  std::memset(&K, 0, sizeof K);
+  K.N = N;
+  if (N == 2) {
+    double a = 0.5 / std::sqrt(3.0);
+    K.x[0] = 0.5 - a;
+    K.x[1] = 0.5 + a;
+    K.w[0] = K.w[1] = 0.5;
+  } else {
+    double a = 0.5 * std::sqrt(0.6);
+    K.x[0] = 0.5 - a;
+    K.x[1] = 0.5;
+    K.x[2] = 0.5 + a;
+    K.w[0] = K.w[2] = 5.0 / 18.0;
+    K.w[1] = 4.0 / 9.0;
+  }
+  Poly psi[MAXN];
+  for (int j = 0; j < N; j++) {
+    Poly p{};
+    p[0] = 1.0;
+    for (int m = 0; m < N; m++) {
+      if (m == j) continue;
+      double den = K.x[j] - K.x[m];
+      Poly q{};
+      for (int i = 0; i < 7; i++) {
+        q[i + 1] += p[i] / den;
+        q[i] -= p[i] * K.x[m] / den;
+      }
+      p = q;
+    }
+    psi[j] = p;
+  }
+  auto integ = [&](const Poly &p, double lo) {  // integral over [lo, lo + 1]
+    double s = 0;
+    for (int q = 0; q < N; q++) s += K.w[q] * peval(p, lo + K.x[q]);
+    return s;
+  };
+  const int CN2 = N / 2;  // ceil((N - 1) / 2)
+  const int FN2 = (N - 1) / 2;
+  double mL[9], mR[9], mCL[9];
+  for (int j = 0; j < N; j++) {
+    Poly d1 = pderiv(psi[j]);
+    K.endv[0][j] = peval(psi[j], 0.0);
+    K.endv[1][j] = peval(psi[j], 1.0);
+    for (int i = 0; i < N; i++) {
+      K.der[i][j] = peval(d1, K.x[i]);
+      mL[i * N + j] = integ(psi[j], i - N + 1);
+      mR[i * N + j] = integ(psi[j], i);
+      mCL[i * N + j] = integ(psi[j], i - CN2);
+    }
+  }
+  (void)FN2;
+  double t[9];
+  invert(N, mL, t);
+  for (int i = 0; i < N * N; i++) K.mLinv[i / N][i % N] = t[i];
+  invert(N, mR, t);
+  for (int i = 0; i < N * N; i++) K.mRinv[i / N][i % N] = t[i];
+  invert(N, mCL, t);
+  for (int i = 0; i < N * N; i++) K.mCLinv[i / N][i % N] = t[i];
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++) {
+      Poly pi = psi[i], pj = psi[j];
+      double s = 0;
+      for (int a = 1; a < N; a++) {
+        pi = pderiv(pi);
+        pj = pderiv(pj);
+        for (int q = 0; q < N; q++) s += K.w[q] * peval(pi, K.x[q]) * peval(pj, K.x[q]);
+      }
+      K.sig[i][j] = s;
+    }
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      K.dgder[i][j] = i == j ? (K.endv[1][i] * K.endv[1][i] - K.endv[0][i] * K.endv[0][i]) / 2
+                             : K.w[i] * K.der[i][j];
+  double dm[9];
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++) {
+      K.dgmat[i][j] = K.endv[1][i] * K.endv[1][j] - K.dgder[j][i];
+      dm[i * N + j] = K.dgmat[i][j];
+    }
+  invert(N, dm, t);
+  for (int i = 0; i < N * N; i++) K.dgmatinv[i / N][i % N] = t[i];
+  return K;
+}
+
+const Basis &basis(int N) {
+  static const Basis b2 = make_basis(2), b3 = make_basis(3);
+  return N == 2 ? b2 : b3;
+}
+
+int make_mat(const gpr_par *p, Mat &m) {
+  if (!p) return fail(GPR_E_ARG, "null material parameters");
+  if (p->EOS < 0 || p->EOS > 4) return fail(GPR_E_EOS, "EOS not recognized");
+  if (p->REACTION > -1) return fail(GPR_E_UNSUPPORTED, "reactive materials are not on this path");
+  std::memset(&m, 0, sizeof m);
+  m.p = *p;
+  m.mu_fac = p->rho0 / (2.0 * std::pow(p->mu, 1.0 / p->n));
+  m.beta_zero = p->beta == 0.0;
+  return GPR_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- context
+struct gpr_ctx {
+  int N, ndim, thermal, V;
+  gpr_solver_opts opts;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::map<std::string, std::pair<void *, size_t>> bufs;
+  StepState *d_st = nullptr;
+  int *d_fail = nullptr;
+  // device-resident run
+  double *run_u = nullptr;
+  int run_nx = 0, run_ny = 1;
+  double run_dx = 1, run_dy = 1;
+  int4 run_bc{0, 0, 0, 0};
+  Mat run_mat;
+  bool run_ready = false;
+};
+
+namespace {
+
+void *buf(gpr_ctx *c, const char *name, size_t bytes, int &rc) {
+  auto &e = c->bufs[name];
+  if (e.second >= bytes && e.first) return e.first;
+  if (e.first) (void)hipFree(e.first);
+  e.first = nullptr;
+  e.second = 0;
+  void *p = nullptr;
+  hipError_t err = hipMalloc(&p, std::max<size_t>(bytes, 256));
+  if (err != hipSuccess) {
+    rc = fail(GPR_E_HIP, std::string("hipMalloc(") + name + "): " + hipGetErrorString(err));
+    return nullptr;
+  }
+  e.first = p;
+  e.second = bytes;
+  rc = GPR_OK;
+  return p;
+}
+
+int grid_for(long total, int block) {
+  long g = (total + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > (1L << 20)) g = 1L << 20;
+  return (int)g;
+}
+
+long ext_cells(const gpr_ctx *c, const int32_t *nX, int ext) {
+  long r = nX[0] + 2 * ext;
+  if (c->ndim == 2) r *= nX[1] + 2 * ext;
+  return r;
+}
+
+int check_nX(const gpr_ctx *c, const int32_t *nX) {
+  if (!nX) return fail(GPR_E_ARG, "null nX");
+  for (int d = 0; d < c->ndim; d++)
+    if (nX[d] < 1) return fail(GPR_E_ARG, "grid dimensions must be positive");
+  if (c->ndim == 2 && (nX[0] < 1 || nX[1] < 1)) return fail(GPR_E_ARG, "bad grid");
+  return GPR_OK;
+}
+
+// ---------------------------------------------------------- dispatch helpers
+template <template <int, bool, int> class Op, typename... A> int dispatch(const gpr_ctx *c, A &&...args) {
+  int N = c->N;
+  bool th = c->thermal;
+  int nd = c->ndim;
+#define GPR_CASE(n, t, d) \
+  if (N == n && th == t && nd == d) return Op<n, t, d>::run(args...);
+  GPR_CASE(3, false, 2) GPR_CASE(3, true, 2) GPR_CASE(3, false, 1) GPR_CASE(3, true, 1)
+  GPR_CASE(2, false, 2) GPR_CASE(2, true, 2) GPR_CASE(2, false, 1) GPR_CASE(2, true, 1)
+#undef GPR_CASE
+  return fail(GPR_E_UNSUPPORTED, "unsupported (N, thermal, ndim)");
+}
+
+// ------------------------------------------------------------- launches
+template <int N, bool TH, int ND> struct BoundariesOp {
+  static int run(gpr_ctx *c, const double *d_u, double *d_ub, int nx, int ny, int4 bc) {
+    constexpr int V = Dims<TH>::V;
+    long total = (long)(nx + 2 * N) * (ND == 2 ? ny + 2 * N : 1) * V;
+    hipLaunchKernelGGL((k_boundaries<V, ND>), dim3(grid_for(total, 256)), dim3(256), 0, c->stream, d_u, d_ub, nx,
+                       ny, N, bc);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+template <int N, bool TH, int ND> struct WenoOp {
+  static int run(gpr_ctx *c, const double *d_ub, double *d_wh, int nx, int ny) {
+    constexpr int V = Dims<TH>::V;
+    const Basis &K = basis(N);
+    if (ND == 1) {
+      long total = (long)(nx + 2) * V;
+      hipLaunchKernelGGL((k_weno_1d<double, N, V>), dim3(grid_for(total, 256)), dim3(256), 0, c->stream, d_ub,
+                         d_wh, nx, K);
+    } else {
+      int rc;
+      long nux = (long)(nx + 2) * (ny + 2 * N) * N * V;
+      double *ux = (double *)buf(c, "ux", nux * sizeof(double), rc);
+      if (!ux) return rc;
+      long t1 = (long)(nx + 2) * (ny + 2 * N) * V;
+      hipLaunchKernelGGL((k_weno_x<double, N, V>), dim3(grid_for(t1, 256)), dim3(256), 0, c->stream, d_ub, ux, nx,
+                         ny, K);
+      long t2 = (long)(nx + 2) * (ny + 2) * N * V;
+      hipLaunchKernelGGL((k_weno_y<double, N, V>), dim3(grid_for(t2, 256)), dim3(256), 0, c->stream, ux, d_wh,
+                         nx, ny, K);
+    }
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+template <int N, bool TH, int ND> struct PredictorOp {
+  static int run(gpr_ctx *c, PredArgs a) {
+    constexpr int CPW = PredShape<N, ND>::CPW;
+    a.K = basis(N);
+    long blocks = (a.ncell + CPW - 1) / CPW;
+    hipLaunchKernelGGL((k_predictor<N, TH, ND>), dim3(blocks), dim3(WAVE), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+template <int N, bool TH, int ND> struct FvOp {
+  static int run(gpr_ctx *c, FvArgs a) {
+    a.K = basis(N);
+    long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
+    hipLaunchKernelGGL((k_fv_faces<N, TH, ND>), dim3(grid_for(nf, 128)), dim3(128), 0, c->stream, a);
+    long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
+    hipLaunchKernelGGL((k_fv_cells<N, TH, ND>), dim3(grid_for(nc, 128)), dim3(128), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+template <int N, bool TH, int ND> struct CflOp {
+  static int run(gpr_ctx *c, const double *d_u, long ncell, double dx, double dy, const Mat &m,
+                 unsigned long long *dst) {
+    hipLaunchKernelGGL((k_cfl<TH, ND>), dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, d_u, ncell, dx, dy, m,
+                       dst);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+template <int N, bool TH, int ND> struct ResetOp {
+  static int run(gpr_ctx *c, double *d_u, long ncell, double dx, double dy, const Mat &m, StepState *st) {
+    hipLaunchKernelGGL((k_reset<TH, ND>), dim3(grid_for(ncell, 256)), dim3(256), 0, c->stream, d_u, ncell, dx, dy,
+                       m, st);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+// pointwise operations over a batch of states
+enum PointOp { OP_FLUX, OP_SOURCE, OP_BDOT, OP_EIGS, OP_TAT, OP_DRUS, OP_BINT, OP_RENORM, OP_RESET, OP_CONTORTED };
+
+template <int N, bool TH>
+__global__ void k_pointwise(int op, double *ret, const double *A, const double *B, long count, int d, Mat m,
+                            Basis K, double tol, int *flag) {
+  constexpr int V = Dims<TH>::V;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+    double q[V];
+    for (int v = 0; v < V; v++) q[v] = A[i * V + v];
+    switch (op) {
+    case OP_FLUX: {
+      NodeState<double, TH> s;
+      node_state<double, TH>(q, m, s);
+      double F[V];
+      flux<double, TH>(q, s, m, d, F);
+      for (int v = 0; v < V; v++) ret[i * V + v] = F[v];
+    } break;
+    case OP_SOURCE: {
+      NodeState<double, TH> s;
+      node_state<double, TH>(q, m, s);
+      double S[V];
+      source<double, TH>(q, s, m, S);
+      for (int v = 0; v < V; v++) ret[i * V + v] = S[v];
+    } break;
+    case OP_BDOT: {
+      double vv[3] = {q[2] / q[0], q[3] / q[0], q[4] / q[0]};
+      double r[V];
+      bdot<double, V>(vv, B + i * V, d, r);
+      for (int v = 0; v < V; v++) ret[i * V + v] = r[v];
+    } break;
+    case OP_EIGS: ret[i] = max_abs_eigs<TH>(q, m, d); break;
+    case OP_TAT: {
+      double X[16];
+      int n = thermo_acoustic_tensor<TH>(q, m, d, X);
+      for (int k = 0; k < n * n; k++) ret[i * n * n + k] = X[k];
+    } break;
+    case OP_DRUS:
+    case OP_BINT: {
+      double qR[V], r[V];
+      for (int v = 0; v < V; v++) qR[v] = B[i * V + v];
+      if (op == OP_DRUS)
+        rusanov<TH>(q, qR, d, m, r);
+      else
+        bint<N, TH>(q, qR, d, K, r);
+      for (int v = 0; v < V; v++) ret[i * V + v] = r[v];
+    } break;
+    case OP_RENORM: {
+      if (q[0] > 0) {
+        double cf = cbrt(q[0] / (m.p.rho0 * det3(q + 5)));
+        for (int k = 0; k < 9; k++) ret[i * V + 5 + k] = q[5 + k] * cf;
+      }
+    } break;
+    case OP_RESET: {
+      NodeState<double, TH> s;
+      node_state<double, TH>(q, m, s);
+      double r = q[0], cf = cbrt(r / m.p.rho0);
+      double Gm = Gamma_MG(r, m.p);
+      double C0 = (1 - m.p.beta / Gm) * s.cs2;
+      ret[i * V + 1] = q[1] - r * (C0 / 4 * s.devGsq);
+      for (int k = 0; k < 9; k++) ret[i * V + 5 + k] = 0.0;
+      ret[i * V + 5] = cf;
+      ret[i * V + 9] = cf;
+      ret[i * V + 13] = cf;
+    } break;
+    case OP_CONTORTED: {
+      double f = 0;
+      for (int k = 0; k < 9; k++) f += q[5 + k] * q[5 + k];
+      double d13 = cbrt(fabs(det3(q + 5)));
+      if (f / (d13 * d13) / 3.0 - 1.0 > tol) atomicOr(flag, 1);
+    } break;
+    }
+  }
+}
+
+int pointwise(gpr_ctx *c, int op, double *ret, const double *A, const double *B, int64_t count, int d,
+              const gpr_par *MP, size_t ret_per, double tol = 0, int32_t *flag_out = nullptr) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (count < 0 || !A) return fail(GPR_E_ARG, "bad arguments");
+  if (d < 0 || d > 2) return fail(GPR_E_ARG, "direction out of range");
+  Mat m;
+  std::memset(&m, 0, sizeof m);
+  if (MP) {
+    int rc = make_mat(MP, m);
+    if (rc) return rc;
+  }
+  if (count == 0) return GPR_OK;
+  int rc;
+  size_t V = c->V;
+  double *dA = (double *)buf(c, "pA", count * V * 8, rc);
+  if (!dA) return rc;
+  double *dB = nullptr;
+  if (B) {
+    dB = (double *)buf(c, "pB", count * V * 8, rc);
+    if (!dB) return rc;
+    HIP_TRY(hipMemcpyAsync(dB, B, count * V * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  double *dR = (double *)buf(c, "pR", count * std::max<size_t>(ret_per, 1) * 8, rc);
+  if (!dR) return rc;
+  HIP_TRY(hipMemcpyAsync(dA, A, count * V * 8, hipMemcpyHostToDevice, c->stream));
+  if (ret_per) HIP_TRY(hipMemcpyAsync(dR, ret, count * ret_per * 8, hipMemcpyHostToDevice, c->stream));
+  int *dflag = c->d_fail;
+  HIP_TRY(hipMemsetAsync(dflag, 0, sizeof(int), c->stream));
+  const Basis &K = basis(c->N);
+  int g = grid_for(count, 128);
+#define GPR_PW(n, t)                                                                                          \
+  if (c->N == n && (bool)c->thermal == t)                                                                   \
+    hipLaunchKernelGGL((k_pointwise<n, t>), dim3(g), dim3(128), 0, c->stream, op, dR, dA, dB, (long)count, d, m, \
+                       K, tol, dflag);
+  GPR_PW(3, false) GPR_PW(3, true) GPR_PW(2, false) GPR_PW(2, true)
+#undef GPR_PW
+  HIP_TRY(hipGetLastError());
+  if (ret_per) HIP_TRY(hipMemcpyAsync(ret, dR, count * ret_per * 8, hipMemcpyDeviceToHost, c->stream));
+  if (flag_out) HIP_TRY(hipMemcpyAsync(flag_out, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+void host_extend_mask(const gpr_ctx *c, const uint8_t *mask, uint8_t *maskb, const int32_t *nX) {
+  // extend_mask (grid.cpp:176-230); the 2D border assignments are loop-invariant
+  int nx = nX[0];
+  if (c->ndim == 1) {
+    for (int i = 0; i < nx; i++)
+      maskb[i + 1] = mask[i] || (i > 0 && mask[i - 1]) || (i < nx - 1 && mask[i + 1]);
+    maskb[0] = maskb[1];
+    maskb[nx + 1] = maskb[nx];
+    return;
+  }
+  int ny = nX[1];
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      long ind = (long)i * ny + j;
+      maskb[(long)(i + 1) * (ny + 2) + (j + 1)] = mask[ind] || (i > 0 && mask[ind - ny]) ||
+                                                  (i < nx - 1 && mask[ind + ny]) || (j > 0 && mask[ind - 1]) ||
+                                                  (j < ny - 1 && mask[ind + 1]);
+    }
+  for (int i = 0; i < nx; i++) {
+    maskb[(long)(i + 1) * (ny + 2)] = mask[(long)i * ny];
+    maskb[(long)(i + 1) * (ny + 2) + (ny + 1)] = mask[(long)i * ny + (ny - 1)];
+  }
+  for (int j = 0; j < ny; j++) {
+    maskb[j + 1] = mask[j];
+    maskb[(long)(nx + 1) * (ny + 2) + (j + 1)] = mask[(long)(nx - 1) * ny + j];
+  }
+  maskb[0] = 0;
+  maskb[ny + 1] = 0;
+  maskb[(long)(nx + 1) * (ny + 2)] = 0;
+  maskb[(long)(nx + 1) * (ny + 2) + (ny + 1)] = 0;
+}
+
+int check_bcs(const gpr_ctx *c, const int32_t *bcs, int4 &bc) {
+  if (!bcs) return fail(GPR_E_ARG, "null boundary codes");
+  int v[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 2 * c->ndim; i++) {
+    if (bcs[i] < 0 || bcs[i] > 5 || bcs[i] == 4)
+      return fail(GPR_E_UNSUPPORTED, "boundary code must be 0, 1, 2, 3 or 5");
+    v[i] = bcs[i];
+  }
+  bc = make_int4(v[0], v[1], v[2], v[3]);
+  return GPR_OK;
+}
+
+PredArgs pred_args(gpr_ctx *c, const Mat &m, double dt, const double *dX) {
+  PredArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.dt = dt;
+  a.dx = dX[0];
+  a.dy = c->ndim == 2 ? dX[1] : 1.0;
+  a.dg_it = c->opts.dg_it;
+  a.dg_tol = c->opts.dg_tol;
+  a.stiff_it = c->opts.stiff_it;
+  a.stiff_tol = c->opts.stiff_tol;
+  a.bailout = c->opts.bailout;
+  a.mode = c->opts.mode;
+  a.m = m;
+  a.fail = c->d_fail;
+  return a;
+}
+
+FvArgs fv_args(gpr_ctx *c, const Mat &m, double dt, const double *dX, const int32_t *nX) {
+  FvArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.dt = dt;
+  a.dx = dX[0];
+  a.dy = c->ndim == 2 ? dX[1] : 1.0;
+  a.nx = nX[0];
+  a.ny = c->ndim == 2 ? nX[1] : 1;
+  a.m = m;
+  return a;
+}
+
+int fv_buffers(gpr_ctx *c, FvArgs &a) {
+  int rc;
+  long nfx = (long)(a.nx + 1) * a.ny, nfy = c->ndim == 2 ? (long)a.nx * (a.ny + 1) : 1;
+  a.fx = (double *)buf(c, "fx", nfx * 2 * c->V * 8, rc);
+  if (!a.fx) return rc;
+  a.fy = (double *)buf(c, "fy", nfy * 2 * c->V * 8, rc);
+  if (!a.fy) return rc;
+  return GPR_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *gpr_last_error(void) { return g_err.c_str(); }
+int gpr_version(void) { return 1; }
+
+int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
+  if (!ctx) return fail(GPR_E_ARG, "null ctx pointer");
+  *ctx = nullptr;
+  if (N != 2 && N != 3) return fail(GPR_E_UNSUPPORTED, "N must be 2 or 3");
+  if (ndim != 1 && ndim != 2) return fail(GPR_E_UNSUPPORTED, "ndim must be 1 or 2");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(GPR_E_HIP, "no HIP device available");
+  gpr_ctx *c = new gpr_ctx();
+  c->N = N;
+  c->ndim = ndim;
+  c->thermal = thermal ? 1 : 0;
+  c->V = thermal ? 17 : 14;
+  c->opts.dg_it = 50;
+  c->opts.dg_tol = 6e-6;
+  c->opts.stiff_it = 200;
+  c->opts.stiff_tol = 6e-6;
+  c->opts.bailout = 0;
+  c->opts.mode = 0;
+  c->opts.contorted_tol = 1.0;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_st, sizeof(StepState));
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_fail, sizeof(int) * 4);
+  if (e != hipSuccess) {
+    int rc = fail(GPR_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    gpr_ctx_destroy(c);
+    return rc;
+  }
+  *ctx = c;
+  return GPR_OK;
+}
+
+int gpr_ctx_destroy(gpr_ctx *c) {
+  if (!c) return GPR_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &kv : c->bufs)
+    if (kv.second.first) (void)hipFree(kv.second.first);
+  if (c->d_st) (void)hipFree(c->d_st);
+  if (c->d_fail) (void)hipFree(c->d_fail);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GPR_OK;
+}
+
+int gpr_ctx_set_opts(gpr_ctx *c, const gpr_solver_opts *o) {
+  if (!c || !o) return fail(GPR_E_ARG, "null argument");
+  if (o->dg_it < 1 || o->stiff_it < 1 || !(o->dg_tol > 0) || !(o->stiff_tol > 0) || o->bailout < 0 ||
+      o->mode < 0 || o->mode > 1)
+    return fail(GPR_E_ARG, "invalid solver options");
+  c->opts = *o;
+  return GPR_OK;
+}
+
+int gpr_ctx_get_opts(const gpr_ctx *c, gpr_solver_opts *o) {
+  if (!c || !o) return fail(GPR_E_ARG, "null argument");
+  *o = c->opts;
+  return GPR_OK;
+}
+
+void *gpr_ctx_stream(gpr_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int gpr_options(const gpr_ctx *c, int32_t *N, int32_t *V, int32_t *ndim, int32_t *thermal) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (N) *N = c->N;
+  if (V) *V = c->V;
+  if (ndim) *ndim = c->ndim;
+  if (thermal) *thermal = c->thermal;
+  return GPR_OK;
+}
+
+int gpr_constants(int N, double *nodes, double *wghts, double *endvals, double *dervals, double *mLinv,
+                  double *mRinv, double *mCLinv, double *sig, double *dg_end, double *dg_der, double *dg_mat) {
+  if (N != 2 && N != 3) return fail(GPR_E_UNSUPPORTED, "N must be 2 or 3");
+  const Basis &K = basis(N);
+  for (int i = 0; i < N; i++) {
+    if (nodes) nodes[i] = K.x[i];
+    if (wghts) wghts[i] = K.w[i];
+    if (endvals) {
+      endvals[i] = K.endv[0][i];
+      endvals[N + i] = K.endv[1][i];
+    }
+    for (int j = 0; j < N; j++) {
+      if (dervals) dervals[i * N + j] = K.der[i][j];
+      if (mLinv) mLinv[i * N + j] = K.mLinv[i][j];
+      if (mRinv) mRinv[i * N + j] = K.mRinv[i][j];
+      if (mCLinv) mCLinv[i * N + j] = K.mCLinv[i][j];
+      if (sig) sig[i * N + j] = K.sig[i][j];
+      if (dg_end) dg_end[i * N + j] = K.endv[1][i] * K.endv[1][j];
+      if (dg_der) dg_der[i * N + j] = K.dgder[i][j];
+      if (dg_mat) dg_mat[i * N + j] = K.dgmat[i][j];
+    }
+  }
+  return GPR_OK;
+}
+
+int gpr_flux(gpr_ctx *c, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_FLUX, ret, Q, nullptr, count, d, MP, c->V);
+}
+int gpr_source(gpr_ctx *c, double *ret, const double *Q, int64_t count, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_SOURCE, ret, Q, nullptr, count, 0, MP, c->V);
+}
+int gpr_bdot(gpr_ctx *c, double *ret, const double *Q, const double *x, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (!x) return fail(GPR_E_ARG, "null x");
+  return pointwise(c, OP_BDOT, ret, Q, x, count, d, MP, c->V);
+}
+int gpr_max_abs_eigs(gpr_ctx *c, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP) {
+  return pointwise(c, OP_EIGS, ret, Q, nullptr, count, d, MP, 1);
+}
+int gpr_thermo_acoustic_tensor(gpr_ctx *c, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  int n = c->thermal ? 4 : 3;
+  return pointwise(c, OP_TAT, ret, Q, nullptr, count, d, MP, n * n);
+}
+int gpr_d_rus(gpr_ctx *c, double *ret, const double *qL, const double *qR, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (!qR) return fail(GPR_E_ARG, "null qR");
+  return pointwise(c, OP_DRUS, ret, qL, qR, count, d, MP, c->V);
+}
+int gpr_bint(gpr_ctx *c, double *ret, const double *qL, const double *qR, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (!qR) return fail(GPR_E_ARG, "null qR");
+  return pointwise(c, OP_BINT, ret, qL, qR, count, d, MP, c->V);
+}
+int gpr_renorm_distortion(gpr_ctx *c, double *u, int64_t ncell, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_RENORM, u, u, nullptr, ncell, 0, MP, c->V);
+}
+int gpr_reset_distortion(gpr_ctx *c, double *u, int64_t ncell, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_RESET, u, u, nullptr, ncell, 0, MP, c->V);
+}
+int gpr_contorted(gpr_ctx *c, int32_t *result, const double *u, int64_t ncell, double tol) {
+  if (!c || !result) return fail(GPR_E_ARG, "null argument");
+  *result = 0;
+  return pointwise(c, OP_CONTORTED, nullptr, u, nullptr, ncell, 0, nullptr, 0, tol, result);
+}
+
+int gpr_boundaries(gpr_ctx *c, const double *u, double *ub, const int32_t *nX, const int32_t *bcs) {
+  if (!c || !u || !ub) return fail(GPR_E_ARG, "null argument");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  int4 bc;
+  rc = check_bcs(c, bcs, bc);
+  if (rc) return rc;
+  long nc = ext_cells(c, nX, 0), nb = ext_cells(c, nX, c->N);
+  double *du = (double *)buf(c, "u", nc * c->V * 8, rc);
+  if (!du) return rc;
+  double *dub = (double *)buf(c, "ub", nb * c->V * 8, rc);
+  if (!dub) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, nc * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  rc = dispatch<BoundariesOp>(c, c, (const double *)du, dub, nX[0], c->ndim == 2 ? nX[1] : 1, bc);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ub, dub, nb * c->V * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+int gpr_extend_mask(gpr_ctx *c, const uint8_t *mask, uint8_t *maskb, const int32_t *nX) {
+  if (!c || !mask || !maskb) return fail(GPR_E_ARG, "null argument");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  host_extend_mask(c, mask, maskb, nX);
+  return GPR_OK;
+}
+
+int gpr_weno_launcher(gpr_ctx *c, double *wh, const double *ub, const int32_t *nX, const gpr_par *MP) {
+  if (!c || !wh || !ub) return fail(GPR_E_ARG, "null argument");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  (void)MP;  // PRIM_RECONSTRUCT = false (options.h:28): the material does not enter
+  long nb = ext_cells(c, nX, c->N), ne = ext_cells(c, nX, 1);
+  long nw = ne * (c->ndim == 2 ? c->N * c->N : c->N) * c->V;
+  double *dub = (double *)buf(c, "ub", nb * c->V * 8, rc);
+  if (!dub) return rc;
+  double *dwh = (double *)buf(c, "wh", nw * 8, rc);
+  if (!dwh) return rc;
+  HIP_TRY(hipMemcpyAsync(dub, ub, nb * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, nX[0], c->ndim == 2 ? nX[1] : 1);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(wh, dwh, nw * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+int gpr_predictor(gpr_ctx *c, double *qh, const double *wh, double dt, const double *dX, int64_t ncell,
+                  int stiff, int stiff_ig, const gpr_par *MP, const uint8_t *mask, int32_t *iters) {
+  if (!c || !qh || !wh || !dX || !mask) return fail(GPR_E_ARG, "null argument");
+  (void)stiff_ig;  // stiff_initial_guess is the standard guess (initial_guess.cpp:15-51)
+  if (ncell < 0) return fail(GPR_E_ARG, "negative cell count");
+  Mat m;
+  int rc = make_mat(MP, m);
+  if (rc) return rc;
+  if (ncell == 0) return GPR_OK;
+  long ne = ncell;
+  int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  long nw = ne * NS * c->V, nq = ne * NS * c->N * c->V;
+  double *dwh = (double *)buf(c, "wh", nw * 8, rc);
+  if (!dwh) return rc;
+  double *dqh = (double *)buf(c, "qh", nq * 8, rc);
+  if (!dqh) return rc;
+  uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
+  if (!dm) return rc;
+  int32_t *dit = nullptr;
+  if (iters) {
+    dit = (int32_t *)buf(c, "iters", ne * 4, rc);
+    if (!dit) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(dwh, wh, nw * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dqh, qh, nq * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dm, mask, ne, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  PredArgs a = pred_args(c, m, dt, dX);
+  if (stiff) a.mode = 1;
+  a.qh = dqh;
+  a.wh = dwh;
+  a.mask = dm;
+  a.iters = dit;
+  a.ncell = ne;
+  rc = dispatch<PredictorOp>(c, c, a);
+  if (rc) return rc;
+  int nfail = 0;
+  HIP_TRY(hipMemcpyAsync(qh, dqh, nq * 8, hipMemcpyDeviceToHost, c->stream));
+  if (iters) HIP_TRY(hipMemcpyAsync(iters, dit, ne * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&nfail, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (nfail) return fail(GPR_E_PREDICTOR, std::to_string(nfail) + " cell(s): predictor converged in neither phase");
+  return GPR_OK;
+}
+
+int gpr_fv_launcher(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, double dt, const double *dX,
+                    int sources, int time, int flux, const gpr_par *MP, const uint8_t *mask) {
+  if (!c || !u || !qh || !dX || !mask) return fail(GPR_E_ARG, "null argument");
+  if (!sources || !time) return fail(GPR_E_UNSUPPORTED, "only SOURCES = TIME = true (the ADER step) is supported");
+  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  Mat m;
+  rc = make_mat(MP, m);
+  if (rc) return rc;
+  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1);
+  int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  long nq = ne * NS * c->N * c->V;
+  double *du = (double *)buf(c, "u", nc * c->V * 8, rc);
+  if (!du) return rc;
+  double *dqh = (double *)buf(c, "qh", nq * 8, rc);
+  if (!dqh) return rc;
+  uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
+  if (!dm) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, nc * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dqh, qh, nq * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dm, mask, ne, hipMemcpyHostToDevice, c->stream));
+  FvArgs a = fv_args(c, m, dt, dX, nX);
+  a.u = du;
+  a.qh = dqh;
+  a.mask = dm;
+  rc = fv_buffers(c, a);
+  if (rc) return rc;
+  rc = dispatch<FvOp>(c, c, a);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(u, du, nc * c->V * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+int gpr_timestep(gpr_ctx *c, double *dt_out, const double *u, const int32_t *nX, const double *dX, double CFL,
+                 double t, double tf, int64_t count, const gpr_par *MP) {
+  if (!c || !dt_out || !u || !dX) return fail(GPR_E_ARG, "null argument");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  Mat m;
+  rc = make_mat(MP, m);
+  if (rc) return rc;
+  long nc = ext_cells(c, nX, 0);
+  double *du = (double *)buf(c, "u", nc * c->V * 8, rc);
+  if (!du) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, nc * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  StepState h;
+  std::memset(&h, 0, sizeof h);
+  h.t = t;
+  h.tf = tf;
+  h.cfl = CFL;
+  h.count = count;
+  HIP_TRY(hipMemcpyAsync(c->d_st, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+  rc = dispatch<CflOp>(c, c, (const double *)du, nc, dX[0], c->ndim == 2 ? dX[1] : 1.0, m, &c->d_st->maxspeed);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->stream, c->d_st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&h, c->d_st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *dt_out = h.dt;
+  return GPR_OK;
+}
+
+int gpr_ader_stepper(gpr_ctx *c, double *u, const double *ub, const int32_t *nX, double dt, const double *dX,
+                     int stiff, int flux, const gpr_par *MP, const uint8_t *mask) {
+  if (!c || !u || !ub || !dX || !mask) return fail(GPR_E_ARG, "null argument");
+  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  Mat m;
+  rc = make_mat(MP, m);
+  if (rc) return rc;
+  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1), nb = ext_cells(c, nX, c->N);
+  int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  long nw = ne * NS * c->V, nq = ne * NS * c->N * c->V;
+  double *du = (double *)buf(c, "u", nc * c->V * 8, rc);
+  if (!du) return rc;
+  double *dub = (double *)buf(c, "ub", nb * c->V * 8, rc);
+  if (!dub) return rc;
+  double *dwh = (double *)buf(c, "wh", nw * 8, rc);
+  if (!dwh) return rc;
+  double *dqh = (double *)buf(c, "qh", nq * 8, rc);
+  if (!dqh) return rc;
+  uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
+  if (!dm) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, nc * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dub, ub, nb * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dm, mask, ne, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(dqh, 0, nq * 8, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  int ny = c->ndim == 2 ? nX[1] : 1;
+  rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, nX[0], ny);
+  if (rc) return rc;
+  PredArgs pa = pred_args(c, m, dt, dX);
+  if (stiff) pa.mode = 1;
+  pa.qh = dqh;
+  pa.wh = dwh;
+  pa.mask = dm;
+  pa.ncell = ne;
+  rc = dispatch<PredictorOp>(c, c, pa);
+  if (rc) return rc;
+  FvArgs fa = fv_args(c, m, dt, dX, nX);
+  fa.u = du;
+  fa.qh = dqh;
+  fa.mask = dm;
+  rc = fv_buffers(c, fa);
+  if (rc) return rc;
+  rc = dispatch<FvOp>(c, c, fa);
+  if (rc) return rc;
+  int nfail = 0;
+  HIP_TRY(hipMemcpyAsync(u, du, nc * c->V * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&nfail, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (nfail) return fail(GPR_E_PREDICTOR, std::to_string(nfail) + " cell(s): predictor converged in neither phase");
+  return GPR_OK;
+}
+
+// ------------------------------------------------------- device-resident loop
+int gpr_dev_begin(gpr_ctx *c, double *d_u, const int32_t *nX, const double *dX, double CFL, const int32_t *bcs,
+                  double t0, double tf, int64_t count0, const gpr_par *MP) {
+  if (!c || !d_u || !dX) return fail(GPR_E_ARG, "null argument");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  int4 bc;
+  rc = check_bcs(c, bcs, bc);
+  if (rc) return rc;
+  Mat m;
+  rc = make_mat(MP, m);
+  if (rc) return rc;
+  c->run_ready = false;
+  c->run_u = d_u;
+  c->run_nx = nX[0];
+  c->run_ny = c->ndim == 2 ? nX[1] : 1;
+  c->run_dx = dX[0];
+  c->run_dy = c->ndim == 2 ? dX[1] : 1.0;
+  c->run_bc = bc;
+  c->run_mat = m;
+  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1), nb = ext_cells(c, nX, c->N);
+  int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  if (!buf(c, "ub", nb * c->V * 8, rc)) return rc;
+  if (!buf(c, "wh", ne * NS * c->V * 8, rc)) return rc;
+  if (!buf(c, "qh", ne * NS * c->N * c->V * 8, rc)) return rc;
+  uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
+  if (!dm) return rc;
+  if (c->ndim == 2 && !buf(c, "ux", (long)(nX[0] + 2) * (nX[1] + 2 * c->N) * c->N * c->V * 8, rc)) return rc;
+  FvArgs fa = fv_args(c, m, 0, dX, nX);
+  rc = fv_buffers(c, fa);
+  if (rc) return rc;
+  std::vector<uint8_t> m1(nc, 1), mb(ne, 0);
+  host_extend_mask(c, m1.data(), mb.data(), nX);
+  HIP_TRY(hipMemcpyAsync(dm, mb.data(), ne, hipMemcpyHostToDevice, c->stream));
+  StepState h;
+  std::memset(&h, 0, sizeof h);
+  h.t = t0;
+  h.tf = tf;
+  h.cfl = CFL;
+  h.count = count0;
+  h.contorted_tol = c->opts.contorted_tol;
+  HIP_TRY(hipMemcpyAsync(c->d_st, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  rc = dispatch<CflOp>(c, c, (const double *)d_u, nc, c->run_dx, c->run_dy, m, &c->d_st->maxspeed);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->stream, c->d_st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));  // mask vector leaves scope
+  c->run_ready = true;
+  return GPR_OK;
+}
+
+int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
+  if (!c || !c->run_ready) return fail(GPR_E_ARG, "gpr_dev_begin has not been called");
+  int rc;
+  int nX[2] = {c->run_nx, c->run_ny};
+  double dX[2] = {c->run_dx, c->run_dy};
+  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1);
+  double *dub = (double *)c->bufs["ub"].first;
+  double *dwh = (double *)c->bufs["wh"].first;
+  double *dqh = (double *)c->bufs["qh"].first;
+  uint8_t *dm = (uint8_t *)c->bufs["mask"].first;
+  for (int64_t k = 0; k < nsteps; k++) {
+    rc = dispatch<BoundariesOp>(c, c, (const double *)c->run_u, dub, c->run_nx, c->run_ny, c->run_bc);
+    if (rc) return rc;
+    rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
+    if (rc) return rc;
+    PredArgs pa = pred_args(c, c->run_mat, 0, dX);
+    pa.st = c->d_st;
+    pa.qh = dqh;
+    pa.wh = dwh;
+    pa.mask = dm;
+    pa.ncell = ne;
+    rc = dispatch<PredictorOp>(c, c, pa);
+    if (rc) return rc;
+    FvArgs fa = fv_args(c, c->run_mat, 0, dX, nX);
+    fa.st = c->d_st;
+    fa.u = c->run_u;
+    fa.qh = dqh;
+    fa.mask = dm;
+    fa.post = 1;
+    rc = fv_buffers(c, fa);
+    if (rc) return rc;
+    rc = dispatch<FvOp>(c, c, fa);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_st);
+    rc = dispatch<ResetOp>(c, c, c->run_u, nc, dX[0], dX[1], c->run_mat, c->d_st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->stream, c->d_st);
+    HIP_TRY(hipGetLastError());
+  }
+  return GPR_OK;
+}
+
+int gpr_dev_status(gpr_ctx *c, double *t, int64_t *count, double *dt, int32_t *pred_fail, int32_t *nonfinite) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  StepState h;
+  int nfail = 0;
+  HIP_TRY(hipMemcpyAsync(&h, c->d_st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&nfail, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (t) *t = h.t;
+  if (count) *count = h.count;
+  if (dt) *dt = h.dt;
+  if (pred_fail) *pred_fail = nfail;
+  if (nonfinite) *nonfinite = h.nonfinite;
+  return GPR_OK;
+}
+
+int gpr_iterator(gpr_ctx *c, double *u, double tf, const int32_t *nX, const double *dX, double CFL,
+                 const int32_t *bcs, int split, int half_step, int stiff, int flux, const gpr_par *MPs, int nmat,
+                 double contorted_tol, double *snapshots, int32_t *nsnap, int64_t *steps) {
+  (void)half_step;
+  if (!c || !u || !dX || !MPs) return fail(GPR_E_ARG, "null argument");
+  if (split) return fail(GPR_E_UNSUPPORTED, "SPLIT = true (the split-WENO stepper) is not on this path");
+  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  if (nmat != 1) return fail(GPR_E_UNSUPPORTED, "one material (LSET = 0) is supported");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  long nc = ext_cells(c, nX, 0);
+  size_t bytes = nc * c->V * 8;
+  double *du = (double *)buf(c, "it_u", bytes, rc);
+  if (!du) return rc;
+  double *dprev = (double *)buf(c, "it_prev", bytes, rc);
+  if (!dprev) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, bytes, hipMemcpyHostToDevice, c->stream));
+  gpr_solver_opts saved = c->opts;
+  c->opts.contorted_tol = contorted_tol;
+  if (stiff) c->opts.mode = 1;
+  rc = gpr_dev_begin(c, du, nX, dX, CFL, bcs, 0.0, tf, 0, MPs);
+  c->opts = saved;
+  if (rc) return rc;
+  if (stiff) c->opts.mode = 1;
+  int push = 0;
+  int64_t k = 0;
+  int status = GPR_OK;
+  double t = 0;
+  // one step at a time: snapshots at each 1% of tf and the NaN exit need t
+  while (t < tf) {
+    HIP_TRY(hipMemcpyAsync(dprev, du, bytes, hipMemcpyDeviceToDevice, c->stream));
+    rc = gpr_dev_steps(c, 1);
+    if (rc) {
+      c->opts = saved;
+      return rc;
+    }
+    int64_t cnt;
+    int32_t pf, nf;
+    double dtl;
+    rc = gpr_dev_status(c, &t, &cnt, &dtl, &pf, &nf);
+    if (rc) {
+      c->opts = saved;
+      return rc;
+    }
+    k++;
+    if (pf) status = fail(GPR_E_PREDICTOR, std::to_string(pf) + " cell(s): predictor converged in neither phase");
+    if (nf) {
+      if (snapshots && push + 2 <= 100) {
+        HIP_TRY(hipMemcpyAsync(snapshots + (size_t)push * nc * c->V, dprev, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(snapshots + (size_t)(push + 1) * nc * c->V, du, bytes, hipMemcpyDeviceToHost,
+                               c->stream));
+        push += 2;
+      }
+      status = fail(GPR_E_NONFINITE, "state became non-finite");
+      break;
+    }
+    if (snapshots && t >= double(push + 1) / 100.0 * tf && push < 99) {
+      HIP_TRY(hipMemcpyAsync(snapshots + (size_t)push * nc * c->V, du, bytes, hipMemcpyDeviceToHost, c->stream));
+      push++;
+    }
+  }
+  if (snapshots && status != GPR_E_NONFINITE) {
+    HIP_TRY(hipMemcpyAsync(snapshots + (size_t)99 * nc * c->V, du, bytes, hipMemcpyDeviceToHost, c->stream));
+    push = std::max(push, 100);
+  }
+  HIP_TRY(hipMemcpyAsync(u, du, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->opts = saved;
+  if (nsnap) *nsnap = push;
+  if (steps) *steps = k;
+  return status;
+}
+
+}  // extern "C"
