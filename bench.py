@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark: ADER-WENO cell-updates/s on one MI355X.
+
+Workload (default): 2D GPR viscous double shear layer, 512x512 periodic on
+[0,1]^2, P2 (3 Gauss-Legendre nodes per dimension), FP64, μ=1e-3, c_s=8,
+CFL 0.8 (phd_amd.problems.double_shear_2d).  A step is one full ADER-WENO
+step of the whole grid: boundaries, WENO, DG predictor (Picard with the
+reference stopping rule, stiff cells finished by the preconditioned
+fallback), FV corrector, distortion renormalisation/contortion check and
+the CFL reduction for the next dt, all device-resident.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  The timed region starts after W warm-up steps (the
+first 6 steps run at 0.2·dt, iterator.cpp:76-77) with u resident in HBM.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ADER-WENO cell-updates/s, 2D 512² P2: plain FP64 vs batched/FP32/adjoint"
+
+# ---------------------------------------------------------------------------
+# Algorithmic FP64 operation counts (add, sub, mul, div, sqrt, cbrt = 1 flop;
+# fused multiply-add = 2) of one DG-predictor iteration at one space-time
+# node, 2D, V=14, Newtonian ideal gas, counted from the algorithm's formulas
+# (dg2.cpp:8-71 right-hand side, the DG_U2 solve in its factorised form,
+# the stopping test dg2.cpp:280); the table and its derivation are in
+# DESIGN.md.
+FLOPS_NODE = {
+    "state (pressure, A^T A, dev G, dE/dA, stress)": 160,
+    "source (theta1^-1, S_A)": 30,
+    "spatial derivatives (2 x V x N FMA, scaling)": 196,
+    "non-conservative B_x, B_y products": 66,
+    "space-time weights": 14,
+    "fluxes F, G (+ y-weight)": 76,
+    "flux contractions, dt, initial term": 269,
+    "time solve (DG_MAT^-1 (x) W^-1)": 98,
+    "stopping test": 84,
+}
+PICARD_NODE = sum(FLOPS_NODE.values())
+# preconditioned update: residual g (112) + projector (120) + update (165)
+# replace the plain time solve (98)
+FALLBACK_NODE = PICARD_NODE - 98 + 112 + 120 + 165
+# preconditioner set-up per spatial node: state (160), theta (21), two 3x3
+# inverses (~100), polar rotation (8 Newton steps x ~60)
+FALLBACK_SETUP_SPATIAL = 760
+
+
+def predictor_flops(picard_iters, fallback_iters, fallback_cells, nodes=27, spatial=9):
+    return (nodes * (PICARD_NODE * picard_iters + FALLBACK_NODE * fallback_iters)
+            + spatial * FALLBACK_SETUP_SPATIAL * fallback_cells)
+
+
+MI355X_FP64_TFLOPS = 78.6   # dense FP64 (vector and matrix) peak
+MI355X_HBM_GBS = 8000.0
+
+
+def cpu_baseline(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads):
+    """The oracle (CPU restatement, same algorithm) on `nsteps` of the same
+    steps the GPU timed, OpenMP over `threads` cores."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from orcbind import Oracle
+    O = Oracle(3, False, 2, bailout=2)
+    u = u_start.copy()
+    t, c = t0, count0
+    tic = time.perf_counter()
+    for _ in range(nsteps):
+        u, t, c, dt, rc, st = O.step(u, nX, dX, 0.8, bcs, t, c, 1e300, MP)
+    sec = time.perf_counter() - tic
+    return u, sec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bailout", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box in this setup has one GPU and the "
+              "multi-GPU slab decomposition is design-only (DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from phd_amd import problems
+    from phd_amd.runner import DeviceRun
+
+    n = args.n
+    u0, MP, _, dX, bcs = problems.double_shear_2d(n)
+    nX = [n, n]
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout)
+    run.steps(args.warmup)
+    st0 = run.status()
+    if st0["pred_fail"] or st0["nonfinite"]:
+        raise SystemExit(f"warm-up failed: {st0}")
+    u_w = run.host()
+
+    run.set_timing(True)
+    torch.cuda.synchronize()
+    run.sync()
+    tic = time.perf_counter()
+    run.steps(args.steps)
+    run.sync()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - tic
+    tm = run.timing()
+    run.set_timing(False)
+    st1 = run.status()
+    if st1["pred_fail"] or st1["nonfinite"]:
+        raise SystemExit(f"timed steps failed: {st1}")
+
+    cells = n * n
+    value = cells * args.steps / sec
+    ms_step = 1e3 * sec / args.steps
+    K = tm["steps"]
+    pred_ms = tm["ms"]["predictor"] / K
+    flops_launch = predictor_flops(tm["picard_iters"] / K, tm["fallback_iters"] / K,
+                                   tm["fallback_cells"] / K)
+    achieved = flops_launch / (pred_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma", "pipe": "fp64 VALU (peak equal to dense FP64 MFMA on MI355X)",
+        "kernel": "k_predictor<3,false,2>", "achieved": round(achieved, 3),
+        "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved / MI355X_FP64_TFLOPS, 4), "traffic": None,
+        "flops_per_launch": flops_launch, "avg_launch_ms": pred_ms,
+        "picard_iters_per_cell": tm["picard_iters"] / max(tm["cells"], 1),
+        "fallback_iters_per_cell": tm["fallback_iters"] / max(tm["cells"], 1),
+        "fallback_cell_fraction": tm["fallback_cells"] / max(tm["cells"], 1),
+        "phase_ms_per_step": {k: v / K for k, v in tm["ms"].items()},
+    }
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D GPR double shear layer {n}x{n} periodic, P2 (N=3), FP64, "
+                               "mu=1e-3, cs=8, CFL 0.8, steps after warm-up", "cells": cells,
+                   "N": 3, "V": 14, "bailout": args.bailout, "parallelism": "single GPU"},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        _, csec = cpu_baseline(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,
+                               threads)
+        cval = cells * args.cpu_steps / csec
+        out["cpu_baseline"] = {
+            "value": cval, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same {n}x{n} run from the state after "
+                      f"the GPU warm-up (the first timed steps), oracle/liboracle.so with "
+                      f"OpenMP", "gpu_over_cpu": value / cval}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

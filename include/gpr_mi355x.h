@@ -144,6 +144,15 @@ int gpr_dev_steps(gpr_ctx *ctx, int64_t nsteps);
 /* read back t, count, the last dt and the status flags (synchronises) */
 int gpr_dev_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t *pred_fail,
                    int32_t *nonfinite);
+/* Per-phase timing of gpr_dev_steps with HIP events on the context's stream.
+ * enable = 1 clears the totals and starts recording; 0 stops. */
+int gpr_dev_set_timing(gpr_ctx *ctx, int enable);
+/* Totals since timing was enabled (synchronises): ms[5] = boundaries, WENO,
+ * predictor, FV (faces + cells with the fused post-step), reset + dt;
+ * *steps = steps timed; iters[3] = Picard iterations, fallback iterations and
+ * fallback cells summed over the predicted cells; cells = predicted cells
+ * summed over the steps. */
+int gpr_dev_timing(gpr_ctx *ctx, double *ms, int64_t *steps, int64_t *iters, int64_t *cells);
 
 #ifdef __cplusplus
 }

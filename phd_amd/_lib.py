@@ -26,7 +26,7 @@ SYMBOLS = [
     "gpr_bint", "gpr_boundaries", "gpr_extend_mask", "gpr_weno_launcher", "gpr_predictor",
     "gpr_fv_launcher", "gpr_timestep", "gpr_ader_stepper", "gpr_renorm_distortion",
     "gpr_contorted", "gpr_reset_distortion", "gpr_iterator", "gpr_dev_begin", "gpr_dev_steps",
-    "gpr_dev_status",
+    "gpr_dev_status", "gpr_dev_set_timing", "gpr_dev_timing",
 ]
 
 
@@ -83,6 +83,8 @@ ARGTYPES = {
     "gpr_dev_begin": [_P, _P, _Ip, _Dp, _d, _Ip, _d, _d, _l, _M],
     "gpr_dev_steps": [_P, _l],
     "gpr_dev_status": [_P, _Dp, _Lp, _Dp, _Ip, _Ip],
+    "gpr_dev_set_timing": [_P, _i],
+    "gpr_dev_timing": [_P, _Dp, _Lp, _Lp, _Lp],
 }
 
 _lib = None

@@ -229,6 +229,7 @@ struct PredArgs {
   const uint8_t *mask;     // [ncell]
   int32_t *iters;          // optional per-cell iteration count (fallback: 1000 + its)
   int *fail;               // device counter of cells that converged in neither phase
+  unsigned long long *itc; // optional: [0] Picard, [1] fallback iterations, [2] cells, summed
   const StepState *st;     // if non-null, dt is read from st->dt
   double dt, dx, dy;
   long ncell;
@@ -550,7 +551,13 @@ __global__ __launch_bounds__(64) void k_predictor(PredArgs a) {
       } else {
 #pragma unroll
         for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = q1[v];
-        if (a.iters && node == 0) a.iters[cell] = it;
+        if (node == 0) {
+          if (a.iters) a.iters[cell] = it;
+          if (a.itc) {
+            atomicAdd(&a.itc[0], (unsigned long long)it);
+            atomicAdd(&a.itc[2], 1ull);
+          }
+        }
         phase = 2;
       }
       if (to_fallback) {
@@ -570,6 +577,12 @@ __global__ __launch_bounds__(64) void k_predictor(PredArgs a) {
         bool ok = !cmore && !cnan;
         if (node == 0) {
           if (a.iters) a.iters[cell] = ok ? 1000 + pit : -1;
+          if (a.itc) {
+            atomicAdd(&a.itc[0], (unsigned long long)it);
+            atomicAdd(&a.itc[1], (unsigned long long)pit);
+            atomicAdd(&a.itc[2], 1ull);
+            atomicAdd(&a.itc[3], 1ull);
+          }
           if (!ok) atomicAdd(a.fail, 1);
         }
         phase = ok ? 2 : 3;

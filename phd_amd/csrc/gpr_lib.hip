@@ -194,6 +194,12 @@ struct gpr_ctx {
   int4 run_bc{0, 0, 0, 0};
   Mat run_mat;
   bool run_ready = false;
+  // per-phase timing of gpr_dev_steps
+  bool timing = false;
+  std::vector<hipEvent_t> events;  // 6 per timed step
+  size_t ev_used = 0;
+  int64_t timed_steps = 0;
+  unsigned long long *d_itc = nullptr;  // Picard, fallback iterations, cells
 };
 
 namespace {
@@ -561,6 +567,7 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_st, sizeof(StepState));
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_fail, sizeof(int) * 4);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_itc, sizeof(unsigned long long) * 4);
   if (e != hipSuccess) {
     int rc = fail(GPR_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
     gpr_ctx_destroy(c);
@@ -577,6 +584,8 @@ int gpr_ctx_destroy(gpr_ctx *c) {
     if (kv.second.first) (void)hipFree(kv.second.first);
   if (c->d_st) (void)hipFree(c->d_st);
   if (c->d_fail) (void)hipFree(c->d_fail);
+  if (c->d_itc) (void)hipFree(c->d_itc);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GPR_OK;
@@ -951,18 +960,34 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
   double *dqh = (double *)c->bufs["qh"].first;
   uint8_t *dm = (uint8_t *)c->bufs["mask"].first;
   for (int64_t k = 0; k < nsteps; k++) {
+    hipEvent_t *ev = nullptr;
+    if (c->timing) {
+      while (c->events.size() < c->ev_used + 6) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        c->events.push_back(e);
+      }
+      ev = &c->events[c->ev_used];
+      c->ev_used += 6;
+      c->timed_steps++;
+      HIP_TRY(hipEventRecord(ev[0], c->stream));
+    }
     rc = dispatch<BoundariesOp>(c, c, (const double *)c->run_u, dub, c->run_nx, c->run_ny, c->run_bc);
     if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
     rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
     if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
     PredArgs pa = pred_args(c, c->run_mat, 0, dX);
     pa.st = c->d_st;
     pa.qh = dqh;
     pa.wh = dwh;
     pa.mask = dm;
     pa.ncell = ne;
+    pa.itc = c->timing ? c->d_itc : nullptr;
     rc = dispatch<PredictorOp>(c, c, pa);
     if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
     FvArgs fa = fv_args(c, c->run_mat, 0, dX, nX);
     fa.st = c->d_st;
     fa.u = c->run_u;
@@ -973,12 +998,49 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
     if (rc) return rc;
     rc = dispatch<FvOp>(c, c, fa);
     if (rc) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[4], c->stream));
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_st);
     rc = dispatch<ResetOp>(c, c, c->run_u, nc, dX[0], dX[1], c->run_mat, c->d_st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->stream, c->d_st);
     HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[5], c->stream));
   }
+  return GPR_OK;
+}
+
+int gpr_dev_set_timing(gpr_ctx *c, int enable) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (enable) {
+    HIP_TRY(hipMemsetAsync(c->d_itc, 0, sizeof(unsigned long long) * 4, c->stream));
+    c->ev_used = 0;
+    c->timed_steps = 0;
+  }
+  c->timing = enable != 0;
+  return GPR_OK;
+}
+
+int gpr_dev_timing(gpr_ctx *c, double *ms, int64_t *steps, int64_t *iters, int64_t *cells) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double tot[5] = {0, 0, 0, 0, 0};
+  for (size_t s = 0; s + 6 <= c->ev_used; s += 6)
+    for (int p = 0; p < 5; p++) {
+      float f = 0;
+      HIP_TRY(hipEventElapsedTime(&f, c->events[s + p], c->events[s + p + 1]));
+      tot[p] += f;
+    }
+  unsigned long long h[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(h, c->d_itc, sizeof h, hipMemcpyDeviceToHost));
+  if (ms)
+    for (int p = 0; p < 5; p++) ms[p] = tot[p];
+  if (steps) *steps = c->timed_steps;
+  if (iters) {
+    iters[0] = (int64_t)h[0];
+    iters[1] = (int64_t)h[1];
+    iters[2] = (int64_t)h[3];
+  }
+  if (cells) *cells = (int64_t)h[2];
   return GPR_OK;
 }
 
