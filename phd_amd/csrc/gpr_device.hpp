@@ -350,6 +350,20 @@ __device__ __forceinline__ double max_abs_eig3(const double *a) {
   double Qc = -2.0 * m * m * m + c1 * m - c0;
   double disc = (Qc * Qc) / 4.0 + (P * P * P) / 27.0;
   double best;
+  if (disc <= 0.0 && c2 >= 0.0 && c1 >= 0.0 && c0 >= 0.0) {
+    // three real roots, all >= 0 (signs of the coefficients): Newton from the
+    // trace (>= the largest root) decreases monotonically onto the largest root
+    double x = c2;
+    for (int it = 0; it < 60; it++) {
+      double f = ((x - c2) * x + c1) * x - c0;
+      double fp = (3 * x - 2 * c2) * x + c1;
+      if (!(fp > 0.0)) break;
+      double nx = x - f / fp;
+      if (!(nx < x)) break;
+      x = nx;
+    }
+    return fabs(x);
+  }
   if (disc <= 0.0) {
     double rr = sqrt(fmax(-P / 3.0, 0.0));
     double arg = (rr > 0) ? (-Qc / 2.0) / (rr * rr * rr) : 0.0;

@@ -663,61 +663,87 @@ struct FvArgs {
   Mat m;
 };
 
-// One thread per face; interfs1/interfs2 (fv.cpp:111-160, 162-268).
+// Faces: interfs1/interfs2 (fv.cpp:111-160, 162-268).  One lane per (face,
+// face node s); each lane accumulates its N time nodes, then the N lanes of a
+// face are summed with cross-lane shuffles.  Face d = 0 joins extended cells
+// (i, j+1) and (i+1, j+1) for i in [0, nx], j in [0, ny); face d = 1 joins
+// (i+1, j) and (i+1, j+1) for i in [0, nx), j in [0, ny].
 template <int N, bool TH, int ND>
-__global__ __launch_bounds__(128) void k_fv_faces(FvArgs a) {
+__global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
   constexpr int NN = NS * N;
+  constexpr int NSF = ND == 2 ? N : 1;  // nodes per face
+  constexpr int FPW = WAVE / NSF;       // faces per wavefront
   const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
   const long nfx = (long)(nx + 1) * ny;
   const long nfy = ND == 2 ? (long)nx * (ny + 1) : 0;
   const double dt = a.st ? a.st->dt : a.dt;
-  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < nfx + nfy; id += (long)gridDim.x * blockDim.x) {
-    const int d = id < nfx ? 0 : 1;
-    long e0, e1;
-    double *out;
+  const int lane = threadIdx.x;
+  const int fl = lane / NSF, sf = lane - fl * NSF;
+  const long id = (long)blockIdx.x * FPW + fl;
+  const bool ok = fl < FPW && id < nfx + nfy;
+  const int d = id < nfx ? 0 : 1;
+  long e0 = 0, e1 = 0;
+  double *out = nullptr;
+  if (ok) {
     if (d == 0) {
-      int i = id / ny, j = id % ny;  // face between extended (i, j+1) and (i+1, j+1)
+      int i = id / ny, j = id % ny;
       e0 = ND == 2 ? (long)i * (ny + 2) + (j + 1) : i;
       e1 = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
       out = a.fx + id * 2 * V;
     } else {
       long k = id - nfx;
-      int i = k / (ny + 1), j = k % (ny + 1);  // between extended (i+1, j) and (i+1, j+1)
+      int i = k / (ny + 1), j = k % (ny + 1);
       e0 = (long)(i + 1) * (ny + 2) + j;
       e1 = e0 + 1;
       out = a.fy + k * 2 * V;
     }
-    double lo[V], hi[V];
+  }
+  double lo[V], hi[V];
 #pragma unroll
-    for (int v = 0; v < V; v++) { lo[v] = 0.0; hi[v] = 0.0; }
-    if (a.mask[e0] && a.mask[e1]) {
-      const double h = d == 0 ? a.dx : a.dy;
-      for (int t = 0; t < N; t++) {
-        const double kt = dt * a.K.w[t] / (2.0 * h);
-        const int nsf = ND == 2 ? N : 1;
-        for (int sf = 0; sf < nsf; sf++) {
-          double qL[V], qR[V];
-          const double *pl = a.qh + (e0 * NN + t * NS) * V;
-          const double *pr = a.qh + (e1 * NN + t * NS) * V;
+  for (int v = 0; v < V; v++) {
+    lo[v] = 0.0;
+    hi[v] = 0.0;
+  }
+  if (ok && a.mask[e0] && a.mask[e1]) {
+    const double h = d == 0 ? a.dx : a.dy;
+    for (int t = 0; t < N; t++) {
+      const double kt = dt * a.K.w[t] / (2.0 * h);
+      double qL[V], qR[V];
+      const double *pl = a.qh + (e0 * NN + t * NS) * V;
+      const double *pr = a.qh + (e1 * NN + t * NS) * V;
 #pragma unroll
-          for (int v = 0; v < V; v++) {
-            double l = 0, r = 0;
+      for (int v = 0; v < V; v++) {
+        double l = 0, r = 0;
 #pragma unroll
-            for (int k = 0; k < N; k++) {
-              int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
-              l += a.K.endv[1][k] * pl[nl * V + v];
-              r += a.K.endv[0][k] * pr[nl * V + v];
-            }
-            qL[v] = l;
-            qR[v] = r;
-          }
-          const double kk = ND == 2 ? kt * a.K.w[sf] : kt;
-          riemann_accum<N, TH>(qL, qR, d, a.m, a.K, kk, lo, hi);
+        for (int k = 0; k < N; k++) {
+          int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
+          l += a.K.endv[1][k] * pl[nl * V + v];
+          r += a.K.endv[0][k] * pr[nl * V + v];
         }
+        qL[v] = l;
+        qR[v] = r;
       }
+      const double kk = ND == 2 ? kt * a.K.w[sf] : kt;
+      riemann_accum<N, TH>(qL, qR, d, a.m, a.K, kk, lo, hi);
     }
+  }
+  if (NSF > 1) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double sl = lo[v], sh = hi[v];
+#pragma unroll
+      for (int k = 1; k < NSF; k++) {
+        int src = lane + k < WAVE ? lane + k : lane;
+        sl += __shfl(lo[v], src);
+        sh += __shfl(hi[v], src);
+      }
+      lo[v] = sl;
+      hi[v] = sh;
+    }
+  }
+  if (ok && sf == 0) {
 #pragma unroll
     for (int v = 0; v < V; v++) {
       out[v] = lo[v];
@@ -726,107 +752,130 @@ __global__ __launch_bounds__(128) void k_fv_faces(FvArgs a) {
   }
 }
 
-// One thread per real cell: centres (fv.cpp:11-109), minus the face terms,
-// then the optional post-step work.
+// Cell centres (fv.cpp:11-109) with the face terms: one lane per space-time
+// node of a real cell, as in the predictor.  Each lane forms
+// dt W_t W_a W_b (S(q) - B_x dq/dx - B_y dq/dy) at its node; the cell's
+// lanes are summed through LDS and the face jumps subtracted.
 template <int N, bool TH, int ND>
-__global__ __launch_bounds__(128) void k_fv_cells(FvArgs a) {
+__global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   constexpr int V = Dims<TH>::V;
-  constexpr int NS = ND == 1 ? N : N * N;
-  constexpr int NN = NS * N;
+  constexpr int NS = PredShape<N, ND>::NS;
+  constexpr int NN = PredShape<N, ND>::NN;
+  constexpr int CPW = PredShape<N, ND>::CPW;
+  __shared__ double sQ[CPW * NN * V];
+  const Basis &K = a.K;
+  const int lane = threadIdx.x;
+  const int slot = lane / NN;
+  const int node = lane - slot * NN;
+  const int t = node / NS;
+  const int s = node - t * NS;
+  const int ia = ND == 1 ? s : s / N;
+  const int ib = ND == 1 ? 0 : s - (s / N) * N;
   const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
   const long ncell = (long)nx * ny;
+  const long c = (long)blockIdx.x * CPW + slot;
+  const bool lane_ok = slot < CPW;
+  const bool valid = lane_ok && c < ncell;
+  const int ci = valid ? (ND == 2 ? c / ny : c) : 0, cj = valid ? (ND == 2 ? c % ny : 0) : 0;
+  const long e = ND == 2 ? (long)(ci + 1) * (ny + 2) + (cj + 1) : ci + 1;
+  const bool active = valid && a.mask[e];
   const double dt = a.st ? a.st->dt : a.dt;
+  const int sl = lane_ok ? slot : 0;
+  double *myQ = sQ + sl * NN * V;
+
+  double q[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) q[v] = active ? a.qh[(e * NN + node) * V + v] : (v == 0 ? 1.0 : 0.0);
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myQ[node * V + v] = q[v];
+  __syncthreads();
+  double S[V];
+  {
+    NodeState<double, TH> st;
+    node_state<double, TH>(q, a.m, st);
+    source<double, TH>(q, st, a.m, S);
+    double dq[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += K.der[ia][k] * myQ[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
+      dq[v] = acc / a.dx;
+    }
+    bdot_sub(st.v, dq, 0, 1.0, S);
+    if (ND == 2) {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += K.der[ib][k] * myQ[(t * NS + ia * N + k) * V + v];
+        dq[v] = acc / a.dy;
+      }
+      bdot_sub(st.v, dq, 1, 1.0, S);
+    }
+    const double wgt = ND == 2 ? dt * K.w[t] * K.w[ia] * K.w[ib] : dt * K.w[t] * K.w[ia];
+#pragma unroll
+    for (int v = 0; v < V; v++) S[v] = active ? wgt * S[v] : 0.0;
+  }
+  __syncthreads();
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myQ[node * V + v] = S[v];
+  __syncthreads();
+  // one lane per variable (a cell has fewer nodes than variables in 1D)
+  for (int v = node; valid && v < V; v += NN) {
+    double du = 0;
+    for (int k = 0; k < NN; k++) du += myQ[k * V + v];
+    const double *fl = a.fx + ((long)ci * ny + cj) * 2 * V;        // left face: hi side
+    const double *fr = a.fx + ((long)(ci + 1) * ny + cj) * 2 * V;  // right face: lo side
+    du -= fl[V + v] + fr[v];
+    if (ND == 2) {
+      const double *fb = a.fy + ((long)ci * (ny + 1) + cj) * 2 * V;
+      const double *ft = a.fy + ((long)ci * (ny + 1) + cj + 1) * 2 * V;
+      du -= fb[V + v] + ft[v];
+    }
+    a.u[c * V + v] += du;
+  }
+}
+
+// Post-step work on the updated u, one thread per cell: renorm_distortion
+// (utils.cpp:6-23), the contortion flag (utils.cpp:65-85), a non-finite flag
+// (iterator.cpp:161) and the CFL maximum for the next dt (iterator.cpp:58-72).
+template <bool TH, int ND>
+__global__ __launch_bounds__(256) void k_post(double *u, long ncell, double dx, double dy, Mat m, StepState *st) {
+  constexpr int V = Dims<TH>::V;
   double local_max = 0.0;
   int contorted = 0, nonfinite = 0;
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
-    const int i = ND == 2 ? c / ny : c, j = ND == 2 ? c % ny : 0;
-    const long e = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
-    double du[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) du[v] = 0.0;
-    if (a.mask[e]) {
-      for (int t = 0; t < N; t++) {
-        const double *qc = a.qh + (e * NN + t * NS) * V;
-        const double wt = dt * a.K.w[t];
-        for (int sn = 0; sn < NS; sn++) {
-          const int ia = ND == 1 ? sn : sn / N, ib = ND == 1 ? 0 : sn % N;
-          double q[V], S[V], dq[V];
-#pragma unroll
-          for (int v = 0; v < V; v++) q[v] = qc[sn * V + v];
-          NodeState<double, TH> st;
-          node_state<double, TH>(q, a.m, st);
-          source<double, TH>(q, st, a.m, S);
-#pragma unroll
-          for (int v = 0; v < V; v++) {
-            double acc = 0;
-#pragma unroll
-            for (int k = 0; k < N; k++) acc += a.K.der[ia][k] * qc[(ND == 1 ? k : k * N + ib) * V + v];
-            dq[v] = acc / a.dx;
-          }
-          bdot_sub(st.v, dq, 0, 1.0, S);
-          if (ND == 2) {
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-              double acc = 0;
-#pragma unroll
-              for (int k = 0; k < N; k++) acc += a.K.der[ib][k] * qc[(ia * N + k) * V + v];
-              dq[v] = acc / a.dy;
-            }
-            bdot_sub(st.v, dq, 1, 1.0, S);
-          }
-          const double ws = ND == 2 ? wt * a.K.w[ia] * a.K.w[ib] : wt * a.K.w[ia];
-#pragma unroll
-          for (int v = 0; v < V; v++) du[v] += ws * S[v];
-        }
-      }
-    }
-    // faces: this cell is "hi" of its left face and "lo" of its right face
-    {
-      const double *fl = a.fx + ((long)i * ny + j) * 2 * V;        // face i: hi side
-      const double *fr = a.fx + ((long)(i + 1) * ny + j) * 2 * V;  // face i+1: lo side
-#pragma unroll
-      for (int v = 0; v < V; v++) du[v] -= fl[V + v] + fr[v];
-      if (ND == 2) {
-        const double *fb = a.fy + ((long)i * (ny + 1) + j) * 2 * V;
-        const double *ft = a.fy + ((long)i * (ny + 1) + j + 1) * 2 * V;
-#pragma unroll
-        for (int v = 0; v < V; v++) du[v] -= fb[V + v] + ft[v];
-      }
-    }
     double q[V];
 #pragma unroll
-    for (int v = 0; v < V; v++) q[v] = a.u[c * V + v] + du[v];
-    if (a.post) {
-      // renorm_distortion (utils.cpp:6-23)
-      if (q[0] > 0) {
-        double cf = cbrt(q[0] / (a.m.p.rho0 * det3(q + 5)));
+    for (int v = 0; v < V; v++) q[v] = u[c * V + v];
+    if (q[0] > 0) {
+      double cf = cbrt(q[0] / (m.p.rho0 * det3(q + 5)));
 #pragma unroll
-        for (int k = 0; k < 9; k++) q[5 + k] *= cf;
-      }
-      // contorted (utils.cpp:65-85): mean squared singular value over |det A|^(2/3)
-      double f = 0;
-#pragma unroll
-      for (int k = 0; k < 9; k++) f += q[5 + k] * q[5 + k];
-      double d13 = cbrt(fabs(det3(q + 5)));
-      if (f / (d13 * d13) / 3.0 - 1.0 > a.st->contorted_tol) contorted = 1;
-#pragma unroll
-      for (int v = 0; v < V; v++)
-        if (!(q[v] - q[v] == 0.0)) nonfinite = 1;
-      double sp = 0;
-      for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, a.m, d) / (d == 0 ? a.dx : a.dy);
-      if (sp > local_max) local_max = sp;
+      for (int k = 0; k < 9; k++) q[5 + k] *= cf;
     }
+    double f = 0;
 #pragma unroll
-    for (int v = 0; v < V; v++) a.u[c * V + v] = q[v];
+    for (int k = 0; k < 9; k++) f += q[5 + k] * q[5 + k];
+    double d13 = cbrt(fabs(det3(q + 5)));
+    if (f / (d13 * d13) / 3.0 - 1.0 > st->contorted_tol) contorted = 1;
+#pragma unroll
+    for (int v = 0; v < V; v++)
+      if (!(q[v] - q[v] == 0.0)) nonfinite = 1;
+    double sp = 0;
+    for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
+    if (sp > local_max) local_max = sp;
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[c * V + 5 + k] = q[5 + k];
   }
-  if (a.post) {
-    double m = wave_max(local_max);
-    int cf = __any(contorted), nf = __any(nonfinite);
-    if ((threadIdx.x & 63) == 0) {
-      atomic_max_pos(&a.st->maxspeed, m);
-      if (cf) atomicOr(&a.st->contorted, 1);
-      if (nf) atomicOr(&a.st->nonfinite, 1);
-    }
+  double mx = wave_max(local_max);
+  int cf = __any(contorted), nf = __any(nonfinite);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&st->maxspeed, mx);
+    if (cf) atomicOr(&st->contorted, 1);
+    if (nf) atomicOr(&st->nonfinite, 1);
   }
 }
 

@@ -308,9 +308,14 @@ template <int N, bool TH, int ND> struct FvOp {
   static int run(gpr_ctx *c, FvArgs a) {
     a.K = basis(N);
     long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
-    hipLaunchKernelGGL((k_fv_faces<N, TH, ND>), dim3(grid_for(nf, 128)), dim3(128), 0, c->stream, a);
+    constexpr int FPW = WAVE / (ND == 2 ? N : 1);
+    hipLaunchKernelGGL((k_fv_faces<N, TH, ND>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
     long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
-    hipLaunchKernelGGL((k_fv_cells<N, TH, ND>), dim3(grid_for(nc, 128)), dim3(128), 0, c->stream, a);
+    constexpr int CPW = PredShape<N, ND>::CPW;
+    hipLaunchKernelGGL((k_fv_cells<N, TH, ND>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a);
+    if (a.post)
+      hipLaunchKernelGGL((k_post<TH, ND>), dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a.u, nc, a.dx, a.dy,
+                         a.m, a.st);
     HIP_TRY(hipGetLastError());
     return GPR_OK;
   }
