@@ -12,7 +12,9 @@ import numpy as np
 from .params import Par
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpr_mi355x.so")
+# GPR_LIB names an alternative build of the same sources in this directory
+# (phd_amd/build.py with GPR_LIB_NAME), e.g. for comparing compile options
+LIB_PATH = os.path.join(HERE, os.environ.get("GPR_LIB", "libgpr_mi355x.so"))
 
 GPR_OK = 0
 ERRORS = {-1: "GPR_E_ARG", -2: "GPR_E_UNSUPPORTED", -3: "GPR_E_HIP", -4: "GPR_E_EOS",

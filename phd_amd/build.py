@@ -13,14 +13,14 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-LIB = os.path.join(HERE, "libgpr_mi355x.so")
+LIB = os.path.join(HERE, os.environ.get("GPR_LIB_NAME", "libgpr_mi355x.so"))
 ARCH = os.environ.get("GPR_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
-]
+] + os.environ.get("GPR_EXTRA_FLAGS", "").split()
 
 
 def hipcc():
@@ -37,9 +37,10 @@ def sources():
 def build(verbose=False, jobs=None):
     objs = []
     procs = []
-    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    bdir = os.path.join(HERE, "_build", os.path.splitext(os.path.basename(LIB))[0])
+    os.makedirs(bdir, exist_ok=True)
     for src in sources():
-        obj = os.path.join(HERE, "_build", os.path.basename(src) + ".o")
+        obj = os.path.join(bdir, os.path.basename(src) + ".o")
         cmd = [hipcc()] + HIPCC_FLAGS + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -14,11 +14,10 @@
 //   k_reset / k_finalize_dt   (iterator.cpp:54-83,145-151, utils.cpp:44-85)
 #pragma once
 
-#include "gpr_device.hpp"
+#include "gpr_predictor.hpp"
 
 namespace gpr {
 
-constexpr int WAVE = 64;
 
 // Device-resident time-loop state.
 struct StepState {
@@ -219,375 +218,6 @@ __global__ void k_weno_y(const R *__restrict__ ux, R *__restrict__ wh, int nx, i
     weno_point<R, N>(d, K, NY, gap, j, o);
 #pragma unroll
     for (int b = 0; b < N; b++) wh[((((long)i * NY + j) * N + a) * N + b) * V + v] = o[b];
-  }
-}
-
-// ----------------------------------------------------------------- predictor
-struct PredArgs {
-  double *qh;              // [ncell][NN][V]
-  const double *wh;        // [ncell][NS][V]
-  const uint8_t *mask;     // [ncell]
-  int32_t *iters;          // optional per-cell iteration count (fallback: 1000 + its)
-  int *fail;               // device counter of cells that converged in neither phase
-  unsigned long long *itc; // optional: [0] Picard, [1] fallback iterations, [2] cells, summed
-  const StepState *st;     // if non-null, dt is read from st->dt
-  double dt, dx, dy;
-  long ncell;
-  int dg_it, stiff_it, bailout, mode;  // mode 0: Picard then fallback; 1: fallback only
-  double dg_tol, stiff_tol;
-  Basis K;
-  Mat m;
-};
-
-// Preconditioner of one spatial node, frozen at its WENO state ws:
-// Ainv = (DG_MAT + dt kappa W)^-1, Jinv = (DG_MAT + dt rJ W)^-1 and the polar
-// rotation Rot of A, where the distortion source has Jacobian -kappa Pi at a
-// relaxed state (Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3)).
-template <int N, bool TH>
-__device__ void stiff_precond(const double *ws, const PredArgs &a, double dt, double *out27) {
-  NodeState<double, TH> s;
-  node_state<double, TH>(ws, a.m, s);
-  double dA = det3(ws + 5);
-  double s13 = cbrt(dA);
-  double kap = 2.0 * theta1inv<double, TH>(ws, s, a.m) * s.cs2 * s13 * s13;
-  if (!(kap >= 0.0) || !(kap < 1e300)) kap = 0.0;
-  double rJ = 0.0;
-  if (TH) {
-    rJ = a.m.p.calpha2 * s.T / (a.m.p.kappa * s.r);
-    if (!(rJ >= 0.0) || !(rJ < 1e300)) rJ = 0.0;
-  }
-  for (int which = 0; which < 2; which++) {
-    double rate = which == 0 ? kap : rJ;
-    double M[N][2 * N];
-    for (int i = 0; i < N; i++)
-      for (int j = 0; j < 2 * N; j++)
-        M[i][j] = j < N ? a.K.dgmat[i][j] + (i == j ? dt * rate * a.K.w[i] : 0.0) : (j - N == i ? 1.0 : 0.0);
-    for (int c = 0; c < N; c++) {
-      int piv = c;
-      for (int r = c + 1; r < N; r++)
-        if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
-      if (piv != c)
-        for (int j = 0; j < 2 * N; j++) { double tt = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = tt; }
-      double dg = M[c][c];
-      for (int j = 0; j < 2 * N; j++) M[c][j] /= dg;
-      for (int r = 0; r < N; r++) {
-        if (r == c) continue;
-        double f = M[r][c];
-        for (int j = 0; j < 2 * N; j++) M[r][j] -= f * M[c][j];
-      }
-    }
-    for (int i = 0; i < N; i++)
-      for (int j = 0; j < N; j++) out27[which * 9 + i * N + j] = M[i][N + j];
-  }
-  double X[9];
-  for (int i = 0; i < 9; i++) X[i] = ws[5 + i];
-  if (dA > 0) {
-    for (int it = 0; it < 40; it++) {
-      double d = det3(X), inv[9];
-      inv[0] = (X[4] * X[8] - X[5] * X[7]) / d;
-      inv[1] = (X[2] * X[7] - X[1] * X[8]) / d;
-      inv[2] = (X[1] * X[5] - X[2] * X[4]) / d;
-      inv[3] = (X[5] * X[6] - X[3] * X[8]) / d;
-      inv[4] = (X[0] * X[8] - X[2] * X[6]) / d;
-      inv[5] = (X[2] * X[3] - X[0] * X[5]) / d;
-      inv[6] = (X[3] * X[7] - X[4] * X[6]) / d;
-      inv[7] = (X[1] * X[6] - X[0] * X[7]) / d;
-      inv[8] = (X[0] * X[4] - X[1] * X[3]) / d;
-      double ch = 0;
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-          double nx = 0.5 * (X[i * 3 + j] + inv[j * 3 + i]);
-          ch = fmax(ch, fabs(nx - X[i * 3 + j]));
-          X[i * 3 + j] = nx;
-        }
-      if (ch < 1e-15) break;
-    }
-  } else {
-    for (int i = 0; i < 9; i++) X[i] = (i % 4 == 0) ? 1.0 : 0.0;
-  }
-  for (int i = 0; i < 9; i++) out27[18 + i] = X[i];
-}
-
-// Pi(X) = Rot dev(sym(Rot^T X))
-template <typename R> __device__ __forceinline__ void proj_dev_sym(const double *Rot, const R *X, R *out) {
-  R Y[9], S[9];
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-      Y[i * 3 + j] = R(Rot[0 * 3 + i]) * X[0 * 3 + j] + R(Rot[1 * 3 + i]) * X[1 * 3 + j] + R(Rot[2 * 3 + i]) * X[2 * 3 + j];
-  R tr = (Y[0] + Y[4] + Y[8]) / R(3);
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) S[i * 3 + j] = R(0.5) * (Y[i * 3 + j] + Y[j * 3 + i]) - (i == j ? tr : R(0));
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-      out[i * 3 + j] = R(Rot[i * 3 + 0]) * S[0 * 3 + j] + R(Rot[i * 3 + 1]) * S[1 * 3 + j] + R(Rot[i * 3 + 2]) * S[2 * 3 + j];
-}
-
-// One wavefront of 64 lanes holds CPW cells, one space-time node per lane.
-template <int N, int ND> struct PredShape {
-  static constexpr int NS = ND == 1 ? N : N * N;  // spatial nodes per cell
-  static constexpr int NN = NS * N;                // space-time nodes per cell
-  static constexpr int CPW = WAVE / NN;            // cells per wavefront
-};
-
-template <int N, bool TH, int ND>
-__global__ __launch_bounds__(64) void k_predictor(PredArgs a) {
-  constexpr int V = Dims<TH>::V;
-  constexpr int NS = PredShape<N, ND>::NS;
-  constexpr int NN = PredShape<N, ND>::NN;
-  constexpr int CPW = PredShape<N, ND>::CPW;
-  using R = double;
-  __shared__ R sQ[CPW * NN * V];
-  __shared__ R sF[CPW * NN * V];
-  __shared__ R sG[ND == 2 ? CPW * NN * V : CPW * NN * 9];
-  __shared__ double sPre[CPW * NS * 27];
-  __shared__ R sRat[WAVE];
-
-  const Basis &K = a.K;
-  const int lane = threadIdx.x;
-  const int slot = lane / NN;
-  const int node = lane - slot * NN;
-  const int t = node / NS;
-  const int s = node - t * NS;
-  const int ia = ND == 1 ? s : s / N;
-  const int ib = ND == 1 ? 0 : s - (s / N) * N;
-  const long cell = (long)blockIdx.x * CPW + slot;
-  const bool lane_ok = slot < CPW;
-  const bool valid = lane_ok && cell < a.ncell;
-  const bool active = valid && a.mask[cell];
-  const int sl = slot < CPW ? slot : 0;  // idle lanes alias slot 0 for LDS addresses they never write
-  R *myQ = sQ + (sl * NN) * V;
-  R *myF = sF + (sl * NN) * V;
-  R *myG = sG + (sl * NN) * (ND == 2 ? V : 9);
-  double *myPre = sPre + (sl * NS + s) * 27;
-
-  const double dt = a.st ? a.st->dt : a.dt;
-  const R Ws = ND == 1 ? R(K.w[ia]) : R(K.w[ia]) * R(K.w[ib]);
-  const R Wt = R(K.w[t]);
-  const R e0t = R(K.endv[0][t]);
-
-  R w[V], q0[V], q1[V];
-  if (active) {
-#pragma unroll
-    for (int v = 0; v < V; v++) w[v] = a.wh[(cell * NS + s) * V + v];
-  } else {
-#pragma unroll
-    for (int v = 0; v < V; v++) w[v] = v == 0 ? R(1) : (v == 5 || v == 9 || v == 13 ? R(1) : R(0));
-  }
-#pragma unroll
-  for (int v = 0; v < V; v++) q0[v] = w[v];
-
-  // phase: 0 Picard, 1 preconditioned, 2 converged, 3 idle / given up
-  int phase = active ? (a.mode == 1 ? 1 : 0) : 3;
-  int it = 0, pit = 0, grow = 0;
-  R prev = R(1e308);
-  const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * NN);
-  bool need_pre = phase == 1;
-
-  while (__any(phase == 0 || phase == 1)) {
-    if (need_pre && t == 0 && lane_ok) stiff_precond<N, TH>(w, a, dt, myPre);
-    need_pre = false;
-    // (1) share the iterate (lanes past the last whole cell write nothing)
-    if (lane_ok)
-#pragma unroll
-      for (int v = 0; v < V; v++) myQ[node * V + v] = q0[v];
-    __syncthreads();
-
-    // (2) spatial derivatives, local source and non-conservative terms, fluxes
-    R rhs[V];
-    {
-      R dq[V];
-      NodeState<R, TH> st;
-      node_state<R, TH>(q0, a.m, st);
-      source<R, TH>(q0, st, a.m, rhs);
-      const R inv_dx = R(1) / R(a.dx);
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        R acc = 0;
-#pragma unroll
-        for (int j = 0; j < N; j++) acc += R(K.der[ia][j]) * myQ[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
-        dq[v] = acc * inv_dx;
-      }
-      bdot_sub(st.v, dq, 0, R(1), rhs);
-      if (ND == 2) {
-        const R inv_dy = R(1) / R(a.dy);
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-          R acc = 0;
-#pragma unroll
-          for (int j = 0; j < N; j++) acc += R(K.der[ib][j]) * myQ[(t * NS + ia * N + j) * V + v];
-          dq[v] = acc * inv_dy;
-        }
-        bdot_sub(st.v, dq, 1, R(1), rhs);
-      }
-      const R wts = Wt * Ws;
-#pragma unroll
-      for (int v = 0; v < V; v++) rhs[v] *= wts;
-      R F[V];
-      flux<R, TH>(q0, st, a.m, 0, F);
-      const R wb = ND == 2 ? R(K.w[ib]) : R(1);
-      if (lane_ok)
-#pragma unroll
-        for (int v = 0; v < V; v++) myF[node * V + v] = F[v] * wb;
-      if (ND == 2) {
-        flux<R, TH>(q0, st, a.m, 1, F);
-        if (lane_ok)
-#pragma unroll
-          for (int v = 0; v < V; v++) myG[node * V + v] = F[v];
-      }
-    }
-    __syncthreads();
-    // (3) flux contractions (rhs2, dg2.cpp:173-201), then dt and the initial term
-    {
-      const R cx = Wt / R(a.dx);
-      const R cy = ND == 2 ? Wt * R(K.w[ia]) / R(a.dy) : R(0);
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        R sx = 0;
-#pragma unroll
-        for (int j = 0; j < N; j++) sx += R(K.dgder[ia][j]) * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
-        R val = rhs[v] - cx * sx;
-        if (ND == 2) {
-          R sy = 0;
-#pragma unroll
-          for (int j = 0; j < N; j++) sy += R(K.dgder[ib][j]) * myG[(t * NS + ia * N + j) * V + v];
-          val -= cy * sy;
-        }
-        rhs[v] = val * R(dt) + e0t * Ws * w[v];
-      }
-    }
-    __syncthreads();
-    // (4) time solve: Picard q1 = (DG_MAT (x) W_s)^-1 rhs, or the preconditioned update
-    if (phase == 1) {
-      R g[V];
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        R acc = 0;
-#pragma unroll
-        for (int k = 0; k < N; k++) acc += R(K.dgmat[t][k]) * myQ[(k * NS + s) * V + v];
-        g[v] = acc - rhs[v] / Ws;
-      }
-      R P[9];
-      proj_dev_sym<R>(myPre + 18, g + 5, P);
-      R *pg = myG + node * (ND == 2 ? V : 9) + (ND == 2 ? 5 : 0);
-      if (lane_ok) {
-#pragma unroll
-        for (int v = 0; v < V; v++) myF[node * V + v] = g[v];
-#pragma unroll
-        for (int k = 0; k < 9; k++) pg[k] = P[k];
-      }
-    } else if (lane_ok) {
-#pragma unroll
-      for (int v = 0; v < V; v++) myF[node * V + v] = rhs[v];
-    }
-    __syncthreads();
-    if (phase == 1) {
-      const double *Ainv = myPre, *Jinv = myPre + 9;
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        R d = 0;
-        if (v >= 5 && v < 14) {
-#pragma unroll
-          for (int k = 0; k < N; k++) {
-            R gk = myF[(k * NS + s) * V + v];
-            R pk = myG[(k * NS + s) * (ND == 2 ? V : 9) + (ND == 2 ? v : v - 5)];
-            d += R(K.dgmatinv[t][k]) * (gk - pk) + R(Ainv[t * N + k]) * pk;
-          }
-        } else if (v >= 14) {
-#pragma unroll
-          for (int k = 0; k < N; k++) d += R(Jinv[t * N + k]) * myF[(k * NS + s) * V + v];
-        } else {
-#pragma unroll
-          for (int k = 0; k < N; k++) d += R(K.dgmatinv[t][k]) * myF[(k * NS + s) * V + v];
-        }
-        q1[v] = q0[v] - d;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        R acc = 0;
-#pragma unroll
-        for (int k = 0; k < N; k++) acc += R(K.dgmatinv[t][k]) * myF[(k * NS + s) * V + v];
-        q1[v] = acc / Ws;
-      }
-    }
-    // (5) stopping rule over the cell's whole block (dg2.cpp:278-297)
-    const R tol = phase == 1 ? R(a.stiff_tol) : R(a.dg_tol);
-    bool more = false, nan = false;
-    R ratio = 0;
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      R dd = fabs(q1[v] - q0[v]), lim = tol * (R(1) + fabs(q0[v]));
-      more |= dd > lim;
-      nan |= q1[v] != q1[v];
-      ratio = fmax(ratio, dd / lim);
-    }
-    const unsigned long long bm = __ballot(more), bn = __ballot(nan);
-    const bool cmore = (bm & cellbits) != 0, cnan = (bn & cellbits) != 0;
-    if (a.bailout > 0) {
-      sRat[lane] = ratio;
-      __syncthreads();
-      R cr = 0;
-      for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
-      ratio = cr;
-      __syncthreads();
-    }
-    if (phase == 0) {
-      it++;
-      bool to_fallback = false;
-      if (cmore) {
-#pragma unroll
-        for (int v = 0; v < V; v++) q0[v] = q1[v];
-        grow = ratio > prev ? grow + 1 : 0;
-        prev = ratio;
-        if (it >= a.dg_it || (a.bailout > 0 && grow >= a.bailout)) to_fallback = true;
-      } else if (cnan) {
-        to_fallback = true;
-      } else {
-#pragma unroll
-        for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = q1[v];
-        if (node == 0) {
-          if (a.iters) a.iters[cell] = it;
-          if (a.itc) {
-            atomicAdd(&a.itc[0], (unsigned long long)it);
-            atomicAdd(&a.itc[2], 1ull);
-          }
-        }
-        phase = 2;
-      }
-      if (to_fallback) {
-        phase = 1;
-        pit = 0;
-        need_pre = true;
-#pragma unroll
-        for (int v = 0; v < V; v++) q0[v] = w[v];
-      }
-    } else if (phase == 1) {
-      pit++;
-#pragma unroll
-      for (int v = 0; v < V; v++) q0[v] = q1[v];
-      if (!cmore || pit >= a.stiff_it) {
-#pragma unroll
-        for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = q1[v];
-        bool ok = !cmore && !cnan;
-        if (node == 0) {
-          if (a.iters) a.iters[cell] = ok ? 1000 + pit : -1;
-          if (a.itc) {
-            atomicAdd(&a.itc[0], (unsigned long long)it);
-            atomicAdd(&a.itc[1], (unsigned long long)pit);
-            atomicAdd(&a.itc[2], 1ull);
-            atomicAdd(&a.itc[3], 1ull);
-          }
-          if (!ok) atomicAdd(a.fail, 1);
-        }
-        phase = ok ? 2 : 3;
-      }
-    }
   }
 }
 

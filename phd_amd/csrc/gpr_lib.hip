@@ -984,7 +984,7 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
     PredArgs pa = pred_args(c, c->run_mat, 0, dX);
-    pa.st = c->d_st;
+    pa.dt_dev = &c->d_st->dt;
     pa.qh = dqh;
     pa.wh = dwh;
     pa.mask = dm;
