@@ -16,6 +16,8 @@
 #include "etc/globals.h"
 #include "etc/grid.h"
 #include "solvers/dg/dg.h"
+#include "solvers/dg/dg1.h"
+#include "solvers/dg/dg2.h"
 #include "solvers/fv/fluxes.h"
 #include "solvers/fv/fv.h"
 #include "solvers/iterator.h"
@@ -28,6 +30,10 @@
 #include "system/variables/state.h"
 
 #include "../include/gpr_par.h"
+
+// defined in solvers/dg/dg1.cpp:68-73 and dg2.cpp:92-100 without a header
+void initial_condition1(Matr Ww, Matr w);
+void initial_condition2(Matr Ww, Matr w);
 
 namespace {
 
@@ -244,6 +250,32 @@ void ref_predictor(double *qh, const double *wh, double dt, const double *dX,
   for (long i = 0; i < ne; i++)
     m(i) = mask[i] != 0;
   predictor(qhm, whv, dt, d, stiff != 0, false, P, m);
+}
+
+// The reference's DG residual of one cell, obj1 / obj2 (dg1.cpp:54-66,
+// dg2.cpp:73-90), with Ww from its initial_condition1/2.
+void ref_dg_residual(double *out, const double *q, const double *w, double dt,
+                     const double *dX, int ndim, const gpr_par *p) {
+  Par P = to_par(p);
+  int nq = ndim == 1 ? N * N : N * N * N;
+  int nw = ndim == 1 ? N : N * N;
+  Mat wm(nw, V), Ww(nq, V);
+  for (int i = 0; i < nw; i++)
+    for (int v = 0; v < V; v++)
+      wm(i, v) = w[i * V + v];
+  Vec qv = Eigen::Map<const Vec>(q, nq * V);
+  Vec r;
+  if (ndim == 1) {
+    initial_condition1(Ww, wm);
+    MatN2_V Wf = Ww;
+    r = obj1(qv, Wf, dt, dX[0], P);
+  } else {
+    initial_condition2(Ww, wm);
+    MatN3_V Wf = Ww;
+    r = obj2(qv, Wf, dt, dX[0], dX[1], P);
+  }
+  for (int i = 0; i < nq * V; i++)
+    out[i] = r(i);
 }
 
 void ref_fv_launcher(double *u, const double *qh, const int32_t *nX, int ndim,

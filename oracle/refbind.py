@@ -162,6 +162,16 @@ class RefSolver:
                                ctypes.byref(MP), _u8(m))
         return qh
 
+    def dg_residual(self, q, w, dt, dX, MP):
+        """obj1/obj2 of one cell (dg1.cpp:54-66, dg2.cpp:73-90)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        dX = np.asarray(dX, dtype=np.float64)
+        out = np.zeros_like(q)
+        self.lib.ref_dg_residual(_dp(out), _dp(q), _dp(w), ctypes.c_double(dt), _dp(dX),
+                                 ctypes.c_int(len(dX)), ctypes.byref(MP))
+        return out
+
     def fv_launcher(self, u, qh, nX, dt, dX, MP, mask):
         nX = np.asarray(nX, dtype=np.int32)
         u = np.array(u, dtype=np.float64, copy=True).ravel()

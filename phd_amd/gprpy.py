@@ -105,9 +105,10 @@ class _Module:
     def flux(self, ret, Q, d, MP):
         """ret += F_d(Q) (equations.cpp:11-48 accumulates)."""
         _ref(ret, np.float64, "ret")
+        mp = par_ref(MP)
         Q, n = self._states(Q)
         out = np.zeros_like(Q)
-        check(_lib.load().gpr_flux(self.ctx().h, dptr(out), dptr(Q), n, int(d), par_ref(MP)))
+        check(_lib.load().gpr_flux(self.ctx().h, dptr(out), dptr(Q), n, int(d), mp))
         ret += out.reshape(ret.shape)
 
     def source(self, ret, Q, MP):
