@@ -63,21 +63,84 @@ def predictor_flops(picard_iters, fallback_iters, fallback_cells, nodes=27, spat
 MI355X_FP64_TFLOPS = 78.6   # dense FP64 (vector and matrix) peak
 MI355X_HBM_GBS = 8000.0
 
+# HBM bytes per predictor launch on the 512x512 workload, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a
+# plain 512 MiB device copy reads in the same runs); profiles/dsl512_pmc_traffic.md
+PREDICTOR_TRAFFIC_512 = 1235.2e6
 
-def cpu_baseline(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads):
-    """The oracle (CPU restatement, same algorithm) on `nsteps` of the same
-    steps the GPU timed, OpenMP over `threads` cores."""
+
+def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads):
+    """The reference's own C++ step loop (oracle/_ref, built from its sources
+    by oracle/Makefile) on `nsteps` steps from the GPU's timed starting state,
+    OpenMP over `threads` cores; seconds per step as the reference's loop
+    times them (timestep + boundaries + ader_stepper_para + post-step).  It
+    runs in a child process: the reference keeps Eigen's assertions, and its
+    stiff path can abort."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from refbind import available
+    if not available(3, False):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "state.npz")
+        np.savez(path, u=u_start, nX=np.array(nX), dX=np.array(dX), bcs=np.array(bcs), t=t0,
+                 count=count0, nsteps=nsteps, V=14, cfl=0.8, par=json.dumps(MP.as_dict()))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        p = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "refbind.py"), path],
+                           env=env, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        return {"error": f"exit {p.returncode}: {p.stderr.strip()[-300:]}"}
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    return {"sec": float(np.sum(r["secs"])), "steps": len(r["secs"]), "status": r["status"]}
+
+
+def cpu_port(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, bailout):
+    """The oracle (CPU restatement of the same algorithm) on the same steps."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from orcbind import Oracle
-    O = Oracle(3, False, 2, bailout=2)
+    O = Oracle(3, False, 2, bailout=bailout)
     u = u_start.copy()
     t, c = t0, count0
     tic = time.perf_counter()
     for _ in range(nsteps):
         u, t, c, dt, rc, st = O.step(u, nX, dX, 0.8, bcs, t, c, 1e300, MP)
+    return time.perf_counter() - tic
+
+
+def copy_probe(mib):
+    """One plain device-to-device copy of `mib` MiB through the library's
+    stream: a kernel of known traffic to scale HBM counters against."""
+    import ctypes
+    from phd_amd import _lib
+    ctx = _lib.Context(3, 2, False)
+    L = _lib.load()
+    nb = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(L.gpr_dev_malloc(ctx.h, ctypes.byref(a), nb))
+    _lib.check(L.gpr_dev_malloc(ctx.h, ctypes.byref(b), nb))
+    _lib.check(L.gpr_dev_memcpy(ctx.h, b, a, nb, 2))
+    _lib.check(L.gpr_dev_free(ctx.h, a))
+    _lib.check(L.gpr_dev_free(ctx.h, b))
+
+
+def timed_run(run, steps):
+    """K steps bracketed by a synchronisation of the library's stream (all
+    device work of this process runs on it); HIP-event phase timing on the
+    same stream over the same steps."""
+    run.set_timing(True)
+    run.sync()
+    tic = time.perf_counter()
+    run.steps(steps)
+    run.sync()
     sec = time.perf_counter() - tic
-    return u, sec
+    tm = run.timing()
+    run.set_timing(False)
+    st = run.status()
+    if st["pred_fail"] or st["nonfinite"]:
+        raise SystemExit(f"timed steps failed: {st}")
+    return sec, tm
 
 
 def main():
@@ -86,23 +149,29 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--bailout", type=int, default=2)
+    ap.add_argument("--no-faithful", action="store_true",
+                    help="skip the second timed run with bailout = 0")
+    ap.add_argument("--bailout", type=int, default=5)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="before the timed region, run one plain device copy of MIB MiB "
+                         "(a known-size kernel to scale HBM counters against)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every box in this setup has one GPU and the "
-              "multi-GPU slab decomposition is design-only (DESIGN.md). Run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: every box in this setup has one GPU, and the "
+              "multi-GPU slab decomposition of this solver is a design, not a run. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from phd_amd import problems
     from phd_amd.runner import DeviceRun
 
     n = args.n
     u0, MP, _, dX, bcs = problems.double_shear_2d(n)
     nX = [n, n]
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout)
     run.steps(args.warmup)
     st0 = run.status()
@@ -110,19 +179,7 @@ def main():
         raise SystemExit(f"warm-up failed: {st0}")
     u_w = run.host()
 
-    run.set_timing(True)
-    torch.cuda.synchronize()
-    run.sync()
-    tic = time.perf_counter()
-    run.steps(args.steps)
-    run.sync()
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - tic
-    tm = run.timing()
-    run.set_timing(False)
-    st1 = run.status()
-    if st1["pred_fail"] or st1["nonfinite"]:
-        raise SystemExit(f"timed steps failed: {st1}")
+    sec, tm = timed_run(run, args.steps)
 
     cells = n * n
     value = cells * args.steps / sec
@@ -136,7 +193,8 @@ def main():
         "bound": "mfma", "pipe": "fp64 VALU (peak equal to dense FP64 MFMA on MI355X)",
         "kernel": "k_predictor<3,false,2>", "achieved": round(achieved, 3),
         "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(achieved / MI355X_FP64_TFLOPS, 4), "traffic": None,
+        "frac": round(achieved / MI355X_FP64_TFLOPS, 4),
+        "traffic": PREDICTOR_TRAFFIC_512 if n == 512 else None,
         "flops_per_launch": flops_launch, "avg_launch_ms": pred_ms,
         "picard_iters_per_cell": tm["picard_iters"] / max(tm["cells"], 1),
         "fallback_iters_per_cell": tm["fallback_iters"] / max(tm["cells"], 1),
@@ -154,16 +212,40 @@ def main():
                    "N": 3, "V": 14, "bailout": args.bailout, "parallelism": "single GPU"},
         "roofline": roofline,
     }
+    if args.bailout != 0 and not args.no_faithful:
+        # the same steps with Picard run to DG_IT = 50 for every cell, as the
+        # reference does, from the same starting state
+        run0 = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=0, t0=st0["t"],
+                         count0=st0["count"])
+        sec0, _ = timed_run(run0, args.steps)
+        out["bailout0_value"] = cells * args.steps / sec0
+        out["bailout0_ms_per_step"] = 1e3 * sec0 / args.steps
+        del run0
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        _, csec = cpu_baseline(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,
-                               threads)
-        cval = cells * args.cpu_steps / csec
-        out["cpu_baseline"] = {
-            "value": cval, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same {n}x{n} run from the state after "
-                      f"the GPU warm-up (the first timed steps), oracle/liboracle.so with "
-                      f"OpenMP", "gpu_over_cpu": value / cval}
+        ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,
+                            threads)
+        psec = cpu_port(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps, threads,
+                        args.bailout)
+        pval = cells * args.cpu_steps / psec
+        if ref is not None and "error" not in ref:
+            cval = cells * ref["steps"] / ref["sec"]
+            out["cpu_baseline"] = {
+                "value": cval, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+                "sample": f"{ref['steps']} step(s) of the same {n}x{n} run from the GPU's timed "
+                          f"starting state (step {st0['count']}), the reference C++ built from its "
+                          f"sources (oracle/_ref, -O3 -march=x86-64-v3, OpenMP); its predictor "
+                          f"leaves the cells whose Picard iteration does not converge unassigned, "
+                          f"so its state after the step is "
+                          f"{'non-finite' if ref['status'] else 'finite'}",
+                "gpu_over_cpu": value / cval, "port_value": pval}
+        else:
+            out["cpu_baseline"] = {
+                "value": pval, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                "sample": f"{args.cpu_steps} step(s) of the same {n}x{n} run from the GPU's timed "
+                          f"starting state, oracle/liboracle.so with OpenMP"
+                          + (f" (the reference's own run failed: {ref['error']})" if ref else ""),
+                "gpu_over_cpu": value / pval}
     print(json.dumps(out))
 
 

@@ -134,6 +134,14 @@ int gpr_iterator(gpr_ctx *ctx, double *u, double tf, const int32_t *nX, const do
                  const int32_t *bcs, int split, int half_step, int stiff, int flux, const gpr_par *MPs,
                  int nmat, double contorted_tol, double *snapshots, int32_t *nsnap, int64_t *steps);
 
+/* --- device memory for callers without another allocator ------------------
+ * (no reference counterpart: the reference keeps everything in host memory).
+ * gpr_dev_memcpy is synchronous on the context's stream; kind 0 copies host
+ * to device, 1 device to host, 2 device to device. */
+int gpr_dev_malloc(gpr_ctx *ctx, void **d_ptr, int64_t bytes);
+int gpr_dev_free(gpr_ctx *ctx, void *d_ptr);
+int gpr_dev_memcpy(gpr_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);
+
 /* --- device-resident time loop (device pointers) ---------------------------
  * Prepare a run on a device-resident u (d_u, [cells][V] doubles): builds the
  * mask, sizes the work buffers and computes the first dt on the device. */

@@ -837,9 +837,9 @@ static void bc_copy_row(double *dst, const double *src, int V) {
   memcpy(dst, src, sizeof(double) * V);
 }
 /* boundaries1 (grid.cpp:12-125) on a strided line: rows of u at u + i*su,
- * rows of ub at ub + i*sb, with types (left, right) and direction d.  Only the
- * codes reachable from the benchmark set are restated: 0 transmissive,
- * 1 periodic, 2 slip, 3 stick, 5 symmetric. */
+ * rows of ub at ub + i*sb, with types (left, right) and direction d: 0
+ * transmissive, 1 periodic, 2 slip, 3 stick, 4 moving (grid.cpp:44-54,
+ * 100-110), 5 symmetric; DESTRESS = false (options.h:12). */
 static void bc_line(const double *u, long su, double *ub, long sb, int n, int N,
                     const int32_t *bt, int d, int V) {
   for (int side = 0; side < 2; side++) {
@@ -865,6 +865,13 @@ static void bc_line(const double *u, long su, double *ub, long sb, int n, int N,
         row[2] *= -1.;
         row[3] *= -1.;
         row[4] *= -1.;
+      } else if (bt[side] == 4) {
+        const double s = u[src * su + 2];
+        row[2] *= -1.;
+        row[3] *= -1.;
+        row[4] *= -1.;
+        row[2] += 2. * row[0];
+        row[1] += (row[2] * row[2] - s * s) / (2. * row[0]);
       } else if (bt[side] == 5) {
         row[2 + d] *= -1.;
         for (int k = 0; k < 3; k++)

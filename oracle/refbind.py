@@ -238,3 +238,30 @@ class RefSolver:
                              ctypes.c_int(int(stiff)), ctypes.byref(MP), _dp(dts), _dp(secs))
         n = k if k >= 0 else int(np.count_nonzero(secs))
         return u, tt.value, cc.value, dts[:n], secs[:n], (0 if k >= 0 else -1)
+
+
+def _time_run_main(argv):
+    """python oracle/refbind.py STATE.npz: time the reference's step loop on
+    the state in STATE.npz (u, nX, dX, bcs, t, count, nsteps, par as JSON) and
+    print one JSON line {"secs": [...], "status": s}.  Run as a child process
+    so that an abort inside the reference (its Eigen assertions are live)
+    cannot take the caller down."""
+    import json
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from phd_amd.params import Par
+    d = np.load(argv[0], allow_pickle=False)
+    MP = Par()
+    for k, v in json.loads(str(d["par"])).items():
+        setattr(MP, k, v)
+    nX = [int(x) for x in d["nX"]]
+    R = RefSolver(3, int(d["V"]) == 17)
+    _, _, _, _, secs, status = R.run(d["u"], nX, d["dX"], float(d["cfl"]), d["bcs"], 1e300,
+                                     int(d["nsteps"]), MP, t=float(d["t"]),
+                                     count=int(d["count"]))
+    print(json.dumps({"secs": [float(s) for s in secs], "status": int(status)}))
+
+
+if __name__ == "__main__":
+    import sys
+    _time_run_main(sys.argv[1:])

@@ -27,8 +27,9 @@ SYMBOLS = [
     "gpr_source", "gpr_bdot", "gpr_max_abs_eigs", "gpr_thermo_acoustic_tensor", "gpr_d_rus",
     "gpr_bint", "gpr_boundaries", "gpr_extend_mask", "gpr_weno_launcher", "gpr_predictor",
     "gpr_fv_launcher", "gpr_timestep", "gpr_ader_stepper", "gpr_renorm_distortion",
-    "gpr_contorted", "gpr_reset_distortion", "gpr_iterator", "gpr_dev_begin", "gpr_dev_steps",
-    "gpr_dev_status", "gpr_dev_set_timing", "gpr_dev_timing",
+    "gpr_contorted", "gpr_reset_distortion", "gpr_iterator", "gpr_dev_malloc", "gpr_dev_free",
+    "gpr_dev_memcpy", "gpr_dev_begin", "gpr_dev_steps", "gpr_dev_status", "gpr_dev_set_timing",
+    "gpr_dev_timing",
 ]
 
 
@@ -82,6 +83,9 @@ ARGTYPES = {
     "gpr_contorted": [_P, _Ip, _Dp, _l, _d],
     "gpr_reset_distortion": [_P, _Dp, _l, _M],
     "gpr_iterator": [_P, _Dp, _d, _Ip, _Dp, _d, _Ip, _i, _i, _i, _i, _M, _i, _d, _Dp, _Ip, _Lp],
+    "gpr_dev_malloc": [_P, ctypes.POINTER(ctypes.c_void_p), _l],
+    "gpr_dev_free": [_P, _P],
+    "gpr_dev_memcpy": [_P, _P, _P, _l, _i],
     "gpr_dev_begin": [_P, _P, _Ip, _Dp, _d, _Ip, _d, _d, _l, _M],
     "gpr_dev_steps": [_P, _l],
     "gpr_dev_status": [_P, _Dp, _Lp, _Dp, _Ip, _Ip],

@@ -35,6 +35,21 @@ struct Basis {
   double sig[MAXN][MAXN];    // WENO oscillation indicator
 };
 
+// The basis tables for N = 2 and N = 3 in the constant segment, written at
+// context creation.  Kernels index them per lane; the same per-lane index
+// into a table held in a by-value kernel argument would make the compiler
+// copy the whole argument to the stack.
+__constant__ Basis c_basis[2];
+template <int N> __device__ __forceinline__ const Basis &basis_dev() { return c_basis[N - 2]; }
+
+// *p if p is non-null, else v.  The load is atomic so that the compiler cannot
+// merge the two sources into a load through a selected pointer, which would
+// take the address of the by-value kernel argument holding v and copy the
+// whole argument to the stack.
+__device__ __forceinline__ double dev_or(const double *p, double v) {
+  return p ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : v;
+}
+
 // Material parameters plus the few derived scalars the kernels use often.
 struct Mat {
   gpr_par p;

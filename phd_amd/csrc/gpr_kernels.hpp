@@ -50,7 +50,8 @@ template <typename T> __device__ __forceinline__ T wave_max(T v) {
 
 // ---------------------------------------------------------------- boundaries
 // One thread per (ghost-extended cell, variable).  Boundary codes 0
-// transmissive, 1 periodic, 2 slip, 3 stick, 5 symmetric (grid.cpp:5-10).
+// transmissive, 1 periodic, 2 slip, 3 stick, 4 moving, 5 symmetric
+// (grid.cpp:5-10); DESTRESS is false in the reference (options.h:12).
 __device__ __forceinline__ int bc_src(int I, int n, int N, int code, bool &refl) {
   refl = false;
   if (I >= N && I < n + N) return I - N;
@@ -75,6 +76,22 @@ __device__ __forceinline__ double bc_sign(int v, int code, int d) {
   return 1.0;
 }
 
+// Moving wall (code 4, grid.cpp:44-54,100-110): the stick reflection, then
+// rho v_x += 2 rho (a wall sliding at unit speed along x, whatever the wall's
+// direction) and rho E += ((rho v_x)'^2 - (rho v_x)^2) / (2 rho), with
+// (rho v_x) the source cell's value.  g(k) gives the source cell's variable k.
+template <typename G> __device__ __forceinline__ double bc_value(int v, int code, int d, const G &g) {
+  if (code == 4) {
+    if (v == 1 || v == 2) {
+      const double r = g(0), s = g(2);
+      const double m = -s + 2.0 * r;
+      return v == 2 ? m : g(1) + (m * m - s * s) / (2.0 * r);
+    }
+    return (v == 3 || v == 4) ? -g(v) : g(v);
+  }
+  return bc_sign(v, code, d) * g(v);
+}
+
 template <int V, int ND>
 __global__ void k_boundaries(const double *__restrict__ u, double *__restrict__ ub, int nx, int ny, int N,
                              int4 bc) {
@@ -88,15 +105,16 @@ __global__ void k_boundaries(const double *__restrict__ u, double *__restrict__ 
     bool rx, ry = false;
     int codex = I < N ? bc.x : bc.y;
     int i = bc_src(I, nx, N, codex, rx);
-    double sgn = rx ? bc_sign(v, codex, 0) : 1.0;
-    int j = 0;
+    int j = 0, codey = 0;
     if (ND == 2) {
-      int codey = J < N ? bc.z : bc.w;
+      codey = J < N ? bc.z : bc.w;
       j = bc_src(J, ny, N, codey, ry);
-      if (ry) sgn *= bc_sign(v, codey, 1);
     }
-    long src = ND == 2 ? ((long)i * ny + j) * V + v : (long)i * V + v;
-    ub[id] = sgn * u[src];
+    // the x pass (rows of u) and then the y pass over the x-extended rows, as
+    // boundaries2 (grid.cpp:147-170) composes them
+    const double *row = u + (ND == 2 ? (long)i * ny + j : (long)i) * V;
+    auto xval = [&](int k) { return rx ? bc_value(k, codex, 0, [&](int m) { return row[m]; }) : row[k]; };
+    ub[id] = ry ? bc_value(v, codey, 1, xval) : xval(v);
   }
 }
 
@@ -289,45 +307,54 @@ struct FvArgs {
   double dt, dx, dy;
   int nx, ny;
   int post;              // fuse renormalisation, contortion flag and CFL maximum
-  Basis K;
   Mat m;
 };
 
 // Faces: interfs1/interfs2 (fv.cpp:111-160, 162-268).  One lane per (face,
-// face node s); each lane accumulates its N time nodes, then the N lanes of a
-// face are summed with cross-lane shuffles.  Face d = 0 joins extended cells
-// (i, j+1) and (i+1, j+1) for i in [0, nx], j in [0, ny); face d = 1 joins
-// (i+1, j) and (i+1, j+1) for i in [0, nx), j in [0, ny].
+// time node t, face node s): it extrapolates both cells' states to the face
+// node, evaluates one Rusanov flux and one path integral, and weights them;
+// the NT = N * NSF lanes of a face are then summed through LDS.  Face d = 0
+// joins extended cells (i, j+1) and (i+1, j+1) for i in [0, nx], j in
+// [0, ny); face d = 1 joins (i+1, j) and (i+1, j+1) for i in [0, nx),
+// j in [0, ny].
+template <int N, bool TH, int ND> struct FaceShape {
+  static constexpr int NSF = ND == 2 ? N : 1;  // face nodes
+  static constexpr int NT = N * NSF;           // lanes per face
+  static constexpr int FPW = WAVE / NT;        // faces per wavefront
+};
+
 template <int N, bool TH, int ND>
 __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
   constexpr int NN = NS * N;
-  constexpr int NSF = ND == 2 ? N : 1;  // nodes per face
-  constexpr int FPW = WAVE / NSF;       // faces per wavefront
+  constexpr int NSF = FaceShape<N, TH, ND>::NSF;
+  constexpr int NT = FaceShape<N, TH, ND>::NT;
+  constexpr int FPW = FaceShape<N, TH, ND>::FPW;
+  __shared__ double sR[FPW * NT * 2 * V];
+  const Basis &K = basis_dev<N>();
   const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
   const long nfx = (long)(nx + 1) * ny;
   const long nfy = ND == 2 ? (long)nx * (ny + 1) : 0;
-  const double dt = a.st ? a.st->dt : a.dt;
+  const double dt = dev_or(a.st ? &a.st->dt : nullptr, a.dt);
   const int lane = threadIdx.x;
-  const int fl = lane / NSF, sf = lane - fl * NSF;
-  const long id = (long)blockIdx.x * FPW + fl;
+  const int fl = lane / NT, r = lane - fl * NT;
+  const int t = r / NSF, sf = r - t * NSF;
+  const long id0 = (long)blockIdx.x * FPW;
+  const long id = id0 + fl;
   const bool ok = fl < FPW && id < nfx + nfy;
   const int d = id < nfx ? 0 : 1;
   long e0 = 0, e1 = 0;
-  double *out = nullptr;
   if (ok) {
     if (d == 0) {
       int i = id / ny, j = id % ny;
       e0 = ND == 2 ? (long)i * (ny + 2) + (j + 1) : i;
       e1 = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
-      out = a.fx + id * 2 * V;
     } else {
       long k = id - nfx;
       int i = k / (ny + 1), j = k % (ny + 1);
       e0 = (long)(i + 1) * (ny + 2) + j;
       e1 = e0 + 1;
-      out = a.fy + k * 2 * V;
     }
   }
   double lo[V], hi[V];
@@ -338,47 +365,49 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   }
   if (ok && a.mask[e0] && a.mask[e1]) {
     const double h = d == 0 ? a.dx : a.dy;
-    for (int t = 0; t < N; t++) {
-      const double kt = dt * a.K.w[t] / (2.0 * h);
-      double qL[V], qR[V];
-      const double *pl = a.qh + (e0 * NN + t * NS) * V;
-      const double *pr = a.qh + (e1 * NN + t * NS) * V;
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        double l = 0, r = 0;
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-          int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
-          l += a.K.endv[1][k] * pl[nl * V + v];
-          r += a.K.endv[0][k] * pr[nl * V + v];
-        }
-        qL[v] = l;
-        qR[v] = r;
-      }
-      const double kk = ND == 2 ? kt * a.K.w[sf] : kt;
-      riemann_accum<N, TH>(qL, qR, d, a.m, a.K, kk, lo, hi);
-    }
-  }
-  if (NSF > 1) {
+    const double kt = dt * K.w[t] / (2.0 * h);
+    const double kk = ND == 2 ? kt * K.w[sf] : kt;
+    double qL[V], qR[V];
+    const double *pl = a.qh + (e0 * NN + t * NS) * V;
+    const double *pr = a.qh + (e1 * NN + t * NS) * V;
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      double sl = lo[v], sh = hi[v];
+      double l = 0, rr = 0;
 #pragma unroll
-      for (int k = 1; k < NSF; k++) {
-        int src = lane + k < WAVE ? lane + k : lane;
-        sl += __shfl(lo[v], src);
-        sh += __shfl(hi[v], src);
+      for (int k = 0; k < N; k++) {
+        int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
+        l += K.endv[1][k] * pl[nl * V + v];
+        rr += K.endv[0][k] * pr[nl * V + v];
       }
-      lo[v] = sl;
-      hi[v] = sh;
+      qL[v] = l;
+      qR[v] = rr;
     }
+    // the direction as a compile-time constant in each branch, so that the
+    // physics indexes its small arrays statically (no stack frame)
+    if (d == 0)
+      riemann_accum<N, TH>(qL, qR, 0, a.m, K, kk, lo, hi);
+    else
+      riemann_accum<N, TH>(qL, qR, 1, a.m, K, kk, lo, hi);
   }
-  if (ok && sf == 0) {
+  if (fl < FPW) {
+    double *my = sR + (fl * NT + r) * 2 * V;
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      out[v] = lo[v];
-      out[V + v] = hi[v];
+      my[v] = lo[v];
+      my[V + v] = hi[v];
     }
+  }
+  __syncthreads();
+  // one lane per (face, output): sum the face's NT lanes in a fixed order
+  for (int o = lane; o < FPW * 2 * V; o += WAVE) {
+    const int f = o / (2 * V), k = o - f * 2 * V;
+    const long fid = id0 + f;
+    if (fid >= nfx + nfy) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT; q++) acc += sR[(f * NT + q) * 2 * V + k];
+    double *out = fid < nfx ? a.fx + fid * 2 * V : a.fy + (fid - nfx) * 2 * V;
+    out[k] = acc;
   }
 }
 
@@ -393,7 +422,7 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   constexpr int NN = PredShape<N, ND>::NN;
   constexpr int CPW = PredShape<N, ND>::CPW;
   __shared__ double sQ[CPW * NN * V];
-  const Basis &K = a.K;
+  const Basis &K = basis_dev<N>();
   const int lane = threadIdx.x;
   const int slot = lane / NN;
   const int node = lane - slot * NN;
@@ -409,7 +438,7 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   const int ci = valid ? (ND == 2 ? c / ny : c) : 0, cj = valid ? (ND == 2 ? c % ny : 0) : 0;
   const long e = ND == 2 ? (long)(ci + 1) * (ny + 2) + (cj + 1) : ci + 1;
   const bool active = valid && a.mask[e];
-  const double dt = a.st ? a.st->dt : a.dt;
+  const double dt = dev_or(a.st ? &a.st->dt : nullptr, a.dt);
   const int sl = lane_ok ? slot : 0;
   double *myQ = sQ + sl * NN * V;
 
@@ -495,6 +524,7 @@ __global__ __launch_bounds__(256) void k_post(double *u, long ncell, double dx, 
     for (int v = 0; v < V; v++)
       if (!(q[v] - q[v] == 0.0)) nonfinite = 1;
     double sp = 0;
+#pragma unroll
     for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
     if (sp > local_max) local_max = sp;
 #pragma unroll
@@ -520,6 +550,7 @@ __global__ __launch_bounds__(256) void k_cfl(const double *u, long ncell, double
 #pragma unroll
     for (int v = 0; v < V; v++) q[v] = u[c * V + v];
     double sp = 0;
+#pragma unroll
     for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
     if (sp > local_max) local_max = sp;
   }
@@ -553,6 +584,7 @@ __global__ __launch_bounds__(256) void k_reset(double *u, long ncell, double dx,
 #pragma unroll
     for (int v = 0; v < V; v++) u[c * V + v] = q[v];
     double sp = 0;
+#pragma unroll
     for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
     if (sp > local_max) local_max = sp;
   }

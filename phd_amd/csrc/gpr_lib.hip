@@ -296,7 +296,6 @@ template <int N, bool TH, int ND> struct WenoOp {
 template <int N, bool TH, int ND> struct PredictorOp {
   static int run(gpr_ctx *c, PredArgs a) {
     constexpr int CPW = PredShape<N, ND>::CPW;
-    a.K = basis(N);
     long blocks = (a.ncell + CPW - 1) / CPW;
     hipLaunchKernelGGL((k_predictor<N, TH, ND>), dim3(blocks), dim3(WAVE), 0, c->stream, a);
     HIP_TRY(hipGetLastError());
@@ -306,9 +305,8 @@ template <int N, bool TH, int ND> struct PredictorOp {
 
 template <int N, bool TH, int ND> struct FvOp {
   static int run(gpr_ctx *c, FvArgs a) {
-    a.K = basis(N);
     long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
-    constexpr int FPW = WAVE / (ND == 2 ? N : 1);
+    constexpr int FPW = FaceShape<N, TH, ND>::FPW;
     hipLaunchKernelGGL((k_fv_faces<N, TH, ND>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
     long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
     constexpr int CPW = PredShape<N, ND>::CPW;
@@ -494,8 +492,8 @@ int check_bcs(const gpr_ctx *c, const int32_t *bcs, int4 &bc) {
   if (!bcs) return fail(GPR_E_ARG, "null boundary codes");
   int v[4] = {0, 0, 0, 0};
   for (int i = 0; i < 2 * c->ndim; i++) {
-    if (bcs[i] < 0 || bcs[i] > 5 || bcs[i] == 4)
-      return fail(GPR_E_UNSUPPORTED, "boundary code must be 0, 1, 2, 3 or 5");
+    if (bcs[i] < 0 || bcs[i] > 5)
+      return fail(GPR_E_ARG, "boundary code must be 0..5 (grid.cpp:5-10)");
     v[i] = bcs[i];
   }
   bc = make_int4(v[0], v[1], v[2], v[3]);
@@ -573,6 +571,12 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_st, sizeof(StepState));
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_fail, sizeof(int) * 4);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_itc, sizeof(unsigned long long) * 4);
+  if (e == hipSuccess) {
+    // basis tables of the constant segment on this device (same values for
+    // every context, so concurrent writers agree)
+    const Basis hb[2] = {basis(2), basis(3)};
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_basis), hb, sizeof(hb));
+  }
   if (e != hipSuccess) {
     int rc = fail(GPR_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
     gpr_ctx_destroy(c);
@@ -899,6 +903,31 @@ int gpr_ader_stepper(gpr_ctx *c, double *u, const double *ub, const int32_t *nX,
   HIP_TRY(hipMemcpyAsync(&nfail, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (nfail) return fail(GPR_E_PREDICTOR, std::to_string(nfail) + " cell(s): predictor converged in neither phase");
+  return GPR_OK;
+}
+
+// ------------------------------------------------------------ device memory
+int gpr_dev_malloc(gpr_ctx *c, void **d, int64_t bytes) {
+  if (!c || !d || bytes <= 0) return fail(GPR_E_ARG, "gpr_dev_malloc: bad argument");
+  *d = nullptr;
+  HIP_TRY(hipMalloc(d, (size_t)bytes));
+  return GPR_OK;
+}
+
+int gpr_dev_free(gpr_ctx *c, void *d) {
+  if (!c) return fail(GPR_E_ARG, "null ctx");
+  if (d) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(d));
+  }
+  return GPR_OK;
+}
+
+int gpr_dev_memcpy(gpr_ctx *c, void *dst, const void *src, int64_t bytes, int kind) {
+  if (!c || !dst || !src || bytes < 0 || kind < 0 || kind > 2) return fail(GPR_E_ARG, "gpr_dev_memcpy: bad argument");
+  const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+  HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, k[kind], c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return GPR_OK;
 }
 

@@ -43,7 +43,6 @@ struct PredArgs {
   long ncell;
   int dg_it, stiff_it, bailout, mode;  // mode 0: Picard then fallback; 1: fallback only
   double dg_tol, stiff_tol;
-  Basis K;
   Mat m;
 };
 
@@ -62,6 +61,7 @@ template <int N, int ND> struct PredShape {
 // the thermal source -(c_alpha^2 T / (kappa rho)) rho J has rate rJ.
 template <int N, bool TH>
 __device__ void stiff_precond(const double *ws, const PredArgs &a, double dt, double *out) {
+  const Basis &K = basis_dev<N>();
   NodeState<double, TH> s;
   node_state<double, TH>(ws, a.m, s);
   double dA = det3(ws + 5);
@@ -78,7 +78,7 @@ __device__ void stiff_precond(const double *ws, const PredArgs &a, double dt, do
     double M[N][2 * N];
     for (int i = 0; i < N; i++)
       for (int j = 0; j < 2 * N; j++)
-        M[i][j] = j < N ? a.K.dgmat[i][j] + (i == j ? dt * rate * a.K.w[i] : 0.0) : (j - N == i ? 1.0 : 0.0);
+        M[i][j] = j < N ? K.dgmat[i][j] + (i == j ? dt * rate * K.w[i] : 0.0) : (j - N == i ? 1.0 : 0.0);
     for (int c = 0; c < N; c++) {
       int piv = c;
       for (int r = c + 1; r < N; r++)
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
   __shared__ double sPre[CPW * NS * PRE];
   __shared__ double sRat[WAVE];
 
-  const Basis &K = a.K;
+  const Basis &K = basis_dev<N>();
   const int lane = threadIdx.x;
   const int slot = lane / NN;
   const int node = lane - slot * NN;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
   double *myPre = sPre + (sl * NS + s) * PRE;
   const double *wsrc = a.wh + (cell * NS + s) * V;
 
-  const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+  const double dt = dev_or(a.dt_dev, a.dt);
   const double Ws = ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib];
   const double iWs = 1.0 / Ws;
   const double Wt = K.w[t];

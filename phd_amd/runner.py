@@ -1,8 +1,9 @@
 """Device-resident time loop: the state stays in HBM between steps and dt is
 computed and consumed on the device (no host synchronisation per step).
 
-PyTorch only supplies the device allocation of u; every step runs in
-libgpr_mi355x.so on the context's HIP stream.
+The state buffer is allocated by libgpr_mi355x.so itself (gpr_dev_malloc)
+and every step runs in the library on the context's HIP stream.  Nothing
+here initialises another HIP runtime in the process (PyTorch ships its own).
 """
 
 import ctypes
@@ -17,8 +18,7 @@ PHASES = ("boundaries", "weno", "predictor", "fv", "post")
 
 class DeviceRun:
     def __init__(self, u0, MP, nX, dX, bcs, CFL=0.8, tf=float("inf"), N=3, thermal=None,
-                 ctx=None, **opts):
-        import torch
+                 ctx=None, t0=0.0, count0=0, **opts):
         nX = [int(n) for n in nX]
         ndim = len(nX)
         V = u0.size // int(np.prod(nX))
@@ -33,12 +33,16 @@ class DeviceRun:
         self.MP = MP
         self.V = V
         self.ncell = int(np.prod(nX))
-        self.u = torch.from_numpy(np.ascontiguousarray(u0, dtype=np.float64).ravel()).to("cuda")
-        torch.cuda.synchronize()
         self.L = _lib.load()
-        check(self.L.gpr_dev_begin(self.ctx.h, ctypes.c_void_p(self.u.data_ptr()), iptr(self.nX),
-                                   dptr(self.dX), float(CFL), iptr(self.bcs), 0.0, float(tf), 0,
-                                   par_ref(MP)))
+        host = np.ascontiguousarray(u0, dtype=np.float64).ravel()
+        self.nbytes = host.nbytes
+        self.d_u = ctypes.c_void_p()
+        check(self.L.gpr_dev_malloc(self.ctx.h, ctypes.byref(self.d_u), self.nbytes))
+        check(self.L.gpr_dev_memcpy(self.ctx.h, self.d_u, host.ctypes.data_as(ctypes.c_void_p),
+                                    self.nbytes, 0))
+        check(self.L.gpr_dev_begin(self.ctx.h, self.d_u, iptr(self.nX),
+                                   dptr(self.dX), float(CFL), iptr(self.bcs), float(t0),
+                                   float(tf), int(count0), par_ref(MP)))
 
     @property
     def stream_handle(self):
@@ -74,10 +78,21 @@ class DeviceRun:
                 "fallback_cells": int(iters[2]), "cells": cells.value}
 
     def host(self):
-        import torch
-        self.sync()  # the library's stream first, then torch's copy
-        torch.cuda.synchronize()
-        return self.u.cpu().numpy().copy()
+        out = np.empty(self.nbytes // 8)
+        check(self.L.gpr_dev_memcpy(self.ctx.h, out.ctypes.data_as(ctypes.c_void_p), self.d_u,
+                                    self.nbytes, 1))
+        return out
 
     def sync(self):
         self.status()
+
+    def close(self):
+        if self.d_u:
+            check(self.L.gpr_dev_free(self.ctx.h, self.d_u))
+            self.d_u = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -168,6 +168,72 @@ def stiff():
          wh=wh, mask=mask, dt=np.array(dt), t=np.array(t), count=np.array(c), qh_nk=qh_nk)
 
 
+BC_SETS = [(0, 0, 1, 1), (1, 1, 0, 0), (2, 3, 5, 2), (5, 5, 3, 0), (4, 4, 4, 4), (3, 4, 4, 2)]
+
+
+def boundaries():
+    """boundaries (grid.cpp:12-170) for every code, 1D and 2D, random states."""
+    out = {}
+    for thermal in (False, True):
+        R = RefSolver(3, thermal)
+        MP = mats(thermal)["sg_fluid"]
+        tag = f"v{R.V}"
+        u2 = problems.random_states(7 * 5, MP, thermal=thermal, seed=51).ravel()
+        u1 = u2[: 7 * R.V]
+        out[f"{tag}_u2"] = u2
+        out[f"{tag}_u1"] = u1
+        for k, b in enumerate(BC_SETS):
+            out[f"{tag}_ub2_{k}"] = R.boundaries(u2, [7, 5], list(b))
+            out[f"{tag}_ub1_{k}"] = R.boundaries(u1, [7], list(b[:2]))
+    out["bcs"] = np.array(BC_SETS)
+    save("boundaries.npz", **out)
+
+
+def post():
+    """renorm_distortion, contorted and reset_distortion (utils.cpp:6-85) on
+    states with perturbed distortion, and steps with a contortion threshold
+    low enough that the global reset runs (iterator.cpp:145-151)."""
+    out = {}
+    for thermal in (False, True):
+        R = RefSolver(3, thermal)
+        MP = mats(thermal)["sg_fluid"]
+        rng = np.random.default_rng(41)
+        Q = problems.random_states(32, MP, thermal=thermal, seed=42)
+        A = Q[:, 5:14].reshape(-1, 3, 3)
+        Q[:, 5:14] = (A + 0.05 * rng.standard_normal(A.shape)).reshape(-1, 9)
+        tag = f"v{R.V}"
+        out[f"{tag}_par"] = np.array(par_json(MP))
+        out[f"{tag}_Q"] = Q
+        out[f"{tag}_renorm"] = R.renorm_distortion(Q.ravel(), MP)
+        out[f"{tag}_reset"] = R.reset_distortion(Q.ravel(), MP)
+        tols = np.array([1e-4, 1e-3, 1e-2, 1e-1, 1.0])
+        out[f"{tag}_tols"] = tols
+        out[f"{tag}_contorted"] = np.array([R.contorted(Q.ravel(), t) for t in tols])
+        # per-state contortion flags: each state alone
+        out[f"{tag}_contorted_each"] = np.array([[R.contorted(q, t) for t in tols] for q in Q])
+    R = RefSolver(3, False)
+    R2 = RefSolver(3, False, nofma=True)
+    u, MP, tf, dX, bcs = problems.double_shear_2d(12, μ=0.1)
+    ua, t, c = u.ravel(), 0.0, 0
+    steps, steps2, dts = [], [], []
+    for _ in range(3):
+        u1, t1, c1, d1, _, st = R.run(ua, [12, 12], dX, 0.8, bcs, 1e9, 1, MP, t=t, count=c,
+                                      contorted_tol=1e-9)
+        u2, _, _, _, _, _ = R2.run(ua, [12, 12], dX, 0.8, bcs, 1e9, 1, MP, t=t, count=c,
+                                   contorted_tol=1e-9)
+        assert st == 0
+        steps.append(u1)
+        steps2.append(u2)
+        dts.append(d1[0])
+        ua, t, c = u1, t1, c1
+    out["reset_run_u0"] = u.ravel()
+    out["reset_run_steps"] = np.array(steps)
+    out["reset_run_steps_nofma"] = np.array(steps2)
+    out["reset_run_dts"] = np.array(dts)
+    out["reset_run_par"] = np.array(par_json(MP))
+    save("post.npz", **out)
+
+
 if __name__ == "__main__":
     constants()
     pointwise()
@@ -175,3 +241,5 @@ if __name__ == "__main__":
     stages("2d_dsl12", False, 2, 12, 0.1, 3)
     stages("1d_sod48_th", True, 1, 48, 1e-2, 4)
     stiff()
+    post()
+    boundaries()

@@ -179,3 +179,56 @@ def test_stiff_fallback_vs_reference_newton_krylov():
     On = oracle(3, True, 1, dg_tol=1e-14, stiff_tol=1e-14, stiff_it=400, fallback=1, dg_it=1)
     qn = On.predictor(wh, dt, dX, nX, MP, mask)
     assert relerr(qn, qt) < 1e-10
+
+
+@pytest.mark.parametrize("V", [14, 17])
+def test_post_step_vs_reference(V):
+    g = load("post.npz")
+    tag = f"v{V}"
+    O = oracle(3, V == 17)
+    MP = par(g[f"{tag}_par"])
+    Q = g[f"{tag}_Q"]
+    assert relerr(O.renorm_distortion(Q.ravel(), MP), g[f"{tag}_renorm"]) < 1e-14
+    assert relerr(O.reset_distortion(Q.ravel(), MP), g[f"{tag}_reset"]) < 1e-14
+    for k, tol in enumerate(g[f"{tag}_tols"]):
+        assert O.contorted(Q.ravel(), tol) == bool(g[f"{tag}_contorted"][k])
+        for i, q in enumerate(Q):
+            assert O.contorted(q, tol) == bool(g[f"{tag}_contorted_each"][i, k])
+
+
+def test_steps_with_global_reset_vs_reference():
+    """A contortion threshold of 1e-9 makes the steps end in the global
+    distortion reset (iterator.cpp:145-151).  The reset makes the distortion
+    field uniform up to rounding, where the WENO smoothness indicators cancel;
+    the bar is again the reference's own FMA / no-FMA gap."""
+    g = load("post.npz")
+    O = oracle(3, False, 2)
+    MP = par(g["reset_run_par"])
+    u, t, c = g["reset_run_u0"], 0.0, 0
+    nX = np.array([12, 12])
+    for k in range(len(g["reset_run_dts"])):
+        uo, t, c, dt, rc, st = O.step(u, nX, [1 / 12, 1 / 12], 0.8, [1, 1, 1, 1], t, c, 1e9, MP,
+                                      contorted_tol=1e-9)
+        assert rc == 0 and relerr(dt, g["reset_run_dts"][k]) < 1e-12
+        ref = g["reset_run_steps"][k]
+        floor = relerr(g["reset_run_steps_nofma"][k], ref)
+        assert relerr(uo, ref) <= max(1e-12, 4 * floor), (k, floor)
+        u = ref.copy()
+
+
+@pytest.mark.parametrize("V", [14, 17])
+def test_boundaries_vs_reference(V):
+    """Every boundary code, 1D and 2D (grid.cpp:12-170); the moving wall's
+    energy term is the only arithmetic, so copies are exact and it is within
+    an ulp."""
+    g = load("boundaries.npz")
+    tag = f"v{V}"
+    O1, O2 = oracle(3, V == 17, 1), oracle(3, V == 17, 2)
+    for k, b in enumerate(g["bcs"]):
+        for O, nX, key in ((O2, [7, 5], "2"), (O1, [7], "1")):
+            ub = O.boundaries(g[f"{tag}_u{key}"], nX, b[: 2 * len(nX)])
+            ref = g[f"{tag}_ub{key}_{k}"]
+            if 4 in b[: 2 * len(nX)]:
+                assert relerr(ub, ref) < 1e-15, (k, key)
+            else:
+                assert np.array_equal(ub, ref), (k, key)
