@@ -125,6 +125,88 @@ def copy_probe(mib):
     _lib.check(L.gpr_dev_free(ctx.h, b))
 
 
+def fp32_bulk_leg(n, steps):
+    """The FP32-bulk form against the plain FP64 form on the elastoplastic
+    solid (Godunov-Romenski EOS, power-law plastic relaxation, Gaussian
+    velocity pulse, transmissive, phd_amd.problems.plastic_pulse_2d): both run
+    `steps` full steps from the initial state; throughput of each and the
+    FP32 form's error against the FP64 run per variable."""
+    from phd_amd import problems
+    from phd_amd.runner import DeviceRun
+    u0, MP, _, dX, bcs = problems.plastic_pulse_2d(n)
+    out = {"config": f"2D elastoplastic solid {n}x{n}, GR EOS, power-law plasticity "
+                     f"(sigmaY 0.4, tau0 1, n 10), {steps} steps from t = 0, CFL 0.8, P2",
+           "cells": n * n, "steps": steps}
+    states = {}
+    for name, prec in (("fp64", 0), ("fp32_bulk", 1)):
+        run = DeviceRun(u0.ravel(), MP, [n, n], dX, bcs, CFL=0.8, bailout=5, precision=prec)
+        sec, tm = timed_run(run, steps)
+        out[name] = {"value": n * n * steps / sec, "ms_per_step": 1e3 * sec / steps,
+                     "phase_ms_per_step": {k: v / tm["steps"] for k, v in tm["ms"].items()},
+                     "fallback_cells": tm["fallback_cells"]}
+        states[name] = run.host().reshape(-1, 14)
+        run.close()
+    a, b = states["fp32_bulk"], states["fp64"]
+    scale = np.max(np.abs(b), axis=0)
+    scale[scale == 0] = 1.0
+    names = ["rho", "rhoE", "rho_vx", "rho_vy", "rho_vz"] + [f"A{i}{j}" for i in range(3)
+                                                               for j in range(3)]
+    out["error_vs_fp64"] = {
+        "norm": "per variable, mean |a-b| (L1) and max |a-b| (Linf) over cells, divided by "
+                "max |fp64 value| of that variable",
+        "L1": {k: float(v) for k, v in zip(names, np.mean(np.abs(a - b), axis=0) / scale)},
+        "Linf": {k: float(v) for k, v in zip(names, np.max(np.abs(a - b), axis=0) / scale)}}
+    out["speedup"] = out["fp32_bulk"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def batched_leg(count, seed, sample):
+    """The batched form against the plain form on independent 1D shock tubes
+    (phd_amd.problems.shock_tube_batch: nx uniform in [100, 4000], mu
+    log-uniform in [1e-5, 1e-1], left/right rho in [0.1, 2], p in [0.1, 10],
+    gamma in [1.3, 1.67], each to its own tf = 0.2).  The batch runs every
+    instance to its end in one graph-captured loop after the tuner picked its
+    launch configuration; the plain form runs `sample` of the same instances
+    one after another through the single-domain device loop."""
+    from phd_amd import problems
+    from phd_amd.batch import Batch
+    from phd_amd.runner import DeviceRun
+    inst = problems.shock_tube_batch(count, seed=seed)
+    b = Batch(inst, bailout=5)
+    tune = b.autotune()
+    b.reset()
+    tic = time.perf_counter()
+    graphs, running = b.run()
+    sec = time.perf_counter() - tic
+    st = b.status()
+    if running or st["nonfinite"].any() or st["pred_fail"]:
+        raise SystemExit(f"batched run failed: running {running}, status {st}")
+    work = b.cell_updates()
+    out = {"config": f"{count} independent 1D GPR shock tubes (seed {seed}), nx in [100, 4000], "
+                     "each to tf = 0.2, FP64, P2, CFL 0.8",
+           "instances": count, "cells": int(b.nx.sum()), "cell_updates": work,
+           "steps_max": int(st["count"].max()), "steps_min": int(st["count"].min()),
+           "value": work / sec, "seconds": sec, "graph_replays": graphs,
+           "tuned": tune["choice"], "tuner_trials": tune["trials"]}
+    b.close()
+    # the plain form on a sample of the same instances, one at a time
+    pw, ps = 0, 0.0
+    for u0, MP, tf, dX, bcs in inst[:sample]:
+        r = DeviceRun(np.asarray(u0).ravel(), MP, [u0.shape[0]], dX, bcs, CFL=0.8, tf=tf,
+                      bailout=5)
+        r.sync()
+        tic = time.perf_counter()
+        while r.status()["t"] < tf:
+            r.steps(16)
+        ps += time.perf_counter() - tic
+        pw += u0.shape[0] * r.status()["count"]
+        r.close()
+    out["plain_sample"] = {"instances": sample, "value": pw / ps,
+                           "note": "16 steps enqueued per host check; steps past tf do no work"}
+    out["speedup"] = out["value"] / out["plain_sample"]["value"]
+    return out
+
+
 def timed_run(run, steps):
     """K steps bracketed by a synchronisation of the library's stream (all
     device work of this process runs on it); HIP-event phase timing on the
@@ -154,6 +236,12 @@ def main():
     ap.add_argument("--no-faithful", action="store_true",
                     help="skip the second timed run with bailout = 0")
     ap.add_argument("--bailout", type=int, default=5)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-sample", type=int, default=8)
+    ap.add_argument("--fp32-n", type=int, default=1024)
+    ap.add_argument("--fp32-steps", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the timed region, run one plain device copy of MIB MiB "
                          "(a known-size kernel to scale HBM counters against)")
@@ -221,6 +309,10 @@ def main():
         out["bailout0_value"] = cells * args.steps / sec0
         out["bailout0_ms_per_step"] = 1e3 * sec0 / args.steps
         del run0
+    if not args.no_fp32:
+        out["fp32_bulk"] = fp32_bulk_leg(args.fp32_n, args.fp32_steps)
+    if not args.no_batch:
+        out["batched"] = batched_leg(args.batch_count, 0, args.batch_sample)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,

@@ -52,6 +52,9 @@ typedef struct gpr_solver_opts {
   int32_t mode;       /* 0: Picard, then the fallback for cells it leaves;
                          1: the preconditioned iteration for every cell */
   double contorted_tol; /* reset threshold of iterator(), its contorted_tol argument */
+  int32_t precision;  /* device loop only (gpr_dev_steps): 0 FP64 throughout;
+                         1 FP32 bulk: WENO and Picard in FP32, the stiff
+                         fallback, FV accumulation, post-step and CFL in FP64 */
 } gpr_solver_opts;
 
 const char *gpr_last_error(void);
@@ -161,6 +164,34 @@ int gpr_dev_set_timing(gpr_ctx *ctx, int enable);
  * fallback cells summed over the predicted cells; cells = predicted cells
  * summed over the steps. */
 int gpr_dev_timing(gpr_ctx *ctx, double *ms, int64_t *steps, int64_t *iters, int64_t *cells);
+
+/* --- batches of independent 1D problems ------------------------------------
+ * No single reference counterpart: each instance is what iterator() does for
+ * one 1D problem (iterator.cpp:85-170, one material, SPLIT = 0, FLUX = 0).
+ * The context must be 1D.  Instance k has nx[k] cells of width dx[k],
+ * boundary codes bcs[2k], bcs[2k+1], material MPs[k] and end time tf[k]; the
+ * state of all instances is one array, the instances' [nx[k]][V] blocks in
+ * order.  Every stage of a step is one launch over all instances, each with
+ * its own dt (CFL, the 0.2 ramp of the first 6 steps, the clamp to its tf);
+ * an instance stops at its tf or when its state becomes non-finite. */
+int gpr_batch_create(gpr_ctx *ctx, int count, const int32_t *nx, const double *dx, const int32_t *bcs,
+                     const gpr_par *MPs, const double *tf, double CFL);
+int gpr_batch_destroy(gpr_ctx *ctx);
+/* host state in (t = 0, step count 0, first dt on the device) / out */
+int gpr_batch_set_state(gpr_ctx *ctx, const double *u);
+int gpr_batch_get_state(gpr_ctx *ctx, double *u);
+/* block size of the element-wise kernels (a multiple of 64, <= 1024) */
+int gpr_batch_set_block(gpr_ctx *ctx, int block);
+/* enqueue nsteps steps on the context's stream */
+int gpr_batch_steps(gpr_ctx *ctx, int64_t nsteps);
+/* capture steps_per_graph steps in a hipGraph (kept until the next change of
+ * steps_per_graph or block) and replay it until every instance has stopped or
+ * max_graphs replays ran; *graphs_run replays, *running instances left */
+int gpr_batch_run(gpr_ctx *ctx, int steps_per_graph, int64_t max_graphs, int64_t *graphs_run, int32_t *running);
+/* per-instance t, step count, next dt, stopped and non-finite flags (arrays of
+ * count; any may be null) and the predictor's failed-cell count */
+int gpr_batch_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t *done, int32_t *nonfinite,
+                     int32_t *pred_fail);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,9 @@ SYMBOLS = [
     "gpr_fv_launcher", "gpr_timestep", "gpr_ader_stepper", "gpr_renorm_distortion",
     "gpr_contorted", "gpr_reset_distortion", "gpr_iterator", "gpr_dev_malloc", "gpr_dev_free",
     "gpr_dev_memcpy", "gpr_dev_begin", "gpr_dev_steps", "gpr_dev_status", "gpr_dev_set_timing",
-    "gpr_dev_timing",
+    "gpr_dev_timing", "gpr_batch_create", "gpr_batch_destroy", "gpr_batch_set_state",
+    "gpr_batch_get_state", "gpr_batch_set_block", "gpr_batch_steps", "gpr_batch_run",
+    "gpr_batch_status",
 ]
 
 
@@ -43,7 +45,7 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("dg_it", ctypes.c_int32), ("dg_tol", ctypes.c_double),
                 ("stiff_it", ctypes.c_int32), ("stiff_tol", ctypes.c_double),
                 ("bailout", ctypes.c_int32), ("mode", ctypes.c_int32),
-                ("contorted_tol", ctypes.c_double)]
+                ("contorted_tol", ctypes.c_double), ("precision", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p
@@ -91,6 +93,14 @@ ARGTYPES = {
     "gpr_dev_status": [_P, _Dp, _Lp, _Dp, _Ip, _Ip],
     "gpr_dev_set_timing": [_P, _i],
     "gpr_dev_timing": [_P, _Dp, _Lp, _Lp, _Lp],
+    "gpr_batch_create": [_P, _i, _Ip, _Dp, _Ip, _M, _Dp, _d],
+    "gpr_batch_destroy": [_P],
+    "gpr_batch_set_state": [_P, _Dp],
+    "gpr_batch_get_state": [_P, _Dp],
+    "gpr_batch_set_block": [_P, _i],
+    "gpr_batch_steps": [_P, _l],
+    "gpr_batch_run": [_P, _i, _l, _Lp, _Ip],
+    "gpr_batch_status": [_P, _Dp, _Lp, _Dp, _Ip, _Ip, _Ip],
 }
 
 _lib = None
@@ -124,6 +134,7 @@ def check(rc):
 
 D = ctypes.POINTER(ctypes.c_double)
 I32 = ctypes.POINTER(ctypes.c_int32)
+L64 = ctypes.POINTER(ctypes.c_int64)
 U8 = ctypes.POINTER(ctypes.c_uint8)
 
 

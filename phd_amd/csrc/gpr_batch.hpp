@@ -1,0 +1,331 @@
+// A batch of independent 1D GPR problems (uneven sizes, own material, own dt,
+// own end time) advanced together: each stage of the step is one launch over
+// every instance, and the step loop is captured in a hipGraph by the host.
+//
+// Layout: the instances' arrays are concatenated (BatchInst offsets).  Kernels
+// that give a wavefront a group of cells (predictor, cell update) or faces
+// never mix instances, so a wavefront's material, dx and dt are uniform and
+// come from the instance's BatchInst / BatchState; the element-wise kernels
+// find their instance by binary search over the offsets.
+#pragma once
+
+#include "gpr_kernels.hpp"
+
+namespace gpr {
+
+struct BatchArgs {
+  const BatchInst *inst;
+  BatchState *st;
+  int count;
+  const long *u0s, *b0s, *e0s;  // instance offsets (count + 1 entries, last = total)
+  const int *cblk, *fblk;       // cell-update and face block -> instance
+  double *u;                    // [sum nx][V]
+  double *ub;                   // [sum (nx + 2N)][V]
+  double *wh;                   // [sum (nx + 2)][N][V]
+  double *qh;                   // [sum (nx + 2)][N^2][V]
+  double *fx;                   // [sum (nx + 1)][2][V]
+};
+
+// index k with starts[k] <= x < starts[k + 1]
+__device__ __forceinline__ int find_inst(const long *starts, int count, long x) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// boundaries1 (grid.cpp:12-145) per instance: one thread per (row of ub, variable)
+template <int N, int V>
+__global__ void kb_boundaries(BatchArgs b) {
+  const long total = b.b0s[b.count] * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    const int v = id % V;
+    const long row = id / V;
+    const int k = find_inst(b.b0s, b.count, row);
+    const BatchInst &I = b.inst[k];
+    if (b.st[k].done) continue;
+    const int Irow = (int)(row - I.b0);
+    bool refl;
+    const int code = Irow < N ? I.bc0 : I.bc1;
+    const int i = bc_src(Irow, I.nx, N, code, refl);
+    const double *src = b.u + (I.u0 + i) * V;
+    b.ub[id] = refl ? bc_value(v, code, 0, [&](int m) { return src[m]; }) : src[v];
+  }
+}
+
+// weno_launcher in 1D (weno.cpp:145-147): one thread per (extended cell, variable)
+template <int N, int V>
+__global__ void kb_weno(BatchArgs b) {
+  const Basis &K = basis_dev<N>();
+  const long total = b.e0s[b.count] * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    const int v = id % V;
+    const long e = id / V;
+    const int k = find_inst(b.e0s, b.count, e);
+    const BatchInst &I = b.inst[k];
+    if (b.st[k].done) continue;
+    const int i = (int)(e - I.e0);
+    double d[2 * N - 1], o[N];
+#pragma unroll
+    for (int r = 0; r < 2 * N - 1; r++) d[r] = b.ub[(I.b0 + i + r) * V + v];
+    weno_point<double, N>(d, K, I.nx + 2, 0, i, o);
+#pragma unroll
+    for (int a = 0; a < N; a++) b.wh[(e * N + a) * V + v] = o[a];
+  }
+}
+
+// interfs1 (fv.cpp:111-160): one lane per (face, time node), as k_fv_faces
+template <int N, bool TH>
+__global__ __launch_bounds__(64) void kb_faces(BatchArgs b) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int NT = N, FPW = WAVE / NT, NN = N * N;
+  __shared__ double sR[FPW * NT * 2 * V];
+  const Basis &K = basis_dev<N>();
+  const int k = b.fblk[blockIdx.x];
+  const BatchInst &I = b.inst[k];
+  if (b.st[k].done) return;  // uniform over the block
+  const int lane = threadIdx.x;
+  const int fl = lane / NT, t = lane - fl * NT;
+  const long lf0 = (long)(blockIdx.x - I.blk_f) * FPW;
+  const long lf = lf0 + fl;
+  const bool ok = fl < FPW && lf < I.nx + 1;
+  const double dt = b.st[k].dt;
+  double lo[V], hi[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    lo[v] = 0.0;
+    hi[v] = 0.0;
+  }
+  if (ok) {
+    const long e0 = I.e0 + lf, e1 = e0 + 1;
+    const double kk = dt * K.w[t] / (2.0 * I.dx);
+    double qL[V], qR[V];
+    const double *pl = b.qh + (e0 * NN + t * N) * V;
+    const double *pr = b.qh + (e1 * NN + t * N) * V;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double l = 0, rr = 0;
+#pragma unroll
+      for (int a = 0; a < N; a++) {
+        l += K.endv[1][a] * pl[a * V + v];
+        rr += K.endv[0][a] * pr[a * V + v];
+      }
+      qL[v] = l;
+      qR[v] = rr;
+    }
+    riemann_accum<N, TH>(qL, qR, 0, I.m, K, kk, lo, hi);
+  }
+  if (fl < FPW) {
+    double *my = sR + (fl * NT + t) * 2 * V;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      my[v] = lo[v];
+      my[V + v] = hi[v];
+    }
+  }
+  __syncthreads();
+  for (int o = lane; o < FPW * 2 * V; o += WAVE) {
+    const int f = o / (2 * V), c = o - f * 2 * V;
+    if (lf0 + f >= I.nx + 1) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT; q++) acc += sR[(f * NT + q) * 2 * V + c];
+    b.fx[(I.f0 + lf0 + f) * 2 * V + c] = acc;
+  }
+}
+
+// centers1 (fv.cpp:11-54) and the face terms: one lane per space-time node,
+// as k_fv_cells in 1D
+template <int N, bool TH>
+__global__ __launch_bounds__(64) void kb_cells(BatchArgs b) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int NS = N, NN = N * N, CPW = WAVE / NN;
+  __shared__ double sQ[CPW * NN * V];
+  const Basis &K = basis_dev<N>();
+  const int k = b.cblk[blockIdx.x];
+  const BatchInst &I = b.inst[k];
+  if (b.st[k].done) return;  // uniform over the block
+  const int lane = threadIdx.x;
+  const int slot = lane / NN;
+  const int node = lane - slot * NN;
+  const int t = node / NS, ia = node - t * NS;
+  const bool lane_ok = slot < CPW;
+  const long lc = (long)(blockIdx.x - I.blk_c) * CPW + slot;
+  const bool valid = lane_ok && lc < I.nx;
+  const long e = I.e0 + lc + 1;
+  const double dt = b.st[k].dt;
+  const int sl = lane_ok ? slot : 0;
+  double *myQ = sQ + sl * NN * V;
+  double q[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) q[v] = valid ? b.qh[(e * NN + node) * V + v] : (v == 0 ? 1.0 : 0.0);
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myQ[node * V + v] = q[v];
+  __syncthreads();
+  double S[V];
+  {
+    NodeState<double, TH> st;
+    node_state<double, TH>(q, I.m, st);
+    source<double, TH>(q, st, I.m, S);
+    double dq[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < N; j++) acc += K.der[ia][j] * myQ[(t * NS + j) * V + v];
+      dq[v] = acc / I.dx;
+    }
+    bdot_sub(st.v, dq, 0, 1.0, S);
+    const double wgt = dt * K.w[t] * K.w[ia];
+#pragma unroll
+    for (int v = 0; v < V; v++) S[v] = valid ? wgt * S[v] : 0.0;
+  }
+  __syncthreads();
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myQ[node * V + v] = S[v];
+  __syncthreads();
+  for (int v = node; valid && v < V; v += NN) {
+    double du = 0;
+    for (int j = 0; j < NN; j++) du += myQ[j * V + v];
+    const double *fl = b.fx + (I.f0 + lc) * 2 * V;      // left face: hi side
+    const double *fr = b.fx + (I.f0 + lc + 1) * 2 * V;  // right face: lo side
+    du -= fl[V + v] + fr[v];
+    b.u[(I.u0 + lc) * V + v] += du;
+  }
+}
+
+// CFL maximum of one value per lane into its instance: one atomic per
+// wavefront when the wavefront's cells share an instance.  sp is never NaN
+// (callers keep a running maximum that NaN cannot enter, as std::max).
+__device__ __forceinline__ void batch_max(BatchState *st, int k, bool work, double sp, bool second) {
+  const int k0 = __shfl(k, 0);
+  if (__all(k == k0)) {
+    const double mx = wave_max(sp);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(second ? &st[k0].maxspeed2 : &st[k0].maxspeed, mx);
+  } else if (work) {
+    atomic_max_pos(second ? &st[k].maxspeed2 : &st[k].maxspeed, sp);
+  }
+}
+
+// Post-step work, one thread per cell, as k_post: renormalise A, the
+// instance's contortion and non-finite flags, and its CFL maximum for the
+// next dt (cfl_only: the maximum alone, for the first step).
+template <bool TH>
+__global__ void kb_post(BatchArgs b, int cfl_only) {
+  constexpr int V = Dims<TH>::V;
+  const long total = b.u0s[b.count];
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool in = c < total;
+  const int k = find_inst(b.u0s, b.count, in ? c : total - 1);
+  const BatchInst &I = b.inst[k];
+  BatchState &S = b.st[k];
+  const bool work = in && !S.done;
+  double sp = 0.0;
+  if (work) {
+    double q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = b.u[c * V + v];
+    if (!cfl_only) {
+      if (q[0] > 0) {
+        double cf = cbrt(q[0] / (I.m.p.rho0 * det3(q + 5)));
+#pragma unroll
+        for (int j = 0; j < 9; j++) q[5 + j] *= cf;
+      }
+      double f = 0;
+#pragma unroll
+      for (int j = 0; j < 9; j++) f += q[5 + j] * q[5 + j];
+      double d13 = cbrt(fabs(det3(q + 5)));
+      if (f / (d13 * d13) / 3.0 - 1.0 > S.contorted_tol) atomicOr(&S.contorted, 1);
+      bool nonfinite = false;
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        if (!(q[v] - q[v] == 0.0)) nonfinite = true;
+      if (nonfinite) atomicOr(&S.nonfinite, 1);
+#pragma unroll
+      for (int j = 0; j < 9; j++) b.u[c * V + 5 + j] = q[5 + j];
+    }
+    const double s = max_abs_eigs<TH>(q, I.m, 0) / I.dx;
+    if (s > sp) sp = s;
+  }
+  batch_max(b.st, k, work, sp, false);
+}
+
+// reset_distortion of every cell of a contorted instance (utils.cpp:44-63,
+// iterator.cpp:150-151), with the CFL maximum recomputed into maxspeed2
+template <bool TH>
+__global__ void kb_reset(BatchArgs b) {
+  constexpr int V = Dims<TH>::V;
+  const long total = b.u0s[b.count];
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool in = c < total;
+  const int k = find_inst(b.u0s, b.count, in ? c : total - 1);
+  const BatchInst &I = b.inst[k];
+  BatchState &S = b.st[k];
+  const bool work = in && !S.done && S.contorted;
+  double sp = 0.0;
+  if (work) {
+    const Mat &m = I.m;
+    double q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = b.u[c * V + v];
+    double r = q[0];
+    double cf = cbrt(r / m.p.rho0);
+    NodeState<double, TH> s;
+    node_state<double, TH>(q, m, s);
+    double Gm = Gamma_MG(r, m.p);
+    double C0 = (1 - m.p.beta / Gm) * s.cs2;
+    q[1] -= r * (C0 / 4 * s.devGsq);
+#pragma unroll
+    for (int j = 0; j < 9; j++) q[5 + j] = 0.0;
+    q[5] = cf;
+    q[9] = cf;
+    q[13] = cf;
+#pragma unroll
+    for (int v = 0; v < V; v++) b.u[c * V + v] = q[v];
+    const double x = max_abs_eigs<TH>(q, m, 0) / I.dx;
+    if (x > sp) sp = x;
+  }
+  batch_max(b.st, k, work, sp, true);
+}
+
+// t += dt, count += 1 for the instances still running (iterator.cpp:142-143)
+__global__ void kb_advance(BatchArgs b) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b.count) return;
+  BatchState &S = b.st[k];
+  if (S.done) return;
+  S.t += S.dt;
+  S.count += 1;
+}
+
+// the next dt of each running instance (timestep, iterator.cpp:54-83); an
+// instance that reached its end time, or whose state became non-finite
+// (iterator.cpp:161-165), stops
+__global__ void kb_finalize(BatchArgs b) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b.count) return;
+  BatchState &S = b.st[k];
+  if (S.done) return;
+  unsigned long long bits = S.contorted ? S.maxspeed2 : S.maxspeed;
+  double MAX = __longlong_as_double((long long)bits);
+  double dt = S.cfl / MAX;
+  if (S.count <= 5) dt *= 0.2;
+  if (S.t + dt > S.tf) dt = S.tf - S.t;
+  S.dt = dt;
+  S.maxspeed = 0ull;
+  S.maxspeed2 = 0ull;
+  S.contorted = 0;
+  if (S.t >= S.tf || S.nonfinite) {
+    S.done = 1;
+    S.dt = 0.0;
+  }
+}
+
+}  // namespace gpr

@@ -146,7 +146,8 @@ __device__ __forceinline__ void weno_point(const R *d, const Basis &K, int ncell
     ws[2][a] = sc;
   }
   constexpr int NSTEN = N > 2 ? 3 : 2;
-  R o[3];
+  bool useL = ind >= gap, useR = ind <= ncell - gap - 1, useC = N > 2 && ind + FN2 >= gap;
+  R o[3], tt[3];
 #pragma unroll
   for (int s = 0; s < NSTEN; s++) {
     R qv = 0;
@@ -154,11 +155,28 @@ __device__ __forceinline__ void weno_point(const R *d, const Basis &K, int ncell
     for (int i = 0; i < N; i++)
 #pragma unroll
       for (int j = 0; j < N; j++) qv += ws[s][i] * R(K.sig[i][j]) * ws[s][j];
-    R t = qv + R(1e-14);
-    R t2 = t * t, t4 = t2 * t2;
-    o[s] = R(s == 2 ? 1e5 : 1.0) / (t4 * t4);
+    tt[s] = qv + R(1e-14);
   }
-  bool useL = ind >= gap, useR = ind <= ncell - gap - 1, useC = N > 2 && ind + FN2 >= gap;
+  if constexpr (sizeof(R) == sizeof(double)) {
+#pragma unroll
+    for (int s = 0; s < NSTEN; s++) {
+      R t2 = tt[s] * tt[s], t4 = t2 * t2;
+      o[s] = R(s == 2 ? 1e5 : 1.0) / (t4 * t4);
+    }
+  } else {
+    // lambda_s / t_s^8 scaled by the smallest t^8 among the stencils in use:
+    // the normalised weights are unchanged, and t^8 (down to 1e-112) would
+    // underflow in single precision
+    R tmin = R(3e38);
+    if (useL) tmin = fmin(tmin, tt[0]);
+    if (useR) tmin = fmin(tmin, tt[1]);
+    if (useC) tmin = fmin(tmin, tt[NSTEN - 1]);
+#pragma unroll
+    for (int s = 0; s < NSTEN; s++) {
+      R r = tmin / tt[s], r2 = r * r, r4 = r2 * r2;
+      o[s] = R(s == 2 ? 1e5 : 1.0) * (r4 * r4);
+    }
+  }
   R num[N], den = 0;
 #pragma unroll
   for (int a = 0; a < N; a++) num[a] = 0;

@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/gpr_mi355x.h"
-#include "gpr_kernels.hpp"
+#include "gpr_batch.hpp"
 
 using namespace gpr;
 
@@ -200,6 +200,7 @@ struct gpr_ctx {
   size_t ev_used = 0;
   int64_t timed_steps = 0;
   unsigned long long *d_itc = nullptr;  // Picard, fallback iterations, cells
+  struct BatchHost *batch = nullptr;    // gpr_batch_* (1D contexts)
 };
 
 namespace {
@@ -268,28 +269,39 @@ template <int N, bool TH, int ND> struct BoundariesOp {
   }
 };
 
+// WENO in the type R (double: the plain method; float: the FP32-bulk form),
+// ub read in double, wh written in R.
+template <typename R, int N, bool TH, int ND>
+int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny) {
+  constexpr int V = Dims<TH>::V;
+  const Basis &K = basis(N);
+  if (ND == 1) {
+    long total = (long)(nx + 2) * V;
+    hipLaunchKernelGGL((k_weno_1d<R, N, V>), dim3(grid_for(total, 256)), dim3(256), 0, c->stream, d_ub, d_wh, nx,
+                       K);
+  } else {
+    int rc;
+    long nux = (long)(nx + 2) * (ny + 2 * N) * N * V;
+    R *ux = (R *)buf(c, sizeof(R) == 8 ? "ux" : "uxf", nux * sizeof(R), rc);
+    if (!ux) return rc;
+    long t1 = (long)(nx + 2) * (ny + 2 * N) * V;
+    hipLaunchKernelGGL((k_weno_x<R, N, V>), dim3(grid_for(t1, 256)), dim3(256), 0, c->stream, d_ub, ux, nx, ny, K);
+    long t2 = (long)(nx + 2) * (ny + 2) * N * V;
+    hipLaunchKernelGGL((k_weno_y<R, N, V>), dim3(grid_for(t2, 256)), dim3(256), 0, c->stream, ux, d_wh, nx, ny, K);
+  }
+  HIP_TRY(hipGetLastError());
+  return GPR_OK;
+}
+
 template <int N, bool TH, int ND> struct WenoOp {
   static int run(gpr_ctx *c, const double *d_ub, double *d_wh, int nx, int ny) {
-    constexpr int V = Dims<TH>::V;
-    const Basis &K = basis(N);
-    if (ND == 1) {
-      long total = (long)(nx + 2) * V;
-      hipLaunchKernelGGL((k_weno_1d<double, N, V>), dim3(grid_for(total, 256)), dim3(256), 0, c->stream, d_ub,
-                         d_wh, nx, K);
-    } else {
-      int rc;
-      long nux = (long)(nx + 2) * (ny + 2 * N) * N * V;
-      double *ux = (double *)buf(c, "ux", nux * sizeof(double), rc);
-      if (!ux) return rc;
-      long t1 = (long)(nx + 2) * (ny + 2 * N) * V;
-      hipLaunchKernelGGL((k_weno_x<double, N, V>), dim3(grid_for(t1, 256)), dim3(256), 0, c->stream, d_ub, ux, nx,
-                         ny, K);
-      long t2 = (long)(nx + 2) * (ny + 2) * N * V;
-      hipLaunchKernelGGL((k_weno_y<double, N, V>), dim3(grid_for(t2, 256)), dim3(256), 0, c->stream, ux, d_wh,
-                         nx, ny, K);
-    }
-    HIP_TRY(hipGetLastError());
-    return GPR_OK;
+    return weno_run<double, N, TH, ND>(c, d_ub, d_wh, nx, ny);
+  }
+};
+
+template <int N, bool TH, int ND> struct WenoF32Op {
+  static int run(gpr_ctx *c, const double *d_ub, float *d_wh, int nx, int ny) {
+    return weno_run<float, N, TH, ND>(c, d_ub, d_wh, nx, ny);
   }
 };
 
@@ -298,6 +310,23 @@ template <int N, bool TH, int ND> struct PredictorOp {
     constexpr int CPW = PredShape<N, ND>::CPW;
     long blocks = (a.ncell + CPW - 1) / CPW;
     hipLaunchKernelGGL((k_predictor<N, TH, ND>), dim3(blocks), dim3(WAVE), 0, c->stream, a);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+// FP32-bulk predictor: Picard in float over a.mask (a.wh float), marking the
+// cells it leaves in a.redo; then the FP64 preconditioned iteration (mode 1)
+// over a.redo, reading the same float wh.
+template <int N, bool TH, int ND> struct PredictorF32Op {
+  static int run(gpr_ctx *c, PredArgs a) {
+    constexpr int CPW = PredShape<N, ND>::CPW;
+    long blocks = (a.ncell + CPW - 1) / CPW;
+    hipLaunchKernelGGL((k_predictor<N, TH, ND, float, float>), dim3(blocks), dim3(WAVE), 0, c->stream, a);
+    PredArgs b = a;
+    b.mask = a.redo;
+    b.mode = 1;
+    hipLaunchKernelGGL((k_predictor<N, TH, ND, double, float>), dim3(blocks), dim3(WAVE), 0, c->stream, b);
     HIP_TRY(hipGetLastError());
     return GPR_OK;
   }
@@ -565,6 +594,7 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
   c->opts.stiff_tol = 6e-6;
   c->opts.bailout = 0;
   c->opts.mode = 0;
+  c->opts.precision = 0;
   c->opts.contorted_tol = 1.0;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -595,6 +625,7 @@ int gpr_ctx_destroy(gpr_ctx *c) {
   if (c->d_fail) (void)hipFree(c->d_fail);
   if (c->d_itc) (void)hipFree(c->d_itc);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  if (c->batch) (void)gpr_batch_destroy(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GPR_OK;
@@ -603,7 +634,7 @@ int gpr_ctx_destroy(gpr_ctx *c) {
 int gpr_ctx_set_opts(gpr_ctx *c, const gpr_solver_opts *o) {
   if (!c || !o) return fail(GPR_E_ARG, "null argument");
   if (o->dg_it < 1 || o->stiff_it < 1 || !(o->dg_tol > 0) || !(o->stiff_tol > 0) || o->bailout < 0 ||
-      o->mode < 0 || o->mode > 1)
+      o->mode < 0 || o->mode > 1 || o->precision < 0 || o->precision > 1)
     return fail(GPR_E_ARG, "invalid solver options");
   c->opts = *o;
   return GPR_OK;
@@ -1009,17 +1040,30 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
     rc = dispatch<BoundariesOp>(c, c, (const double *)c->run_u, dub, c->run_nx, c->run_ny, c->run_bc);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
-    rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
+    const bool f32 = c->opts.precision == 1;
+    float *dwhf = nullptr;
+    uint8_t *dredo = nullptr;
+    if (f32) {
+      int NS = c->ndim == 2 ? c->N * c->N : c->N;
+      dwhf = (float *)buf(c, "whf", ne * NS * c->V * sizeof(float), rc);
+      if (!dwhf) return rc;
+      dredo = (uint8_t *)buf(c, "redo", ne, rc);
+      if (!dredo) return rc;
+      rc = dispatch<WenoF32Op>(c, c, (const double *)dub, dwhf, c->run_nx, c->run_ny);
+    } else {
+      rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
+    }
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
     PredArgs pa = pred_args(c, c->run_mat, 0, dX);
     pa.dt_dev = &c->d_st->dt;
     pa.qh = dqh;
-    pa.wh = dwh;
+    pa.wh = f32 ? (const void *)dwhf : (const void *)dwh;
+    pa.redo = dredo;
     pa.mask = dm;
     pa.ncell = ne;
     pa.itc = c->timing ? c->d_itc : nullptr;
-    rc = dispatch<PredictorOp>(c, c, pa);
+    rc = f32 ? dispatch<PredictorF32Op>(c, c, pa) : dispatch<PredictorOp>(c, c, pa);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
     FvArgs fa = fv_args(c, c->run_mat, 0, dX, nX);
@@ -1164,6 +1208,310 @@ int gpr_iterator(gpr_ctx *c, double *u, double tf, const int32_t *nX, const doub
   if (nsnap) *nsnap = push;
   if (steps) *steps = k;
   return status;
+}
+
+}  // extern "C"
+
+// ===================================================================== batch
+// Many independent 1D problems in one set of launches per stage; the step
+// loop is captured in a hipGraph and replayed.
+struct BatchHost {
+  int count = 0;
+  std::vector<BatchInst> inst;
+  std::vector<long> u0s, b0s, e0s;
+  long nu = 0, nb = 0, ne = 0, nf = 0;
+  int nblk_p = 0, nblk_c = 0, nblk_f = 0;
+  BatchArgs args{};
+  std::vector<void *> allocs;
+  int block = 256;  // block size of the element-wise kernels
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+  bool ready = false;
+};
+
+namespace {
+
+template <int N, bool TH> struct BatchStep {
+  static int run(gpr_ctx *c, BatchHost *B) {
+    constexpr int V = Dims<TH>::V;
+    const BatchArgs &b = B->args;
+    const int blk = B->block;
+    hipLaunchKernelGGL((kb_boundaries<N, V>), dim3(grid_for(B->nb * V, blk)), dim3(blk), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_weno<N, V>), dim3(grid_for(B->ne * V, blk)), dim3(blk), 0, c->stream, b);
+    PredArgs pa = pred_args(c, B->inst[0].m, 0.0, &B->inst[0].dx);
+    pa.qh = b.qh;
+    pa.wh = b.wh;
+    pa.ncell = B->ne;
+    pa.binst = b.inst;
+    pa.pblk = (const int *)B->allocs[3];
+    pa.bst = b.st;
+    hipLaunchKernelGGL((k_predictor<N, TH, 1, double, double, true>), dim3(B->nblk_p), dim3(WAVE), 0, c->stream,
+                       pa);
+    hipLaunchKernelGGL((kb_faces<N, TH>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_cells<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_post<TH>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
+    hipLaunchKernelGGL(kb_advance, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_reset<TH>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, b);
+    hipLaunchKernelGGL(kb_finalize, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+int batch_step(gpr_ctx *c, BatchHost *B) {
+  if (c->N == 3 && !c->thermal) return BatchStep<3, false>::run(c, B);
+  if (c->N == 3 && c->thermal) return BatchStep<3, true>::run(c, B);
+  if (c->N == 2 && !c->thermal) return BatchStep<2, false>::run(c, B);
+  return BatchStep<2, true>::run(c, B);
+}
+
+int batch_first_dt(gpr_ctx *c, BatchHost *B) {
+  const int blk = B->block;
+  if (c->thermal)
+    hipLaunchKernelGGL((kb_post<true>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, B->args, 1);
+  else
+    hipLaunchKernelGGL((kb_post<false>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, B->args, 1);
+  hipLaunchKernelGGL(kb_finalize, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, B->args);
+  HIP_TRY(hipGetLastError());
+  return GPR_OK;
+}
+
+void batch_drop_graph(BatchHost *B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  B->exec = nullptr;
+  B->graph_steps = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpr_batch_destroy(gpr_ctx *c) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  BatchHost *B = c->batch;
+  if (!B) return GPR_OK;
+  (void)hipStreamSynchronize(c->stream);
+  batch_drop_graph(B);
+  for (void *p : B->allocs)
+    if (p) (void)hipFree(p);
+  delete B;
+  c->batch = nullptr;
+  return GPR_OK;
+}
+
+int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx, const int32_t *bcs,
+                     const gpr_par *MPs, const double *tf, double CFL) {
+  if (!c || count < 1 || !nx || !dx || !bcs || !MPs || !tf) return fail(GPR_E_ARG, "gpr_batch_create: bad argument");
+  if (c->ndim != 1) return fail(GPR_E_UNSUPPORTED, "batches are of 1D problems: create the context with ndim = 1");
+  if (c->batch) gpr_batch_destroy(c);
+  const int N = c->N, V = c->V, NN = N * N;
+  const int CPW = WAVE / NN, FPW = WAVE / N;
+  BatchHost *B = new BatchHost();
+  B->count = count;
+  B->inst.resize(count);
+  B->u0s.assign(count + 1, 0);
+  B->b0s.assign(count + 1, 0);
+  B->e0s.assign(count + 1, 0);
+  std::vector<long> f0s(count + 1, 0);
+  std::vector<int> pblk, cblk, fblk;
+  for (int k = 0; k < count; k++) {
+    if (nx[k] < N || !(dx[k] > 0) || !(tf[k] >= 0)) {
+      delete B;
+      return fail(GPR_E_ARG, "instance " + std::to_string(k) + ": need nx >= N, dx > 0, tf >= 0");
+    }
+    for (int s = 0; s < 2; s++)
+      if (bcs[2 * k + s] < 0 || bcs[2 * k + s] > 5) {
+        delete B;
+        return fail(GPR_E_ARG, "instance " + std::to_string(k) + ": boundary code must be 0..5");
+      }
+    BatchInst &I = B->inst[k];
+    std::memset(&I, 0, sizeof I);
+    int rc = make_mat(&MPs[k], I.m);
+    if (rc) {
+      delete B;
+      return fail(rc, "instance " + std::to_string(k) + ": " + g_err);
+    }
+    I.nx = nx[k];
+    I.dx = dx[k];
+    I.bc0 = bcs[2 * k];
+    I.bc1 = bcs[2 * k + 1];
+    I.u0 = B->u0s[k];
+    I.b0 = B->b0s[k];
+    I.e0 = B->e0s[k];
+    I.f0 = f0s[k];
+    B->u0s[k + 1] = I.u0 + nx[k];
+    B->b0s[k + 1] = I.b0 + nx[k] + 2 * N;
+    B->e0s[k + 1] = I.e0 + nx[k] + 2;
+    f0s[k + 1] = I.f0 + nx[k] + 1;
+    I.blk_p = (int)pblk.size();
+    for (int b = 0; b < (nx[k] + 2 + CPW - 1) / CPW; b++) pblk.push_back(k);
+    I.blk_c = (int)cblk.size();
+    for (int b = 0; b < (nx[k] + CPW - 1) / CPW; b++) cblk.push_back(k);
+    I.blk_f = (int)fblk.size();
+    for (int b = 0; b < (nx[k] + 1 + FPW - 1) / FPW; b++) fblk.push_back(k);
+  }
+  B->nu = B->u0s[count];
+  B->nb = B->b0s[count];
+  B->ne = B->e0s[count];
+  B->nf = f0s[count];
+  B->nblk_p = (int)pblk.size();
+  B->nblk_c = (int)cblk.size();
+  B->nblk_f = (int)fblk.size();
+  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 ub, 8 wh, 9 qh, 10 fx
+  const size_t sizes[11] = {sizeof(BatchInst) * count,
+                            sizeof(BatchState) * count,
+                            sizeof(long) * 3 * (count + 1),
+                            sizeof(int) * pblk.size(),
+                            sizeof(int) * cblk.size(),
+                            sizeof(int) * fblk.size(),
+                            sizeof(double) * B->nu * V,
+                            sizeof(double) * B->nb * V,
+                            sizeof(double) * B->ne * N * V,
+                            sizeof(double) * B->ne * NN * V,
+                            sizeof(double) * B->nf * 2 * V};
+  B->allocs.assign(11, nullptr);
+  c->batch = B;
+  for (int i = 0; i < 11; i++) HIP_TRY(hipMalloc(&B->allocs[i], std::max<size_t>(sizes[i], 256)));
+  std::vector<long> offs;
+  offs.insert(offs.end(), B->u0s.begin(), B->u0s.end());
+  offs.insert(offs.end(), B->b0s.begin(), B->b0s.end());
+  offs.insert(offs.end(), B->e0s.begin(), B->e0s.end());
+  HIP_TRY(hipMemcpyAsync(B->allocs[0], B->inst.data(), sizes[0], hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[2], offs.data(), sizes[2], hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[3], pblk.data(), sizes[3], hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[4], cblk.data(), sizes[4], hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[5], fblk.data(), sizes[5], hipMemcpyHostToDevice, c->stream));
+  std::vector<BatchState> st(count);
+  for (int k = 0; k < count; k++) {
+    std::memset(&st[k], 0, sizeof(BatchState));
+    st[k].tf = tf[k];
+    st[k].cfl = CFL;
+    st[k].contorted_tol = c->opts.contorted_tol;
+    st[k].done = 1;  // until gpr_batch_set_state
+  }
+  HIP_TRY(hipMemcpyAsync(B->allocs[1], st.data(), sizes[1], hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  BatchArgs &a = B->args;
+  a.inst = (const BatchInst *)B->allocs[0];
+  a.st = (BatchState *)B->allocs[1];
+  a.count = count;
+  a.u0s = (const long *)B->allocs[2];
+  a.b0s = a.u0s + (count + 1);
+  a.e0s = a.b0s + (count + 1);
+  a.cblk = (const int *)B->allocs[4];
+  a.fblk = (const int *)B->allocs[5];
+  a.u = (double *)B->allocs[6];
+  a.ub = (double *)B->allocs[7];
+  a.wh = (double *)B->allocs[8];
+  a.qh = (double *)B->allocs[9];
+  a.fx = (double *)B->allocs[10];
+  return GPR_OK;
+}
+
+int gpr_batch_set_state(gpr_ctx *c, const double *u) {
+  if (!c || !c->batch || !u) return fail(GPR_E_ARG, "gpr_batch_set_state: no batch or null state");
+  BatchHost *B = c->batch;
+  HIP_TRY(hipMemcpyAsync(B->args.u, u, sizeof(double) * B->nu * c->V, hipMemcpyHostToDevice, c->stream));
+  std::vector<BatchState> st(B->count);
+  HIP_TRY(hipMemcpyAsync(st.data(), B->args.st, sizeof(BatchState) * B->count, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto &s : st) {
+    s.t = 0.0;
+    s.count = 0;
+    s.dt = 0.0;
+    s.maxspeed = s.maxspeed2 = 0ull;
+    s.done = s.contorted = s.nonfinite = 0;
+    s.contorted_tol = c->opts.contorted_tol;
+  }
+  HIP_TRY(hipMemcpyAsync(B->args.st, st.data(), sizeof(BatchState) * B->count, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  int rc = batch_first_dt(c, B);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  B->ready = true;
+  return GPR_OK;
+}
+
+int gpr_batch_get_state(gpr_ctx *c, double *u) {
+  if (!c || !c->batch || !u) return fail(GPR_E_ARG, "gpr_batch_get_state: no batch or null buffer");
+  BatchHost *B = c->batch;
+  HIP_TRY(hipMemcpyAsync(u, B->args.u, sizeof(double) * B->nu * c->V, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+int gpr_batch_set_block(gpr_ctx *c, int block) {
+  if (!c || !c->batch) return fail(GPR_E_ARG, "no batch");
+  if (block < 64 || block > 1024 || block % 64) return fail(GPR_E_ARG, "block must be a multiple of 64 in [64, 1024]");
+  if (block != c->batch->block) batch_drop_graph(c->batch);
+  c->batch->block = block;
+  return GPR_OK;
+}
+
+int gpr_batch_steps(gpr_ctx *c, int64_t nsteps) {
+  if (!c || !c->batch || !c->batch->ready) return fail(GPR_E_ARG, "gpr_batch_set_state has not been called");
+  for (int64_t k = 0; k < nsteps; k++) {
+    int rc = batch_step(c, c->batch);
+    if (rc) return rc;
+  }
+  return GPR_OK;
+}
+
+int gpr_batch_status(gpr_ctx *c, double *t, int64_t *count, double *dt, int32_t *done, int32_t *nonfinite,
+                     int32_t *pred_fail) {
+  if (!c || !c->batch) return fail(GPR_E_ARG, "no batch");
+  BatchHost *B = c->batch;
+  std::vector<BatchState> st(B->count);
+  int pf = 0;
+  HIP_TRY(hipMemcpyAsync(st.data(), B->args.st, sizeof(BatchState) * B->count, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&pf, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < B->count; k++) {
+    if (t) t[k] = st[k].t;
+    if (count) count[k] = st[k].count;
+    if (dt) dt[k] = st[k].dt;
+    if (done) done[k] = st[k].done;
+    if (nonfinite) nonfinite[k] = st[k].nonfinite;
+  }
+  if (pred_fail) *pred_fail = pf;
+  return GPR_OK;
+}
+
+int gpr_batch_run(gpr_ctx *c, int steps_per_graph, int64_t max_graphs, int64_t *graphs_run, int32_t *running) {
+  if (!c || !c->batch || !c->batch->ready) return fail(GPR_E_ARG, "gpr_batch_set_state has not been called");
+  if (steps_per_graph < 1 || max_graphs < 1) return fail(GPR_E_ARG, "steps_per_graph and max_graphs must be >= 1");
+  BatchHost *B = c->batch;
+  if (B->exec && B->graph_steps != steps_per_graph) batch_drop_graph(B);
+  if (!B->exec) {
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = GPR_OK;
+    for (int k = 0; k < steps_per_graph && rc == GPR_OK; k++) rc = batch_step(c, B);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    B->graph_steps = steps_per_graph;
+  }
+  std::vector<BatchState> st(B->count);
+  int64_t n = 0;
+  int run = B->count;
+  while (n < max_graphs && run > 0) {
+    HIP_TRY(hipGraphLaunch(B->exec, c->stream));
+    n++;
+    HIP_TRY(hipMemcpyAsync(st.data(), B->args.st, sizeof(BatchState) * B->count, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    run = 0;
+    for (auto &s : st) run += s.done ? 0 : 1;
+  }
+  if (graphs_run) *graphs_run = n;
+  if (running) *running = run;
+  return GPR_OK;
 }
 
 }  // extern "C"

@@ -31,9 +31,32 @@ constexpr int WAVE = 64;
 
 struct StepState;
 
+// A batch of independent 1D problems in one set of launches.  Instance k's
+// extended cells are [e0, e0 + nx + 2) of the concatenated wh / qh, its cells
+// [u0, u0 + nx) of u, its ghost-extended rows [b0, b0 + nx + 2N) of ub and its
+// faces [f0, f0 + nx + 1).  Kernels that give one wavefront a group of cells
+// or faces never mix instances: the instance's groups start at block blk_p
+// (predictor), blk_c (cell update) and blk_f (faces).
+struct BatchInst {
+  long e0, u0, b0, f0;
+  int nx, blk_p, blk_c, blk_f;
+  int bc0, bc1;  // boundary codes, left and right
+  double dx;
+  Mat m;
+};
+// per-instance time-loop state
+struct BatchState {
+  double t, tf, cfl, dt, contorted_tol;
+  long long count;
+  unsigned long long maxspeed;   // bit pattern of the CFL maximum (non-negative double)
+  unsigned long long maxspeed2;  // the same after a distortion reset
+  int done, contorted, nonfinite, pad_;
+};
+
 struct PredArgs {
   double *qh;              // [ncell][NN][V]
-  const double *wh;        // [ncell][NS][V]
+  const void *wh;          // [ncell][NS][V], double or float (the kernel's WIn)
+  uint8_t *redo;           // FP32 kernel: 1 for cells Picard leaves, 0 otherwise
   const uint8_t *mask;     // [ncell]
   int32_t *iters;          // optional per-cell iteration count (fallback: 1000 + its)
   int *fail;               // device counter of cells that converged in neither phase
@@ -44,6 +67,10 @@ struct PredArgs {
   int dg_it, stiff_it, bailout, mode;  // mode 0: Picard then fallback; 1: fallback only
   double dg_tol, stiff_tol;
   Mat m;
+  // batched launches only
+  const BatchInst *binst;
+  const int *pblk;         // predictor block -> instance
+  const BatchState *bst;
 };
 
 // One wavefront of 64 lanes holds CPW cells, one space-time node per lane.
@@ -60,17 +87,17 @@ template <int N, int ND> struct PredShape {
 // Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3);
 // the thermal source -(c_alpha^2 T / (kappa rho)) rho J has rate rJ.
 template <int N, bool TH>
-__device__ void stiff_precond(const double *ws, const PredArgs &a, double dt, double *out) {
+__device__ void stiff_precond(const double *ws, const Mat &m, double dt, double *out) {
   const Basis &K = basis_dev<N>();
   NodeState<double, TH> s;
-  node_state<double, TH>(ws, a.m, s);
+  node_state<double, TH>(ws, m, s);
   double dA = det3(ws + 5);
   double s13 = cbrt(dA);
-  double kap = 2.0 * theta1inv<double, TH>(ws, s, a.m) * s.cs2 * s13 * s13;
+  double kap = 2.0 * theta1inv<double, TH>(ws, s, m) * s.cs2 * s13 * s13;
   if (!(kap >= 0.0) || !(kap < 1e300)) kap = 0.0;
   double rJ = 0.0;
   if (TH) {
-    rJ = a.m.p.calpha2 * s.T / (a.m.p.kappa * s.r);
+    rJ = m.p.calpha2 * s.T / (m.p.kappa * s.r);
     if (!(rJ >= 0.0) || !(rJ < 1e300)) rJ = 0.0;
   }
   for (int which = 0; which < (TH ? 2 : 1); which++) {
@@ -145,6 +172,7 @@ __device__ __forceinline__ double fast_div(double a, double b) {
   r = fma(fma(-b, r, 1.0), r, r);
   return a * r;
 }
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Pi(X) = Rot dev(sym(Rot^T X))
 __device__ __forceinline__ void proj_dev_sym(const double *Rot, const double *X, double *out) {
@@ -166,17 +194,24 @@ __device__ __forceinline__ void proj_dev_sym(const double *Rot, const double *X,
       out[i * 3 + j] = Rot[i * 3 + 0] * S[0 * 3 + j] + Rot[i * 3 + 1] * S[1 * 3 + j] + Rot[i * 3 + 2] * S[2 * 3 + j];
 }
 
-template <int N, bool TH, int ND>
-__global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
+// R: the type the iteration computes in.  WIn: the element type of wh.
+// R = double runs Picard and then the preconditioned fallback (the plain FP64
+// method).  R = float runs Picard only; a cell that Picard leaves is marked in
+// a.redo and solved again by the R = double kernel in mode 1 over that mask.
+// BATCH: the launch covers a batch of 1D instances (a.binst, a.pblk, a.bst)
+// with per-instance material, dx and dt; every extended cell is predicted.
+template <int N, bool TH, int ND, typename R = double, typename WIn = double, bool BATCH = false>
+__global__ __launch_bounds__(64, sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES) void k_predictor(PredArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = PredShape<N, ND>::NS;
   constexpr int NN = PredShape<N, ND>::NN;
   constexpr int CPW = PredShape<N, ND>::CPW;
   constexpr int PRE = TH ? 27 : 18;
-  __shared__ double sQ[CPW * NN * V];   // iterate; in the fallback update also P g
-  __shared__ double sF[CPW * NN * V];   // fluxes, then rhs or g
-  __shared__ double sPre[CPW * NS * PRE];
-  __shared__ double sRat[WAVE];
+  constexpr bool FB = sizeof(R) == sizeof(double);  // this instantiation has the fallback
+  __shared__ R sQ[CPW * NN * V];   // iterate; in the fallback update also P g
+  __shared__ R sF[CPW * NN * V];   // fluxes, then rhs or g
+  __shared__ double sPre[FB ? CPW * NS * PRE : 1];
+  __shared__ R sRat[WAVE];
 
   const Basis &K = basis_dev<N>();
   const int lane = threadIdx.x;
@@ -186,51 +221,71 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
   const int s = node - t * NS;
   const int ia = ND == 1 ? s : s / N;
   const int ib = ND == 1 ? 0 : s - (s / N) * N;
-  const long cell = (long)blockIdx.x * CPW + slot;
   const bool lane_ok = slot < CPW;  // lanes past the last whole cell write no LDS
-  const bool valid = lane_ok && cell < a.ncell;
-  const bool active = valid && a.mask[cell];
+  long cell;
+  bool valid;
+  double dt, dxv;
+  const Mat *mp;
+  if constexpr (BATCH) {
+    const int k = a.pblk[blockIdx.x];
+    const BatchInst &I = a.binst[k];
+    const long lc = (long)(blockIdx.x - I.blk_p) * CPW + slot;
+    cell = I.e0 + lc;
+    valid = lane_ok && lc < I.nx + 2 && !a.bst[k].done;
+    dt = a.bst[k].dt;
+    dxv = I.dx;
+    mp = &I.m;
+  } else {
+    cell = (long)blockIdx.x * CPW + slot;
+    valid = lane_ok && cell < a.ncell;
+    dt = dev_or(a.dt_dev, a.dt);
+    dxv = a.dx;
+    mp = &a.m;
+  }
+  const Mat &M = *mp;
+  const bool active = valid && (BATCH || a.mask[cell]);
   const int sl = lane_ok ? slot : 0;
-  double *myQ = sQ + (sl * NN) * V;
-  double *myF = sF + (sl * NN) * V;
-  double *myPre = sPre + (sl * NS + s) * PRE;
-  const double *wsrc = a.wh + (cell * NS + s) * V;
+  R *myQ = sQ + (sl * NN) * V;
+  R *myF = sF + (sl * NN) * V;
+  double *myPre = sPre + (FB ? (sl * NS + s) * PRE : 0);
+  const WIn *wsrc = (const WIn *)a.wh + (cell * NS + s) * V;
 
-  const double dt = dev_or(a.dt_dev, a.dt);
-  const double Ws = ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib];
-  const double iWs = 1.0 / Ws;
-  const double Wt = K.w[t];
-  const double cW = K.endv[0][t] * Ws;  // initial-condition weight psi_t(0) W_s
-  const double inv_dx = 1.0 / a.dx, inv_dy = 1.0 / a.dy;
+  const R dtR = R(dt);
+  const R Ws = R(ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib]);
+  const R iWs = R(1.0 / (ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib]));
+  const R Wt = R(K.w[t]);
+  const R cW = R(K.endv[0][t] * (ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib]));  // psi_t(0) W_s
+  const R inv_dx = R(1.0 / dxv), inv_dy = R(1.0 / a.dy);
   // this lane's rows of the basis matrices
-  double cdx[N], cdy[N], cgx[N], cgy[N], cmi[N], cmt[N];
+  R cdx[N], cdy[N], cgx[N], cgy[N], cmi[N], cmt[N];
 #pragma unroll
   for (int j = 0; j < N; j++) {
-    cdx[j] = K.der[ia][j];
-    cdy[j] = K.der[ib][j];
-    cgx[j] = K.dgder[ia][j];
-    cgy[j] = K.dgder[ib][j];
-    cmi[j] = K.dgmatinv[t][j];
-    cmt[j] = K.dgmat[t][j];
+    cdx[j] = R(K.der[ia][j]);
+    cdy[j] = R(K.der[ib][j]);
+    cgx[j] = R(K.dgder[ia][j]);
+    cgy[j] = R(K.dgder[ib][j]);
+    cmi[j] = R(K.dgmatinv[t][j]);
+    cmt[j] = R(K.dgmat[t][j]);
   }
 
-  double q0[V], w[V];
+  R q0[V], w[V];
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    w[v] = active ? wsrc[v] : (v == 0 || v == 5 || v == 9 || v == 13 ? 1.0 : 0.0);
+    w[v] = active ? R(wsrc[v]) : R(v == 0 || v == 5 || v == 9 || v == 13 ? 1 : 0);
     q0[v] = w[v];
   }
+  if (!FB && valid && node == 0) a.redo[cell] = 0;
 
   // phase: 0 Picard, 1 preconditioned, 2 converged, 3 idle / given up
-  int phase = active ? (a.mode == 1 ? 1 : 0) : 3;
+  int phase = active ? (FB && a.mode == 1 ? 1 : 0) : 3;
   int it = 0, pit = 0, grow = 0;
-  double prev = 1e308;
+  R prev = R(1e30);
   const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * NN);
   bool need_pre = phase == 1;
 
   while (__any(phase == 0 || phase == 1)) {
-    if (need_pre && t == 0 && lane_ok) {
-      stiff_precond<N, TH>(w, a, dt, myPre);
+    if constexpr (FB) {
+      if (need_pre && t == 0 && lane_ok) stiff_precond<N, TH>(w, M, dt, myPre);
     }
     need_pre = false;
     // (1) share the iterate
@@ -240,49 +295,49 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
     wave_sync();
 
     // (2) spatial derivatives, source, non-conservative products, fluxes
-    double rhs[V], G[V];
+    R rhs[V], G[V];
     {
-      NodeState<double, TH> st;
-      node_state<double, TH>(q0, a.m, st);
-      source<double, TH>(q0, st, a.m, rhs);
+      NodeState<R, TH> st;
+      node_state<R, TH>(q0, M, st);
+      source<R, TH>(q0, st, M, rhs);
       // B_d(q) . dq/dx_d reads only the distortion rows of dq (equations.cpp:189-247)
-      double dq[14];
+      R dq[14];
 #pragma unroll
       for (int v = 5; v < 14; v++) {
-        double acc = 0;
+        R acc = 0;
 #pragma unroll
         for (int j = 0; j < N; j++) acc += cdx[j] * myQ[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
         dq[v] = acc * inv_dx;
       }
-      bdot_sub(st.v, dq, 0, 1.0, rhs);
+      bdot_sub(st.v, dq, 0, R(1), rhs);
       if (ND == 2) {
 #pragma unroll
         for (int v = 5; v < 14; v++) {
-          double acc = 0;
+          R acc = 0;
 #pragma unroll
           for (int j = 0; j < N; j++) acc += cdy[j] * myQ[(t * NS + ia * N + j) * V + v];
           dq[v] = acc * inv_dy;
         }
-        bdot_sub(st.v, dq, 1, 1.0, rhs);
+        bdot_sub(st.v, dq, 1, R(1), rhs);
       }
-      const double wts = Wt * Ws;
+      const R wts = Wt * Ws;
 #pragma unroll
       for (int v = 0; v < V; v++) rhs[v] *= wts;
-      double F[V];
-      flux<double, TH>(q0, st, a.m, 0, F);
-      const double wb = ND == 2 ? K.w[ib] : 1.0;
+      R F[V];
+      flux<R, TH>(q0, st, M, 0, F);
+      const R wb = R(ND == 2 ? K.w[ib] : 1.0);
       if (lane_ok)
 #pragma unroll
         for (int v = 0; v < V; v++) myF[node * V + v] = F[v] * wb;
-      if (ND == 2) flux<double, TH>(q0, st, a.m, 1, G);
+      if (ND == 2) flux<R, TH>(q0, st, M, 1, G);
     }
     wave_sync();
     // (3) flux contractions (rhs2, dg2.cpp:173-201) through one buffer
     {
-      const double cx = Wt * inv_dx;
+      const R cx = Wt * inv_dx;
 #pragma unroll
       for (int v = 0; v < V; v++) {
-        double sx = 0;
+        R sx = 0;
 #pragma unroll
         for (int j = 0; j < N; j++) sx += cgx[j] * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
         rhs[v] -= cx * sx;
@@ -294,76 +349,82 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
 #pragma unroll
         for (int v = 0; v < V; v++) myF[node * V + v] = G[v];
       wave_sync();
-      const double cy = Wt * K.w[ia] * inv_dy;
+      const R cy = Wt * R(K.w[ia]) * inv_dy;
 #pragma unroll
       for (int v = 0; v < V; v++) {
-        double sy = 0;
+        R sy = 0;
 #pragma unroll
         for (int j = 0; j < N; j++) sy += cgy[j] * myF[(t * NS + ia * N + j) * V + v];
         rhs[v] -= cy * sy;
       }
     }
 #pragma unroll
-    for (int v = 0; v < V; v++) rhs[v] = rhs[v] * dt + cW * w[v];
+    for (int v = 0; v < V; v++) rhs[v] = rhs[v] * dtR + cW * w[v];
     // (4) time solve.  Picard: q1 = (DG_MAT (x) W_s)^-1 rhs.  Fallback:
     // g = DG_MAT q0 - rhs / W_s (q0 of the time column from sQ), P g in sQ.
-    double P[9];
-    if (phase == 1) {
+    [[maybe_unused]] R P[9];
+    if constexpr (FB) {
+      if (phase == 1) {
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        double acc = 0;
+        for (int v = 0; v < V; v++) {
+          R acc = 0;
 #pragma unroll
-        for (int k = 0; k < N; k++) acc += cmt[k] * myQ[(k * NS + s) * V + v];
-        rhs[v] = acc - rhs[v] * iWs;  // rhs now holds g
+          for (int k = 0; k < N; k++) acc += cmt[k] * myQ[(k * NS + s) * V + v];
+          rhs[v] = acc - rhs[v] * iWs;  // rhs now holds g
+        }
+        proj_dev_sym(myPre + 9, rhs + 5, P);
       }
-      proj_dev_sym(myPre + 9, rhs + 5, P);
     }
     wave_sync();
     if (lane_ok) {
 #pragma unroll
       for (int v = 0; v < V; v++) myF[node * V + v] = rhs[v];
-      if (phase == 1)
+      if constexpr (FB) {
+        if (phase == 1)
 #pragma unroll
-        for (int k = 0; k < 9; k++) myQ[node * V + 5 + k] = P[k];
+          for (int k = 0; k < 9; k++) myQ[node * V + 5 + k] = P[k];
+      }
     }
     wave_sync();
-    if (phase == 1) {
-      const double *Ainv = myPre, *Jinv = myPre + 18;
+    if (FB && phase == 1) {
+      if constexpr (FB) {
+        const double *Ainv = myPre, *Jinv = myPre + 18;
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        double d = 0;
-        if (v >= 5 && v < 14) {
+        for (int v = 0; v < V; v++) {
+          R d = 0;
+          if (v >= 5 && v < 14) {
 #pragma unroll
-          for (int k = 0; k < N; k++) {
-            double gk = myF[(k * NS + s) * V + v];
-            double pk = myQ[(k * NS + s) * V + v];
-            d += cmi[k] * (gk - pk) + Ainv[t * N + k] * pk;
+            for (int k = 0; k < N; k++) {
+              R gk = myF[(k * NS + s) * V + v];
+              R pk = myQ[(k * NS + s) * V + v];
+              d += cmi[k] * (gk - pk) + Ainv[t * N + k] * pk;
+            }
+          } else if (v >= 14) {
+#pragma unroll
+            for (int k = 0; k < N; k++) d += Jinv[t * N + k] * myF[(k * NS + s) * V + v];
+          } else {
+#pragma unroll
+            for (int k = 0; k < N; k++) d += cmi[k] * myF[(k * NS + s) * V + v];
           }
-        } else if (v >= 14) {
-#pragma unroll
-          for (int k = 0; k < N; k++) d += Jinv[t * N + k] * myF[(k * NS + s) * V + v];
-        } else {
-#pragma unroll
-          for (int k = 0; k < N; k++) d += cmi[k] * myF[(k * NS + s) * V + v];
+          rhs[v] = q0[v] - d;  // rhs now holds q1
         }
-        rhs[v] = q0[v] - d;  // rhs now holds q1
       }
     } else {
 #pragma unroll
       for (int v = 0; v < V; v++) {
-        double acc = 0;
+        R acc = 0;
 #pragma unroll
         for (int k = 0; k < N; k++) acc += cmi[k] * myF[(k * NS + s) * V + v];
         rhs[v] = acc * iWs;  // rhs now holds q1
       }
     }
     // (5) stopping rule over the cell's whole block (dg2.cpp:278-297)
-    const double tol = phase == 1 ? a.stiff_tol : a.dg_tol;
+    const R tol = R(phase == 1 ? a.stiff_tol : a.dg_tol);
     bool more = false, nan = false;
-    double ratio = 0;
+    R ratio = 0;
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      double dd = fabs(rhs[v] - q0[v]), lim = tol * (1.0 + fabs(q0[v]));
+      R dd = fabs(rhs[v] - q0[v]), lim = tol * (R(1) + fabs(q0[v]));
       more |= dd > lim;
       nan |= rhs[v] != rhs[v];
       if (a.bailout > 0) ratio = fmax(ratio, fast_div(dd, lim));
@@ -373,7 +434,7 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
     if (a.bailout > 0) {
       sRat[lane] = ratio;
       wave_sync();
-      double cr = 0;
+      R cr = 0;
       for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
       ratio = cr;
     }
@@ -401,13 +462,22 @@ __global__ __launch_bounds__(64, GPR_PRED_WAVES) void k_predictor(PredArgs a) {
         phase = 2;
       }
       if (to_fallback) {
-        phase = 1;
-        pit = 0;
-        need_pre = true;
+        if constexpr (FB) {
+          phase = 1;
+          pit = 0;
+          need_pre = true;
 #pragma unroll
-        for (int v = 0; v < V; v++) q0[v] = w[v];
+          for (int v = 0; v < V; v++) q0[v] = w[v];
+        } else {
+          // solved again in FP64 by the fallback over a.redo
+          if (node == 0) {
+            a.redo[cell] = 1;
+            if (a.itc) atomicAdd(&a.itc[0], (unsigned long long)it);
+          }
+          phase = 3;
+        }
       }
-    } else if (phase == 1) {
+    } else if (FB && phase == 1) {
       pit++;
 #pragma unroll
       for (int v = 0; v < V; v++) q0[v] = rhs[v];
