@@ -236,6 +236,9 @@ def main():
     ap.add_argument("--no-faithful", action="store_true",
                     help="skip the second timed run with bailout = 0")
     ap.add_argument("--bailout", type=int, default=5)
+    ap.add_argument("--mode", type=int, default=0,
+                    help="predictor mode: 0 Picard then the fallback in the same launch; 2 the "
+                         "fallback as a second launch over the cells Picard leaves (same values)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -260,7 +263,7 @@ def main():
     nX = [n, n]
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
-    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout)
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout, mode=args.mode)
     run.steps(args.warmup)
     st0 = run.status()
     if st0["pred_fail"] or st0["nonfinite"]:
@@ -297,13 +300,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"2D GPR double shear layer {n}x{n} periodic, P2 (N=3), FP64, "
                                "mu=1e-3, cs=8, CFL 0.8, steps after warm-up", "cells": cells,
-                   "N": 3, "V": 14, "bailout": args.bailout, "parallelism": "single GPU"},
+                   "N": 3, "V": 14, "bailout": args.bailout, "mode": args.mode,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
     }
     if args.bailout != 0 and not args.no_faithful:
         # the same steps with Picard run to DG_IT = 50 for every cell, as the
         # reference does, from the same starting state
-        run0 = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=0, t0=st0["t"],
+        run0 = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=0, mode=args.mode, t0=st0["t"],
                          count0=st0["count"])
         sec0, _ = timed_run(run0, args.steps)
         out["bailout0_value"] = cells * args.steps / sec0

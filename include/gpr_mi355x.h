@@ -49,8 +49,12 @@ typedef struct gpr_solver_opts {
   double stiff_tol;   /* its stopping tolerance (same rule as dg_tol) */
   int32_t bailout;    /* 0: Picard runs to dg_it as in the reference; k > 0:
                          leave it once its stopping statistic grew k times in a row */
-  int32_t mode;       /* 0: Picard, then the fallback for cells it leaves;
-                         1: the preconditioned iteration for every cell */
+  int32_t mode;       /* 0: Picard, then the fallback for cells it leaves, in
+                            the same wavefront;
+                         1: the preconditioned iteration for every cell;
+                         2: as 0 with the same values, the device loop running
+                            the fallback as a second launch over a list of the
+                            cells Picard leaves */
   double contorted_tol; /* reset threshold of iterator(), its contorted_tol argument */
   int32_t precision;  /* device loop only (gpr_dev_steps): 0 FP64 throughout;
                          1 FP32 bulk: WENO and Picard in FP32, the stiff

@@ -77,7 +77,7 @@ template <typename R> __device__ __forceinline__ R dGamma_MG(R r, const gpr_par 
   if (p.EOS == GPR_EOS_SHOCK_MG) return -R(p.Gamma0 * p.rho0) / (r * r);
   return R(0);
 }
-template <typename R> __device__ R p_ref(R r, const gpr_par &p) {
+template <typename R> __device__ __forceinline__ R p_ref(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
   switch (p.EOS) {
   case GPR_EOS_STIFFENED_GAS: return -R(p.pINF);
@@ -101,7 +101,7 @@ template <typename R> __device__ R p_ref(R r, const gpr_par &p) {
   }
   }
 }
-template <typename R> __device__ R e_ref(R r, const gpr_par &p) {
+template <typename R> __device__ __forceinline__ R e_ref(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
   switch (p.EOS) {
   case GPR_EOS_STIFFENED_GAS: return R(p.pINF) / r;
@@ -123,7 +123,7 @@ template <typename R> __device__ R e_ref(R r, const gpr_par &p) {
   }
   }
 }
-template <typename R> __device__ R dp_ref(R r, const gpr_par &p) {
+template <typename R> __device__ __forceinline__ R dp_ref(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
   switch (p.EOS) {
   case GPR_EOS_STIFFENED_GAS: return R(0);
@@ -147,7 +147,7 @@ template <typename R> __device__ R dp_ref(R r, const gpr_par &p) {
   }
   }
 }
-template <typename R> __device__ R de_ref(R r, const gpr_par &p) {
+template <typename R> __device__ __forceinline__ R de_ref(R r, const gpr_par &p) {
   if (p.EOS == GPR_EOS_SHOCK_MG) {
     R r0 = R(p.rho0);
     if (r > r0) {
@@ -158,7 +158,7 @@ template <typename R> __device__ R de_ref(R r, const gpr_par &p) {
   }
   return p_ref(r, p) / (r * r);
 }
-template <typename R> __device__ R phi_int(R r, const gpr_par &p) {
+template <typename R> __device__ __forceinline__ R phi_int(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
   switch (p.EOS) {
   case GPR_EOS_STIFFENED_GAS: return dpow(r / r0, R(p.gamma - 1));

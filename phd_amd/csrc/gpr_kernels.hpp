@@ -627,6 +627,11 @@ __global__ void k_finalize_dt(StepState *st) {
   st->done = st->t >= st->tf ? 1 : 0;
 }
 
+// *p = 0 as a kernel (a node that graph capture records reliably)
+__global__ void k_zero_int(int *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
 // t += dt, count += 1 (iterator.cpp:142-143)
 __global__ void k_advance(StepState *st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;

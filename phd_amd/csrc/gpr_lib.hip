@@ -315,21 +315,28 @@ template <int N, bool TH, int ND> struct PredictorOp {
   }
 };
 
-// FP32-bulk predictor: Picard in float over a.mask (a.wh float), marking the
-// cells it leaves in a.redo; then the FP64 preconditioned iteration (mode 1)
-// over a.redo, reading the same float wh.
+// The predictor as two launches: Picard in R over a.mask (wh elements WIn),
+// appending the cells it leaves to a.list; then the FP64 preconditioned
+// fallback over the list, CPW listed cells per wavefront.  R = double gives
+// the plain method's values; R = float is the FP32-bulk form.
+template <int N, bool TH, int ND, typename R, typename WIn> int predictor_split(gpr_ctx *c, PredArgs a) {
+  constexpr int CPW = PredShape<N, ND>::CPW;
+  long blocks = (a.ncell + CPW - 1) / CPW;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, a.list_count);
+  hipLaunchKernelGGL((k_predictor<N, TH, ND, R, WIn, false, KIND_PICARD>), dim3(blocks), dim3(WAVE), 0, c->stream,
+                     a);
+  hipLaunchKernelGGL((k_predictor<N, TH, ND, double, WIn, false, KIND_FALLBACK>), dim3(blocks), dim3(WAVE), 0,
+                     c->stream, a);
+  HIP_TRY(hipGetLastError());
+  return GPR_OK;
+}
+
 template <int N, bool TH, int ND> struct PredictorF32Op {
-  static int run(gpr_ctx *c, PredArgs a) {
-    constexpr int CPW = PredShape<N, ND>::CPW;
-    long blocks = (a.ncell + CPW - 1) / CPW;
-    hipLaunchKernelGGL((k_predictor<N, TH, ND, float, float>), dim3(blocks), dim3(WAVE), 0, c->stream, a);
-    PredArgs b = a;
-    b.mask = a.redo;
-    b.mode = 1;
-    hipLaunchKernelGGL((k_predictor<N, TH, ND, double, float>), dim3(blocks), dim3(WAVE), 0, c->stream, b);
-    HIP_TRY(hipGetLastError());
-    return GPR_OK;
-  }
+  static int run(gpr_ctx *c, PredArgs a) { return predictor_split<N, TH, ND, float, float>(c, a); }
+};
+
+template <int N, bool TH, int ND> struct PredictorSplitOp {
+  static int run(gpr_ctx *c, PredArgs a) { return predictor_split<N, TH, ND, double, double>(c, a); }
 };
 
 template <int N, bool TH, int ND> struct FvOp {
@@ -634,7 +641,7 @@ int gpr_ctx_destroy(gpr_ctx *c) {
 int gpr_ctx_set_opts(gpr_ctx *c, const gpr_solver_opts *o) {
   if (!c || !o) return fail(GPR_E_ARG, "null argument");
   if (o->dg_it < 1 || o->stiff_it < 1 || !(o->dg_tol > 0) || !(o->stiff_tol > 0) || o->bailout < 0 ||
-      o->mode < 0 || o->mode > 1 || o->precision < 0 || o->precision > 1)
+      o->mode < 0 || o->mode > 2 || o->precision < 0 || o->precision > 1)
     return fail(GPR_E_ARG, "invalid solver options");
   c->opts = *o;
   return GPR_OK;
@@ -1042,13 +1049,14 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
     if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
     const bool f32 = c->opts.precision == 1;
     float *dwhf = nullptr;
-    uint8_t *dredo = nullptr;
+    int *dlist = (int *)buf(c, "plist", ne * sizeof(int), rc);
+    if (!dlist) return rc;
+    int *dlcnt = (int *)buf(c, "plcnt", sizeof(int), rc);
+    if (!dlcnt) return rc;
     if (f32) {
       int NS = c->ndim == 2 ? c->N * c->N : c->N;
       dwhf = (float *)buf(c, "whf", ne * NS * c->V * sizeof(float), rc);
       if (!dwhf) return rc;
-      dredo = (uint8_t *)buf(c, "redo", ne, rc);
-      if (!dredo) return rc;
       rc = dispatch<WenoF32Op>(c, c, (const double *)dub, dwhf, c->run_nx, c->run_ny);
     } else {
       rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
@@ -1059,11 +1067,17 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
     pa.dt_dev = &c->d_st->dt;
     pa.qh = dqh;
     pa.wh = f32 ? (const void *)dwhf : (const void *)dwh;
-    pa.redo = dredo;
+    pa.list = dlist;
+    pa.list_count = dlcnt;
     pa.mask = dm;
     pa.ncell = ne;
     pa.itc = c->timing ? c->d_itc : nullptr;
-    rc = f32 ? dispatch<PredictorF32Op>(c, c, pa) : dispatch<PredictorOp>(c, c, pa);
+    if (f32)
+      rc = dispatch<PredictorF32Op>(c, c, pa);
+    else if (c->opts.mode == 2)
+      rc = dispatch<PredictorSplitOp>(c, c, pa);
+    else
+      rc = dispatch<PredictorOp>(c, c, pa);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
     FvArgs fa = fv_args(c, c->run_mat, 0, dX, nX);

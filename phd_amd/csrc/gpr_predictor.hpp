@@ -56,7 +56,8 @@ struct BatchState {
 struct PredArgs {
   double *qh;              // [ncell][NN][V]
   const void *wh;          // [ncell][NS][V], double or float (the kernel's WIn)
-  uint8_t *redo;           // FP32 kernel: 1 for cells Picard leaves, 0 otherwise
+  int *list;               // split launches: cells Picard leaves, appended ...
+  int *list_count;         // ... at list[atomicAdd(list_count, 1)]
   const uint8_t *mask;     // [ncell]
   int32_t *iters;          // optional per-cell iteration count (fallback: 1000 + its)
   int *fail;               // device counter of cells that converged in neither phase
@@ -64,7 +65,7 @@ struct PredArgs {
   const double *dt_dev;    // if non-null, dt is read from here
   double dt, dx, dy;
   long ncell;
-  int dg_it, stiff_it, bailout, mode;  // mode 0: Picard then fallback; 1: fallback only
+  int dg_it, stiff_it, bailout, mode;  // mode 1: fallback only; 0 or 2: Picard then fallback
   double dg_tol, stiff_tol;
   Mat m;
   // batched launches only
@@ -87,7 +88,7 @@ template <int N, int ND> struct PredShape {
 // Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3);
 // the thermal source -(c_alpha^2 T / (kappa rho)) rho J has rate rJ.
 template <int N, bool TH>
-__device__ void stiff_precond(const double *ws, const Mat &m, double dt, double *out) {
+__device__ __forceinline__ void stiff_precond(const double *ws, const Mat &m, double dt, double *out) {
   const Basis &K = basis_dev<N>();
   NodeState<double, TH> s;
   node_state<double, TH>(ws, m, s);
@@ -194,20 +195,30 @@ __device__ __forceinline__ void proj_dev_sym(const double *Rot, const double *X,
       out[i * 3 + j] = Rot[i * 3 + 0] * S[0 * 3 + j] + Rot[i * 3 + 1] * S[1 * 3 + j] + Rot[i * 3 + 2] * S[2 * 3 + j];
 }
 
-// R: the type the iteration computes in.  WIn: the element type of wh.
-// R = double runs Picard and then the preconditioned fallback (the plain FP64
-// method).  R = float runs Picard only; a cell that Picard leaves is marked in
-// a.redo and solved again by the R = double kernel in mode 1 over that mask.
+// What one launch does (KIND):
+//   KIND_BOTH      Picard, and the preconditioned fallback in the same
+//                  wavefront for the cells Picard leaves (R = double);
+//   KIND_PICARD    Picard only; the cells it leaves are appended to a.list;
+//   KIND_FALLBACK  the fallback only, from the standard initial guess, for
+//                  the cells of a.list, CPW of them per wavefront (R = double).
+// A cell's result does not depend on which launch or wavefront solves it:
+// KIND_PICARD + KIND_FALLBACK gives KIND_BOTH's values.
+enum { KIND_BOTH = 0, KIND_PICARD = 1, KIND_FALLBACK = 2 };
+
+// R: the type the iteration computes in (double, or float for the FP32-bulk
+// form, KIND_PICARD only).  WIn: the element type of wh.
 // BATCH: the launch covers a batch of 1D instances (a.binst, a.pblk, a.bst)
 // with per-instance material, dx and dt; every extended cell is predicted.
-template <int N, bool TH, int ND, typename R = double, typename WIn = double, bool BATCH = false>
-__global__ __launch_bounds__(64, sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES) void k_predictor(PredArgs a) {
+template <int N, bool TH, int ND, typename R = double, typename WIn = double, bool BATCH = false,
+          int KIND = KIND_BOTH>
+__global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES) void k_predictor(PredArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = PredShape<N, ND>::NS;
   constexpr int NN = PredShape<N, ND>::NN;
   constexpr int CPW = PredShape<N, ND>::CPW;
   constexpr int PRE = TH ? 27 : 18;
-  constexpr bool FB = sizeof(R) == sizeof(double);  // this instantiation has the fallback
+  constexpr bool FB = KIND != KIND_PICARD;  // this instantiation has the fallback
+  static_assert(!FB || sizeof(R) == sizeof(double), "the fallback computes in double");
   __shared__ R sQ[CPW * NN * V];   // iterate; in the fallback update also P g
   __shared__ R sF[CPW * NN * V];   // fluxes, then rhs or g
   __shared__ double sPre[FB ? CPW * NS * PRE : 1];
@@ -225,7 +236,15 @@ __global__ __launch_bounds__(64, sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES
   long cell;
   bool valid;
   double dt, dxv;
-  const Mat *mp;
+  // the material: a reference straight into the kernel argument unless
+  // batched (a pointer chosen on a branch would make the compiler copy the
+  // whole argument to the stack)
+  const Mat &M = [&]() -> const Mat & {
+    if constexpr (BATCH)
+      return a.binst[a.pblk[blockIdx.x]].m;
+    else
+      return a.m;
+  }();
   if constexpr (BATCH) {
     const int k = a.pblk[blockIdx.x];
     const BatchInst &I = a.binst[k];
@@ -234,16 +253,21 @@ __global__ __launch_bounds__(64, sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES
     valid = lane_ok && lc < I.nx + 2 && !a.bst[k].done;
     dt = a.bst[k].dt;
     dxv = I.dx;
-    mp = &I.m;
+  } else if constexpr (KIND == KIND_FALLBACK) {
+    const long g0 = (long)blockIdx.x * CPW;
+    const long cnt = *a.list_count;
+    if (g0 >= cnt) return;  // uniform over the block
+    valid = lane_ok && g0 + slot < cnt;
+    cell = valid ? a.list[g0 + slot] : 0;
+    dt = dev_or(a.dt_dev, a.dt);
+    dxv = a.dx;
   } else {
     cell = (long)blockIdx.x * CPW + slot;
     valid = lane_ok && cell < a.ncell;
     dt = dev_or(a.dt_dev, a.dt);
     dxv = a.dx;
-    mp = &a.m;
   }
-  const Mat &M = *mp;
-  const bool active = valid && (BATCH || a.mask[cell]);
+  const bool active = valid && (BATCH || KIND == KIND_FALLBACK || a.mask[cell]);
   const int sl = lane_ok ? slot : 0;
   R *myQ = sQ + (sl * NN) * V;
   R *myF = sF + (sl * NN) * V;
@@ -274,10 +298,9 @@ __global__ __launch_bounds__(64, sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES
     w[v] = active ? R(wsrc[v]) : R(v == 0 || v == 5 || v == 9 || v == 13 ? 1 : 0);
     q0[v] = w[v];
   }
-  if (!FB && valid && node == 0) a.redo[cell] = 0;
 
   // phase: 0 Picard, 1 preconditioned, 2 converged, 3 idle / given up
-  int phase = active ? (FB && a.mode == 1 ? 1 : 0) : 3;
+  int phase = active ? ((KIND == KIND_FALLBACK || (FB && a.mode == 1)) ? 1 : 0) : 3;
   int it = 0, pit = 0, grow = 0;
   R prev = R(1e30);
   const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * NN);
@@ -469,9 +492,9 @@ __global__ __launch_bounds__(64, sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES
 #pragma unroll
           for (int v = 0; v < V; v++) q0[v] = w[v];
         } else {
-          // solved again in FP64 by the fallback over a.redo
+          // solved again by the KIND_FALLBACK launch over a.list
           if (node == 0) {
-            a.redo[cell] = 1;
+            a.list[atomicAdd(a.list_count, 1)] = (int)cell;
             if (a.itc) atomicAdd(&a.itc[0], (unsigned long long)it);
           }
           phase = 3;

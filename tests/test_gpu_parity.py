@@ -336,3 +336,23 @@ def test_boundaries_vs_reference(gp, V):
                 assert relerr(ub, ref) < 1e-15, (k, key)
             else:
                 assert np.array_equal(ub, ref), (k, key)
+
+
+def test_split_predictor_launches_equal_one_launch():
+    """The device loop's two-launch predictor (mode 2: Picard, then the
+    fallback over the listed cells) gives the values of the single launch
+    that does both (mode 0), bit for bit, on a case where Picard leaves many
+    cells."""
+    from phd_amd import problems
+    from phd_amd.runner import DeviceRun
+    u0, MP, _, dX, bcs = problems.double_shear_2d(64, μ=1e-3)
+    out = []
+    for mode in (0, 2):
+        r = DeviceRun(u0.ravel(), MP, [64, 64], dX, bcs, CFL=0.8, bailout=5, mode=mode)
+        r.set_timing(True)
+        r.steps(12)
+        tm = r.timing()
+        assert tm["fallback_cells"] > 0
+        out.append((r.host(), tm["picard_iters"], tm["fallback_iters"], tm["fallback_cells"]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
