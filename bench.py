@@ -207,6 +207,90 @@ def batched_leg(count, seed, sample):
     return out
 
 
+def adjoint_case(n, steps, K, p_amp):
+    """One adjoint case on the 2D viscous fluid with a random density field
+    (phd_amd.problems.grf_density_2d, mu = 1e-3, periodic): the target is
+    rho_T of the same start at mu* = 2e-3, the loss L = sum (rho_T - rho*_T)^2.
+    Timed: the forward sweep (steps steps, a checkpoint every K) and the
+    reverse sweep (recomputation and adjoint), against the plain form's steps
+    from the same start.  Checked: dL/dmu and dL/drho_0 along a seeded random
+    direction against central differences (h = 1e-6, relative for mu) of the
+    same map at the forward run's time steps."""
+    from phd_amd import problems
+    from phd_amd.adjoint import TIGHT, AdjointRun
+    from phd_amd.runner import DeviceRun
+    u0, MP, _, dX, bcs = problems.grf_density_2d(n, seed=0, μ=1e-3, p_amp=p_amp)
+    u0 = u0.ravel()
+    nX = [n, n]
+    _, MPt, *_ = problems.grf_density_2d(n, seed=0, μ=2e-3, p_amp=p_amp)
+    tgt_run = DeviceRun(u0, MPt, nX, dX, bcs, CFL=0.8, **TIGHT)
+    tgt_run.steps(steps)
+    target = tgt_run.host().reshape(-1, 14)[:, 0].copy()
+    tgt_run.close()
+    # the plain form (its default predictor tolerances) over the same steps
+    plain = DeviceRun(u0, MP, nX, dX, bcs, CFL=0.8, bailout=5)
+    psec, _ = timed_run(plain, steps)
+    plain.close()
+    run = AdjointRun(u0, MP, nX, dX, bcs)
+    tic = time.perf_counter()
+    uT = run.forward(steps, K)
+    fsec = time.perf_counter() - tic
+    r = uT.reshape(-1, 14)[:, 0] - target
+    L = float(np.sum(r * r))
+    lamT = np.zeros((n * n, 14))
+    lamT[:, 0] = 2 * r
+    tic = time.perf_counter()
+    lam0, gmu = run.backward(lamT.ravel())
+    bsec = time.perf_counter() - tic
+
+    def loss_of(u):
+        d = u.reshape(-1, 14)[:, 0] - target
+        return float(np.sum(d * d))
+
+    h = 1e-6 * MP.mu
+    fd_mu = (loss_of(run.replay(mu=MP.mu + h)) - loss_of(run.replay(mu=MP.mu - h))) / (2 * h)
+    d = np.zeros((n * n, 14))
+    d[:, 0] = np.random.default_rng(1).standard_normal(n * n)
+    d = d.ravel()
+    hr = 1e-6
+    fd_rho = (loss_of(run.replay(u0=u0 + hr * d)) - loss_of(run.replay(u0=u0 - hr * d))) / (2 * hr)
+    ad_rho = float(np.dot(lam0, d))
+    run.close()
+    cu = n * n * steps
+
+    def rel(a, b):
+        return abs(a - b) / abs(b) if b else None
+
+    return {"p": "1" if p_amp == 0 else f"1 + {p_amp} GRF (second field of seed 0)",
+            "loss": L, "forward_s": fsec, "backward_s": bsec, "plain_s": psec,
+            "value": cu / (fsec + bsec), "plain_value": cu / psec,
+            "cost_vs_plain": (fsec + bsec) / psec,
+            "dL_dmu": {"adjoint": gmu, "fd": fd_mu, "rel_err": rel(gmu, fd_mu)},
+            "dL_drho0_dir": {"adjoint": ad_rho, "fd": fd_rho, "rel_err": rel(ad_rho, fd_rho)}}
+
+
+def adjoint_leg(n, steps, K):
+    """The adjoint form on the stated workload (v = 0, p = 1, A = rho^(1/3) I)
+    and on the same workload with a random pressure field.  The first start is
+    a steady state: with no pressure gradient and no shear stress, rho_T hardly
+    depends on mu (the loss against the mu* = 2e-3 target is at round-off), so
+    its gradients vanish and only its timing carries information; the second
+    start's acoustic waves are damped by the viscosity, and its gradient
+    checks are the meaningful ones."""
+    out = {"config": f"2D GPR viscous fluid {n}x{n} periodic, random density field (length "
+                     f"scale 0.1, seed 0), v = 0, A = rho^(1/3) I, mu = 1e-3, target at "
+                     f"mu* = 2e-3, {steps} steps, checkpoint every {K}, FP64, P2, CFL 0.8",
+           "cells": n * n, "steps": steps, "K": K,
+           "unit": "cell-updates/s (forward and reverse sweep)",
+           "fd_direction_rho0": "standard normal per cell on rho (seed 1), h = 1e-6",
+           "note": "the forward sweep solves the predictor to 1e-13 (the gradient is that of "
+                   "its solution); plain_s uses the plain form's default tolerances"}
+    out["steady_start"] = adjoint_case(n, steps, K, 0.0)
+    out["pressure_waves"] = adjoint_case(n, steps, K, 0.05)
+    out["value"] = out["steady_start"]["value"]
+    return out
+
+
 def timed_run(run, steps):
     """K steps bracketed by a synchronisation of the library's stream (all
     device work of this process runs on it); HIP-event phase timing on the
@@ -243,6 +327,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-sample", type=int, default=8)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adj-n", type=int, default=256)
+    ap.add_argument("--adj-steps", type=int, default=500)
+    ap.add_argument("--adj-k", type=int, default=25)
     ap.add_argument("--fp32-n", type=int, default=1024)
     ap.add_argument("--fp32-steps", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -317,6 +405,8 @@ def main():
         out["fp32_bulk"] = fp32_bulk_leg(args.fp32_n, args.fp32_steps)
     if not args.no_batch:
         out["batched"] = batched_leg(args.batch_count, 0, args.batch_sample)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps, args.adj_k)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,

@@ -197,6 +197,21 @@ int gpr_batch_run(gpr_ctx *ctx, int steps_per_graph, int64_t max_graphs, int64_t
 int gpr_batch_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t *done, int32_t *nonfinite,
                      int32_t *pred_fail);
 
+/* --- adjoint of the device loop (2D, N = 3, V = 14, Newtonian) --------------
+ * No reference counterpart.  After gpr_dev_begin: gpr_dev_steps_dt advances
+ * with prescribed time steps (dts[k] for step k) instead of the CFL rule;
+ * gpr_adj_forward advances nsteps steps as gpr_dev_steps does, keeping a
+ * checkpoint of u every K steps and the time steps taken (dts_out, may be
+ * null); gpr_adj_backward takes lam_T = d loss / d u at the end (host,
+ * [cells][V]) and returns d loss / d u at the start (lam_0) and d loss / d mu,
+ * with the time steps held at the forward run's values; *unconverged counts
+ * the cells whose transposed predictor iteration did not converge.
+ * gpr_adj_release frees the checkpoints. */
+int gpr_dev_steps_dt(gpr_ctx *ctx, const double *dts, int64_t nsteps);
+int gpr_adj_forward(gpr_ctx *ctx, int64_t nsteps, int K, double *dts_out);
+int gpr_adj_backward(gpr_ctx *ctx, const double *lam_T, double *lam_0, double *grad_mu, int32_t *unconverged);
+int gpr_adj_release(gpr_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,8 @@ SYMBOLS = [
     "gpr_dev_memcpy", "gpr_dev_begin", "gpr_dev_steps", "gpr_dev_status", "gpr_dev_set_timing",
     "gpr_dev_timing", "gpr_batch_create", "gpr_batch_destroy", "gpr_batch_set_state",
     "gpr_batch_get_state", "gpr_batch_set_block", "gpr_batch_steps", "gpr_batch_run",
-    "gpr_batch_status",
+    "gpr_batch_status", "gpr_dev_steps_dt", "gpr_adj_forward", "gpr_adj_backward",
+    "gpr_adj_release",
 ]
 
 
@@ -101,6 +102,10 @@ ARGTYPES = {
     "gpr_batch_steps": [_P, _l],
     "gpr_batch_run": [_P, _i, _l, _Lp, _Ip],
     "gpr_batch_status": [_P, _Dp, _Lp, _Dp, _Ip, _Ip, _Ip],
+    "gpr_dev_steps_dt": [_P, _Dp, _l],
+    "gpr_adj_forward": [_P, _l, _i, _Dp],
+    "gpr_adj_backward": [_P, _Dp, _Dp, _Dp, _Ip],
+    "gpr_adj_release": [_P],
 }
 
 _lib = None

@@ -1,0 +1,111 @@
+"""Gradients of a final-state misfit through the ADER-WENO time loop on the
+GPU: the hand-written adjoint of the whole step (gpr_adj_* in
+libgpr_mi355x.so), with checkpoints every K steps and recomputation in the
+reverse sweep, behind a torch.autograd.Function.
+
+The time steps are those of the forward run (the CFL rule evaluated once,
+then held): the gradient is that of the map u_0 -> u_T at that dt schedule,
+and finite differences are taken with the same schedule
+(AdjointRun.replay).  The predictor is differentiated at its solution of the
+DG equations, so the forward run solves them tightly (dg_tol = stiff_tol =
+1e-13 by default here).  2D, N = 3, V = 14, Newtonian viscosity.
+
+    from phd_amd.adjoint import AdjointRun, StepsFunction
+    run = AdjointRun(u0, MP, nX, dX, bcs)
+    uT = run.forward(nsteps=500, K=25)
+    lam0, dmu = run.backward(dL_duT)
+
+    u0_t = torch.tensor(u0, requires_grad=True); mu_t = torch.tensor(1e-3, requires_grad=True)
+    uT = StepsFunction.apply(u0_t, mu_t, spec)   # spec: dict of MP, nX, dX, bcs, nsteps, K
+    loss = ((uT.view(-1, 14)[:, 0] - target) ** 2).sum(); loss.backward()
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+from .runner import DeviceRun
+
+TIGHT = dict(dg_tol=1e-13, stiff_tol=1e-13, stiff_it=400)
+
+
+class AdjointRun:
+    def __init__(self, u0, MP, nX, dX, bcs, CFL=0.8, **opts):
+        o = dict(TIGHT)
+        o.update(opts)
+        self.u0 = np.ascontiguousarray(u0, np.float64).ravel().copy()
+        self.MP, self.nX, self.dX, self.bcs, self.CFL, self.opts = MP, list(nX), list(dX), bcs, CFL, o
+        self.run = DeviceRun(self.u0, MP, nX, dX, bcs, CFL=CFL, **o)
+        self.L = _lib.load()
+        self.dts = None
+
+    def forward(self, nsteps, K):
+        """Advance nsteps steps (CFL time steps), keeping a checkpoint every K;
+        returns u_T."""
+        self.dts = np.zeros(int(nsteps))
+        check(self.L.gpr_adj_forward(self.run.ctx.h, int(nsteps), int(K), dptr(self.dts)))
+        return self.run.host()
+
+    def backward(self, lam_T):
+        """d loss / d u_0 and d loss / d mu from d loss / d u_T."""
+        lam_T = np.ascontiguousarray(lam_T, np.float64).ravel()
+        lam_0 = np.zeros_like(lam_T)
+        gmu = ctypes.c_double()
+        unconv = ctypes.c_int32()
+        check(self.L.gpr_adj_backward(self.run.ctx.h, dptr(lam_T), dptr(lam_0), ctypes.byref(gmu),
+                                      ctypes.byref(unconv)))
+        if unconv.value:
+            raise _lib.GprError(-5, f"the transposed predictor iteration did not converge in "
+                                    f"{unconv.value} cell-steps")
+        return lam_0, gmu.value
+
+    def replay(self, u0=None, mu=None):
+        """The forward map at the recorded time steps, from u0 (default: the
+        initial state) with viscosity mu (default: the run's): u_T."""
+        MP = self.MP if mu is None else _with_mu(self.MP, float(mu))
+        u = self.u0 if u0 is None else np.ascontiguousarray(u0, np.float64).ravel()
+        r = DeviceRun(u, MP, self.nX, self.dX, self.bcs, CFL=self.CFL, **self.opts)
+        check(self.L.gpr_dev_steps_dt(r.ctx.h, dptr(self.dts), len(self.dts)))
+        out = r.host()
+        r.close()
+        return out
+
+    def close(self):
+        check(self.L.gpr_adj_release(self.run.ctx.h))
+        self.run.close()
+
+
+def _with_mu(MP, mu):
+    """A copy of the material with viscosity mu (and the heat conductivity
+    material_params derives from it left as it was)."""
+    from .params import Par
+    P = Par()
+    ctypes.memmove(ctypes.byref(P), ctypes.byref(MP), ctypes.sizeof(Par))
+    P.mu = mu
+    return P
+
+
+try:
+    import torch
+
+    class StepsFunction(torch.autograd.Function):
+        """u_T = Steps(u_0, mu) over `spec` (MP, nX, dX, bcs, nsteps, K; CPU
+        tensors in and out, the steps and their adjoint on the GPU)."""
+
+        @staticmethod
+        def forward(ctx, u0, mu, spec):
+            MP = _with_mu(spec["MP"], float(mu))
+            run = AdjointRun(u0.detach().cpu().numpy(), MP, spec["nX"], spec["dX"], spec["bcs"])
+            uT = run.forward(spec["nsteps"], spec["K"])
+            ctx.run = run
+            return torch.from_numpy(uT.copy())
+
+        @staticmethod
+        def backward(ctx, grad_uT):
+            lam0, gmu = ctx.run.backward(grad_uT.detach().cpu().numpy())
+            ctx.run.close()
+            return (torch.from_numpy(lam0), torch.tensor(gmu, dtype=torch.float64), None)
+except ImportError:  # the adjoint itself needs no torch
+    StepsFunction = None

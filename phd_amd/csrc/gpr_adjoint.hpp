@@ -1,0 +1,733 @@
+// Adjoint (reverse mode) of one plain FP64 ADER-WENO step, V = 14 (no heat
+// conduction), with the step's dt held fixed (the time steps of the forward
+// run).  With lambda the gradient of a loss with respect to u_{n+1}, the
+// kernels below give its gradient with respect to u_n and accumulate its
+// derivative with respect to the viscosity mu.  Order, the reverse of the
+// step:
+//   ka_renorm   u_{n+1} = renorm(u_fv)                    (utils.cpp:6-23)
+//   ka_faces    u_fv -= face terms of qh                  (fv.cpp:162-268)
+//   ka_cells    u_fv += cell terms of qh; gathers faces  (fv.cpp:56-109)
+//   ka_predictor  qh solves the DG equations R(qh; wh, mu) = 0 (dg2.cpp):
+//               (dR/dq)^T m = lambda_qh by the transposed Picard
+//               iteration, then lambda_wh = -(dR/dwh)^T m
+//   ka_weno_y, ka_weno_x   wh = WENO_y(WENO_x(ub))         (weno.cpp:83-124)
+//   ka_boundaries          ub from u (grid.cpp:147-170), plus the identity
+//                          path u_fv = u_n + ...
+// Local derivatives come from forward-mode dual numbers (gpr_dual.hpp) on the
+// same templated physics the forward step runs.  The predictor is
+// differentiated at its solution (implicit function theorem); the forward run
+// that a gradient is checked against solves it tightly (dg_tol 1e-13).
+#pragma once
+
+#include "gpr_dual.hpp"
+#include "gpr_kernels.hpp"
+
+namespace gpr {
+
+struct AdjArgs {
+  const double *u_fv;   // [cells][V] state after FV, before renormalisation
+  const double *lam1;   // [cells][V] d loss / d u_{n+1}
+  double *lam_fv;       // [cells][V]
+  double *lam_face_x;   // [(nx+1) ny][N][NSF][2][V]: adjoints of qL, qR at each x face node
+  double *lam_face_y;   // [nx (ny+1)][N][NSF][2][V]
+  const double *qh;     // [(nx+2)(ny+2)][NN][V]
+  const double *wh;     // [(nx+2)(ny+2)][NS][V] WENO states (the stiff preconditioner's)
+  double *lam_qh;       // [(nx+2)(ny+2)][NN][V]
+  double *lam_wh;       // [(nx+2)(ny+2)][NS][V]
+  const double *ux;     // WENO x-pass output [(nx+2)(ny+2N)][N][V]
+  const double *ub;     // [(nx+2N)(ny+2N)][V]
+  double *lam_ux;       // like ux
+  double *lam_ub;       // like ub
+  double *lam0;         // [cells][V] d loss / d u_n (output)
+  const uint8_t *mask;  // [(nx+2)(ny+2)]
+  double *gmu;          // d loss / d mu, accumulated
+  int *fail;            // cells whose transposed Picard iteration did not converge
+  double dt, dx, dy, mu;
+  int nx, ny;
+  int it_plain;  // transposed Picard iterations before the preconditioned form
+  int it_max;    // iterations of the preconditioned form
+  double tol;
+  int4 bc;
+  Mat m;
+};
+
+// Renormalisation of A (k_post / utils.cpp:6-23) on a dual state
+template <typename R> __device__ __forceinline__ void renorm_state(R *q, double rho0) {
+  if (q[0] > 0) {
+    R cf = cbrt(q[0] / (rho0 * det3(q + 5)));
+#pragma unroll
+    for (int k = 0; k < 9; k++) q[5 + k] *= cf;
+  }
+}
+
+// one thread per cell: lam_fv = (d renorm / d u_fv)^T lam1
+__global__ void ka_renorm(AdjArgs a, long ncell) {
+  constexpr int V = 14;
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+    const double *u = a.u_fv + c * V;
+    const double *l = a.lam1 + c * V;
+    double out[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) out[v] = l[v];  // rho E and rho v pass through; A and rho below
+    for (int j = 0; j < 14; j++) {
+      if (j >= 1 && j < 5) continue;
+      Dual q[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) q[v] = Dual(u[v], v == j ? 1.0 : 0.0);
+      renorm_state(q, a.m.p.rho0);
+      double g = j == 0 ? l[0] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) g += l[5 + k] * q[5 + k].d;
+      out[j] = g;
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) a.lam_fv[c * V + v] = out[v];
+  }
+}
+
+// Faces: one lane per (face, time node, face node), as k_fv_faces.  For the
+// face's lo = k (b + f) (taken from its left cell) and hi = k (b - f) (from its
+// right cell), the adjoints of qL and qR.
+template <int N, int ND>
+__global__ __launch_bounds__(64) void ka_faces(AdjArgs a) {
+  constexpr int V = 14;
+  constexpr int NS = ND == 1 ? N : N * N;
+  constexpr int NN = NS * N;
+  constexpr int NSF = ND == 2 ? N : 1;
+  constexpr int NT = N * NSF;
+  constexpr int FPW = WAVE / NT;
+  const Basis &K = basis_dev<N>();
+  const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
+  const long nfx = (long)(nx + 1) * ny;
+  const long nfy = ND == 2 ? (long)nx * (ny + 1) : 0;
+  const int lane = threadIdx.x;
+  const int fl = lane / NT, r = lane - fl * NT;
+  const int t = r / NSF, sf = r - t * NSF;
+  const long id = (long)blockIdx.x * FPW + fl;
+  if (fl >= FPW || id >= nfx + nfy) return;
+  const int d = id < nfx ? 0 : 1;
+  long e0, e1, cl = -1, cr = -1;  // extended cells, real cells on either side (-1: none)
+  double *out;
+  if (d == 0) {
+    int i = id / ny, j = id % ny;
+    e0 = ND == 2 ? (long)i * (ny + 2) + (j + 1) : i;
+    e1 = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
+    if (i >= 1) cl = (long)(i - 1) * ny + j;
+    if (i < nx) cr = (long)i * ny + j;
+    out = a.lam_face_x + ((id * N + t) * NSF + sf) * 2 * V;
+  } else {
+    long k = id - nfx;
+    int i = k / (ny + 1), j = k % (ny + 1);
+    e0 = (long)(i + 1) * (ny + 2) + j;
+    e1 = e0 + 1;
+    if (j >= 1) cl = (long)i * ny + (j - 1);
+    if (j < ny) cr = (long)i * ny + j;
+    out = a.lam_face_y + ((k * N + t) * NSF + sf) * 2 * V;
+  }
+  if (!(a.mask[e0] && a.mask[e1])) {
+    for (int v = 0; v < 2 * V; v++) out[v] = 0.0;
+    return;
+  }
+  const double h = d == 0 ? a.dx : a.dy;
+  const double kk = a.dt * K.w[t] / (2.0 * h) * (ND == 2 ? K.w[sf] : 1.0);
+  double qL[V], qR[V];
+  const double *pl = a.qh + (e0 * NN + t * NS) * V;
+  const double *pr = a.qh + (e1 * NN + t * NS) * V;
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double l = 0, rr = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
+      l += K.endv[1][k] * pl[nl * V + v];
+      rr += K.endv[0][k] * pr[nl * V + v];
+    }
+    qL[v] = l;
+    qR[v] = rr;
+  }
+  // u_fv[cl] -= lo, u_fv[cr] -= hi
+  double ls[V], ld[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double lL = cl >= 0 ? a.lam_fv[cl * V + v] : 0.0, lR = cr >= 0 ? a.lam_fv[cr * V + v] : 0.0;
+    ls[v] = -kk * (lL + lR);  // weight of db
+    ld[v] = -kk * (lL - lR);  // weight of df
+  }
+  for (int j = 0; j < 2 * V; j++) {
+    Dual xL[V], xR[V], f[V], b[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      xL[v] = Dual(qL[v], j == v ? 1.0 : 0.0);
+      xR[v] = Dual(qR[v], j == V + v ? 1.0 : 0.0);
+    }
+    if (d == 0) {
+      rusanov<false, Dual>(xL, xR, 0, a.m, f);
+      bint<N, false, Dual>(xL, xR, 0, K, b);
+    } else {
+      rusanov<false, Dual>(xL, xR, 1, a.m, f);
+      bint<N, false, Dual>(xL, xR, 1, K, b);
+    }
+    double g = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) g += ls[v] * b[v].d + ld[v] * f[v].d;
+    out[j] = g;
+  }
+}
+
+// The cell terms of one node, h = S(q) - B_x(q) dqx - B_y(q) dqy (dq fixed),
+// as k_fv_cells and the predictor's right-hand side form them.
+template <typename R, int ND>
+__device__ __forceinline__ void node_h(const R *q, const double *dqx, const double *dqy, const Mat &m, R *h) {
+  NodeState<R, false> st;
+  node_state<R, false>(q, m, st);
+  source<R, false>(q, st, m, h);
+  R dq[14];
+#pragma unroll
+  for (int v = 0; v < 14; v++) dq[v] = R(dqx[v]);
+  bdot_sub(st.v, dq, 0, R(1.0), h);
+  if (ND == 2) {
+#pragma unroll
+    for (int v = 0; v < 14; v++) dq[v] = R(dqy[v]);
+    bdot_sub(st.v, dq, 1, R(1.0), h);
+  }
+}
+
+// (B_d(v))^T w on the distortion rows: out[j] = w . (B_d e_j), j = 5..13
+__device__ __forceinline__ void bdot_T(const double *vel, const double *w, int d, double *out) {
+  for (int j = 0; j < 14; j++) out[j] = 0.0;
+#pragma unroll
+  for (int j = 5; j < 14; j++) {
+    double e[14], o[14];
+#pragma unroll
+    for (int v = 0; v < 14; v++) {
+      e[v] = v == j ? 1.0 : 0.0;
+      o[v] = 0.0;
+    }
+    bdot_sub(vel, e, d, -1.0, o);  // o = B_d e_j
+    double s = 0;
+#pragma unroll
+    for (int v = 0; v < 14; v++) s += w[v] * o[v];
+    out[j] = s;
+  }
+}
+
+// Cells: one lane per space-time node of every extended cell, as k_predictor's
+// layout.  lam_qh = (cell terms)^T lam_fv for real cells + face extrapolation
+// adjoints; the mu-derivative of the cell terms into gmu.
+template <int N, int ND>
+__global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
+  constexpr int V = 14;
+  constexpr int NS = PredShape<N, ND>::NS;
+  constexpr int NN = PredShape<N, ND>::NN;
+  constexpr int CPW = PredShape<N, ND>::CPW;
+  constexpr int NSF = ND == 2 ? N : 1;
+  __shared__ double sQ[CPW * NN * V];
+  __shared__ double sG[CPW * NN * V];
+  const Basis &K = basis_dev<N>();
+  const int lane = threadIdx.x;
+  const int slot = lane / NN;
+  const int node = lane - slot * NN;
+  const int t = node / NS;
+  const int s = node - t * NS;
+  const int ia = ND == 1 ? s : s / N;
+  const int ib = ND == 1 ? 0 : s - (s / N) * N;
+  const bool lane_ok = slot < CPW;
+  const long e = (long)blockIdx.x * CPW + slot;
+  const bool valid = lane_ok && e < ncell_ext && a.mask[e];
+  const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
+  const int I = ND == 2 ? (int)(e / (ny + 2)) : (int)e, J = ND == 2 ? (int)(e % (ny + 2)) : 0;
+  const bool real = valid && I >= 1 && I <= nx && (ND == 1 || (J >= 1 && J <= ny));
+  const long c = real ? (ND == 2 ? (long)(I - 1) * ny + (J - 1) : I - 1) : 0;
+  const int sl = lane_ok ? slot : 0;
+  double *myQ = sQ + sl * NN * V;
+  double *myG = sG + sl * NN * V;
+  double q[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) q[v] = valid ? a.qh[(e * NN + node) * V + v] : (v == 0 || v == 5 || v == 9 || v == 13 ? 1.0 : 0.0);
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myQ[node * V + v] = q[v];
+  __syncthreads();
+  double dqx[V], dqy[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double ax = 0, ay = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      ax += K.der[ia][k] * myQ[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
+      if (ND == 2) ay += K.der[ib][k] * myQ[(t * NS + ia * N + k) * V + v];
+    }
+    dqx[v] = ax / a.dx;
+    dqy[v] = ND == 2 ? ay / a.dy : 0.0;
+  }
+  double g[V], wl[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) g[v] = 0.0;
+  double gx[V], gy[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    gx[v] = 0.0;
+    gy[v] = 0.0;
+  }
+  if (real) {
+    const double wgt = ND == 2 ? a.dt * K.w[t] * K.w[ia] * K.w[ib] : a.dt * K.w[t] * K.w[ia];
+#pragma unroll
+    for (int v = 0; v < V; v++) wl[v] = wgt * a.lam_fv[c * V + v];
+    for (int j = 0; j < V; j++) {
+      Dual x[V], h[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
+      node_h<Dual, ND>(x, dqx, dqy, a.m, h);
+      double s2 = 0;
+#pragma unroll
+      for (int v = 0; v < V; v++) s2 += wl[v] * h[v].d;
+      g[j] = s2;
+    }
+    // the dq inputs: h depends on dqx through -B_x(v) dqx
+    NodeState<double, false> st;
+    node_state<double, false>(q, a.m, st);
+    double tmp[V];
+    bdot_T(st.v, wl, 0, tmp);
+#pragma unroll
+    for (int v = 0; v < V; v++) gx[v] = -tmp[v] / a.dx;
+    if (ND == 2) {
+      bdot_T(st.v, wl, 1, tmp);
+#pragma unroll
+      for (int v = 0; v < V; v++) gy[v] = -tmp[v] / a.dy;
+    }
+    // d h / d mu = -S_A / mu for the Newtonian relaxation (theta1^-1 ~ 1/mu)
+    double S[V];
+    source<double, false>(q, st, a.m, S);
+    double gm = 0;
+#pragma unroll
+    for (int k = 5; k < 14; k++) gm += wl[k] * (-S[k] / a.mu);
+    atomicAdd(a.gmu, gm);
+  }
+  __syncthreads();
+  // scatter the dq adjoints along the lines: q_{t,k,ib} gets der[ia][k] gx(node)
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myG[node * V + v] = gx[v];
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) acc += K.der[k][ia] * myG[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
+    g[v] += acc;
+  }
+  if (ND == 2) {
+    __syncthreads();
+    if (lane_ok)
+#pragma unroll
+      for (int v = 0; v < V; v++) myG[node * V + v] = gy[v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += K.der[k][ib] * myG[(t * NS + ia * N + k) * V + v];
+      g[v] += acc;
+    }
+  }
+  // face extrapolations: this cell is qL of its x+ (y+) face and qR of its x- (y-) face
+  if (valid) {
+    if (ND == 1 || (J >= 1 && J <= ny)) {
+      const int j = ND == 2 ? J - 1 : 0;
+      if (I <= nx) {  // x face i = I: this cell on its left (qL)
+        const double *p = a.lam_face_x + ((((long)I * ny + j) * N + t) * NSF + (ND == 2 ? ib : 0)) * 2 * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] += K.endv[1][ia] * p[v];
+      }
+      if (I >= 1) {  // x face i = I - 1: this cell on its right (qR)
+        const double *p = a.lam_face_x + ((((long)(I - 1) * ny + j) * N + t) * NSF + (ND == 2 ? ib : 0)) * 2 * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] += K.endv[0][ia] * p[V + v];
+      }
+    }
+    if (ND == 2 && I >= 1 && I <= nx) {
+      const int i = I - 1;
+      if (J <= ny) {  // y face j = J: lower side (qL)
+        const double *p = a.lam_face_y + ((((long)i * (ny + 1) + J) * N + t) * NSF + ia) * 2 * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] += K.endv[1][ib] * p[v];
+      }
+      if (J >= 1) {  // y face j = J - 1: upper side (qR)
+        const double *p = a.lam_face_y + ((((long)i * (ny + 1) + J - 1) * N + t) * NSF + ia) * 2 * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] += K.endv[0][ib] * p[V + v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) a.lam_qh[(e * NN + node) * V + v] = g[v];
+  }
+}
+
+// Predictor: one lane per space-time node, CPW cells per wavefront.  At the
+// solution q of R(q) = DG_U2 q - rhs(q, w) = 0 solve (dR/dq)^T m = lam_qh by
+//   m <- DG_U2^-T (lam_qh + (d rhs / d q)^T m),
+// the transposed Picard iteration, which converges exactly where the forward
+// one does.  A cell where it does not (NaN, five growing updates in a row, or
+// it_plain iterations) restarts with the transposed form of the forward's
+// preconditioned fallback, on g = W_s^-1 (lam_qh - (dR/dq)^T m):
+//   m <- m + P^-T g,  P^-1 = DG_MAT^-1 (x) (I - Pi_s) + A_s^-1 (x) Pi_s
+// on the distortion rows (DG_MAT^-1 on the others), with Pi_s and
+// A_s = DG_MAT + dt kappa_s W as stiff_precond builds them at the node's WENO
+// state; Pi_s is an orthogonal projection, so Pi_s^T = Pi_s.  Both forms have
+// the same fixed point.  Then lam_wh = (d rhs / d w)^T m =
+// sum_t psi_t(0) W_s m(t, s), and gmu += (d rhs / d mu)^T m.
+template <int N, int ND>
+__global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
+  constexpr int V = 14;
+  constexpr int NS = PredShape<N, ND>::NS;
+  constexpr int NN = PredShape<N, ND>::NN;
+  constexpr int CPW = PredShape<N, ND>::CPW;
+  __shared__ double sQ[CPW * NN * V];  // q (for dq), then the rhs adjoint pieces
+  __shared__ double sM[CPW * NN * V];  // m
+  __shared__ double sA[CPW * NN * V];  // scratch: a_F, a_G, dq adjoints
+  __shared__ double sRat[WAVE];
+  constexpr int PRE = 18;
+  __shared__ double sPre[CPW * NS * PRE];
+  const Basis &K = basis_dev<N>();
+  const int lane = threadIdx.x;
+  const int slot = lane / NN;
+  const int node = lane - slot * NN;
+  const int t = node / NS;
+  const int s = node - t * NS;
+  const int ia = ND == 1 ? s : s / N;
+  const int ib = ND == 1 ? 0 : s - (s / N) * N;
+  const bool lane_ok = slot < CPW;
+  const long e = (long)blockIdx.x * CPW + slot;
+  const bool active = lane_ok && e < ncell_ext && a.mask[e];
+  const int sl = lane_ok ? slot : 0;
+  double *myQ = sQ + sl * NN * V;
+  double *myM = sM + sl * NN * V;
+  double *myA = sA + sl * NN * V;
+  const double dt = a.dt;
+  const double Ws = ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib];
+  const double Wt = K.w[t];
+  double q[V], lq[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    q[v] = active ? a.qh[(e * NN + node) * V + v] : (v == 0 || v == 5 || v == 9 || v == 13 ? 1.0 : 0.0);
+    lq[v] = active ? a.lam_qh[(e * NN + node) * V + v] : 0.0;
+  }
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myQ[node * V + v] = q[v];
+  __syncthreads();
+  double dqx[V], dqy[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double ax = 0, ay = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      ax += K.der[ia][k] * myQ[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
+      if (ND == 2) ay += K.der[ib][k] * myQ[(t * NS + ia * N + k) * V + v];
+    }
+    dqx[v] = ax / a.dx;
+    dqy[v] = ND == 2 ? ay / a.dy : 0.0;
+  }
+  NodeState<double, false> st;
+  node_state<double, false>(q, a.m, st);
+  double S[V];
+  source<double, false>(q, st, a.m, S);
+  // m0 = DG_U2^-T lam_qh: m(t, s) = sum_k DG_MAT^-1[k][t] x(k, s) / W_s
+  auto solveT = [&](const double *x, double *out) {
+    __syncthreads();
+    if (lane_ok)
+#pragma unroll
+      for (int v = 0; v < V; v++) myA[node * V + v] = x[v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += K.dgmatinv[k][t] * myA[(k * NS + s) * V + v];
+      out[v] = acc / Ws;
+    }
+  };
+  double m0[V], mu_[V];
+  solveT(lq, m0);
+#pragma unroll
+  for (int v = 0; v < V; v++) mu_[v] = m0[v];
+  const unsigned long long cellbits = lane_ok ? (((1ull << NN) - 1ull) << (sl * NN)) : 0ull;
+  const double *myPre = sPre + (sl * NS + s) * PRE;
+  int phase = active ? 0 : 2;  // 0 transposed Picard, 1 preconditioned, 2 done
+  int it = 0, grow = 0;
+  double prev = 0.0;
+  bool failed = false, have_pre = false;
+  while (__ballot(phase < 2) != 0ull) {
+    if (!have_pre && __ballot(phase == 1) != 0ull) {
+      if (active && t == 0) stiff_precond<N, false>(a.wh + (e * NS + s) * V, a.m, dt, sPre + (sl * NS + s) * PRE);
+      have_pre = true;
+      __syncthreads();
+    }
+    // (d rhs / d q)^T m at this node
+    __syncthreads();
+    if (lane_ok)
+#pragma unroll
+      for (int v = 0; v < V; v++) myM[node * V + v] = mu_[v];
+    __syncthreads();
+    double ah[V], aF[V], aG[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      ah[v] = dt * Wt * Ws * mu_[v];
+      // rhs_n -= dt Wt / dx sum_j DG_DER[a_n][j] W_b F(q_{t,j,b}): node (t, ia, ib) is j = ia
+      double sF = 0, sG2 = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        const int n_idx = ND == 1 ? k : k * N + ib;  // nodes (t, k, ib)
+        sF += K.dgder[k][ia] * myM[(t * NS + n_idx) * V + v];
+        if (ND == 2) sG2 += K.dgder[k][ib] * myM[(t * NS + ia * N + k) * V + v];
+      }
+      aF[v] = -dt * Wt / a.dx * (ND == 2 ? K.w[ib] : 1.0) * sF;
+      aG[v] = ND == 2 ? -dt * Wt * K.w[ia] / a.dy * sG2 : 0.0;
+    }
+    double g[V];
+    for (int j = 0; j < V; j++) {
+      Dual x[V], h[V], F[V], G[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
+      NodeState<Dual, false> sd;
+      node_state<Dual, false>(x, a.m, sd);
+      source<Dual, false>(x, sd, a.m, h);
+      Dual dq[14];
+#pragma unroll
+      for (int v = 0; v < 14; v++) dq[v] = Dual(dqx[v]);
+      bdot_sub(sd.v, dq, 0, Dual(1.0), h);
+      if (ND == 2) {
+#pragma unroll
+        for (int v = 0; v < 14; v++) dq[v] = Dual(dqy[v]);
+        bdot_sub(sd.v, dq, 1, Dual(1.0), h);
+      }
+      flux<Dual, false>(x, sd, a.m, 0, F);
+      if (ND == 2) flux<Dual, false>(x, sd, a.m, 1, G);
+      double acc = 0;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        acc += ah[v] * h[v].d + aF[v] * F[v].d;
+        if (ND == 2) acc += aG[v] * G[v].d;
+      }
+      g[j] = acc;
+    }
+    // the dq paths: h -= B_d(v) dq_d, dq_x at node n = sum_k der[a_n][k] q_{t,k,b} / dx
+    double tx[V], ty[V];
+    bdot_T(st.v, ah, 0, tx);
+    if (ND == 2) bdot_T(st.v, ah, 1, ty);
+    __syncthreads();
+    if (lane_ok)
+#pragma unroll
+      for (int v = 0; v < V; v++) myA[node * V + v] = -tx[v] / a.dx;
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += K.der[k][ia] * myA[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
+      g[v] += acc;
+    }
+    if (ND == 2) {
+      __syncthreads();
+      if (lane_ok)
+#pragma unroll
+        for (int v = 0; v < V; v++) myA[node * V + v] = -ty[v] / a.dy;
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += K.der[k][ib] * myA[(t * NS + ia * N + k) * V + v];
+        g[v] += acc;
+      }
+    }
+    // z = (lam_qh + (d rhs / d q)^T m) / W_s, less DG_MAT^T m in the preconditioned form
+    double z[V], P[9];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double mt = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) mt += K.dgmat[k][t] * myM[(k * NS + s) * V + v];
+      z[v] = (lq[v] + g[v]) / Ws - (phase == 1 ? mt : 0.0);
+    }
+    if (phase == 1) proj_dev_sym(myPre + 9, z + 5, P);
+    __syncthreads();
+    if (lane_ok) {
+#pragma unroll
+      for (int v = 0; v < V; v++) myA[node * V + v] = z[v];
+      if (phase == 1)
+#pragma unroll
+        for (int k = 0; k < 9; k++) myQ[node * V + 5 + k] = P[k];
+    }
+    __syncthreads();
+    double mn[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += K.dgmatinv[k][t] * myA[(k * NS + s) * V + v];
+      if (phase == 1 && v >= 5)
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += (myPre[k * N + t] - K.dgmatinv[k][t]) * myQ[(k * NS + s) * V + v];
+      mn[v] = phase == 1 ? mu_[v] + acc : acc;
+    }
+    // stopping rule over the cell's block, as the forward iteration's
+    bool more = false, nan = false;
+    double ratio = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      const double dd = fabs(mn[v] - mu_[v]), lim = a.tol * (1.0 + fabs(mu_[v]));
+      more |= dd > lim;
+      nan |= mn[v] != mn[v];
+      ratio = fmax(ratio, dd / lim);
+    }
+    const unsigned long long bm = __ballot(more), bn = __ballot(nan);
+    const bool cmore = (bm & cellbits) != 0ull, cnan = (bn & cellbits) != 0ull;
+    sRat[lane] = ratio;
+    __syncthreads();
+    double cr = 0;
+    if (lane_ok)
+      for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
+    __syncthreads();
+    if (phase == 0) {
+      it++;
+      bool restart = cnan;
+      if (!cnan) {
+#pragma unroll
+        for (int v = 0; v < V; v++) mu_[v] = mn[v];
+        if (!cmore) {
+          phase = 2;
+        } else {
+          grow = cr > prev ? grow + 1 : 0;
+          prev = cr;
+          restart = it >= a.it_plain || grow >= 5;
+        }
+      }
+      if (restart) {
+        phase = 1;
+        it = 0;
+#pragma unroll
+        for (int v = 0; v < V; v++) mu_[v] = m0[v];
+      }
+    } else if (phase == 1) {
+      it++;
+#pragma unroll
+      for (int v = 0; v < V; v++) mu_[v] = mn[v];
+      if (!cmore && !cnan) {
+        phase = 2;
+      } else if (cnan || it >= a.it_max) {
+        phase = 2;
+        failed = true;
+      }
+    }
+  }
+  if (failed && node == 0) atomicAdd(a.fail, 1);
+  // lam_wh(s) = sum_t psi_t(0) W_s m(t, s); d rhs / d mu through S
+  __syncthreads();
+  if (lane_ok)
+#pragma unroll
+    for (int v = 0; v < V; v++) myM[node * V + v] = mu_[v];
+  __syncthreads();
+  if (active) {
+    if (t == 0) {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += K.endv[0][k] * Ws * myM[(k * NS + s) * V + v];
+        a.lam_wh[(e * NS + s) * V + v] = acc;
+      }
+    }
+    double gm = 0;
+#pragma unroll
+    for (int k = 5; k < 14; k++) gm += dt * Wt * Ws * mu_[k] * (-S[k] / a.mu);
+    atomicAdd(a.gmu, gm);
+  }
+}
+
+// WENO y-pass transposed: lam_ux[i][j + r][a][v] += d wh[i][j][a][.][v] / d ux[i][j + r][a][v]
+// . lam_wh.  One thread per line (i, j, a, v), as k_weno_y.
+template <int N>
+__global__ void ka_weno_y(AdjArgs a) {
+  constexpr int V = 14;
+  const Basis &K = basis_dev<N>();
+  const int nx = a.nx, ny = a.ny;
+  const int NX = nx + 2, NY = ny + 2, ncol = ny + 2 * N;
+  const long total = (long)NX * NY * N * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    const int v = id % V;
+    long c = id / V;
+    const int aa = c % N;
+    c /= N;
+    const int j = c % NY, i = c / NY;
+    const int gap = (i == 0 || i == NX - 1) ? 1 : 0;
+    double lw[N];
+#pragma unroll
+    for (int b = 0; b < N; b++) lw[b] = a.lam_wh[((((long)i * NY + j) * N + aa) * N + b) * V + v];
+    for (int r = 0; r < 2 * N - 1; r++) {
+      Dual d[2 * N - 1], o[N];
+#pragma unroll
+      for (int k = 0; k < 2 * N - 1; k++) d[k] = Dual(a.ux[(((long)i * ncol + j + k) * N + aa) * V + v], k == r ? 1.0 : 0.0);
+      weno_point<Dual, N>(d, K, NY, gap, j, o);
+      double g = 0;
+#pragma unroll
+      for (int b = 0; b < N; b++) g += lw[b] * o[b].d;
+      atomicAdd(&a.lam_ux[(((long)i * ncol + j + r) * N + aa) * V + v], g);
+    }
+  }
+}
+
+// WENO x-pass transposed: lam_ub[i + r][j][v] += d ux[i][j][.][v] / d ub[i + r][j][v] . lam_ux
+template <int N>
+__global__ void ka_weno_x(AdjArgs a) {
+  constexpr int V = 14;
+  const Basis &K = basis_dev<N>();
+  const int nx = a.nx, ny = a.ny;
+  const int NX = nx + 2, ncol = ny + 2 * N;
+  const long total = (long)NX * ncol * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    const int v = id % V;
+    const long c = id / V;
+    const int j = c % ncol, i = c / ncol;
+    const int gap = (j < N || j >= ncol - N) ? N : 0;
+    double lw[N];
+#pragma unroll
+    for (int b = 0; b < N; b++) lw[b] = a.lam_ux[(c * N + b) * V + v];
+    for (int r = 0; r < 2 * N - 1; r++) {
+      Dual d[2 * N - 1], o[N];
+#pragma unroll
+      for (int k = 0; k < 2 * N - 1; k++) d[k] = Dual(a.ub[((long)(i + k) * ncol + j) * V + v], k == r ? 1.0 : 0.0);
+      weno_point<Dual, N>(d, K, NX, gap, i, o);
+      double g = 0;
+#pragma unroll
+      for (int b = 0; b < N; b++) g += lw[b] * o[b].d;
+      atomicAdd(&a.lam_ub[((long)(i + r) * ncol + j) * V + v], g);
+    }
+  }
+}
+
+// Boundaries transposed (codes 0, 1, 2, 3, 5: linear copies with signs) and
+// the identity path: lam0 = lam_fv + sum over the ghost-extended positions
+// that copy each cell.  One thread per (ghost-extended cell, variable).
+template <int N>
+__global__ void ka_boundaries(AdjArgs a) {
+  constexpr int V = 14;
+  const int nx = a.nx, ny = a.ny;
+  const long nby = ny + 2 * N;
+  const long total = (long)(nx + 2 * N) * nby * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    const int v = id % V;
+    const long c = id / V;
+    const int J = c % nby, I = c / nby;
+    bool rx, ry;
+    const int codex = I < N ? a.bc.x : a.bc.y;
+    const int i = bc_src(I, nx, N, codex, rx);
+    const int codey = J < N ? a.bc.z : a.bc.w;
+    const int j = bc_src(J, ny, N, codey, ry);
+    double sgn = (rx ? bc_sign(v, codex, 0) : 1.0) * (ry ? bc_sign(v, codey, 1) : 1.0);
+    const double g = sgn * a.lam_ub[id];
+    if (g != 0.0) atomicAdd(&a.lam0[((long)i * ny + j) * V + v], g);
+  }
+}
+
+}  // namespace gpr

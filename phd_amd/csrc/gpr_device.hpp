@@ -354,65 +354,65 @@ __device__ __forceinline__ void bdot(const R *v, const R *x, int d, R *out) {
 // Largest |eigenvalue| of a real 3x3 matrix: roots of the characteristic
 // polynomial (trigonometric form when all are real, Cardano otherwise), the
 // dominant real root refined by Newton steps on the polynomial.
-__device__ __forceinline__ double max_abs_eig3(const double *a) {
-  double c2 = a[0] + a[4] + a[8];
-  double c1 = a[0] * a[4] - a[1] * a[3] + a[0] * a[8] - a[2] * a[6] + a[4] * a[8] - a[5] * a[7];
-  double c0 = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+template <typename R> __device__ __forceinline__ R max_abs_eig3(const R *a) {
+  R c2 = a[0] + a[4] + a[8];
+  R c1 = a[0] * a[4] - a[1] * a[3] + a[0] * a[8] - a[2] * a[6] + a[4] * a[8] - a[5] * a[7];
+  R c0 = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
               a[2] * (a[3] * a[7] - a[4] * a[6]);
   // lambda^3 - c2 lambda^2 + c1 lambda - c0 ; shift lambda = y + c2/3
-  double m = c2 / 3.0;
-  double P = c1 - c2 * c2 / 3.0;                                 // y^3 + P y + Qc
-  double Qc = -2.0 * m * m * m + c1 * m - c0;
-  double disc = (Qc * Qc) / 4.0 + (P * P * P) / 27.0;
-  double best;
+  R m = c2 / 3.0;
+  R P = c1 - c2 * c2 / 3.0;                                 // y^3 + P y + Qc
+  R Qc = -2.0 * m * m * m + c1 * m - c0;
+  R disc = (Qc * Qc) / 4.0 + (P * P * P) / 27.0;
+  R best;
   if (disc <= 0.0 && c2 >= 0.0 && c1 >= 0.0 && c0 >= 0.0) {
     // three real roots, all >= 0 (signs of the coefficients): Newton from the
     // trace (>= the largest root) decreases monotonically onto the largest root
-    double x = c2;
+    R x = c2;
     for (int it = 0; it < 60; it++) {
-      double f = ((x - c2) * x + c1) * x - c0;
-      double fp = (3 * x - 2 * c2) * x + c1;
+      R f = ((x - c2) * x + c1) * x - c0;
+      R fp = (3 * x - 2 * c2) * x + c1;
       if (!(fp > 0.0)) break;
-      double nx = x - f / fp;
+      R nx = x - f / fp;
       if (!(nx < x)) break;
       x = nx;
     }
     return fabs(x);
   }
   if (disc <= 0.0) {
-    double rr = sqrt(fmax(-P / 3.0, 0.0));
-    double arg = (rr > 0) ? (-Qc / 2.0) / (rr * rr * rr) : 0.0;
-    arg = fmin(1.0, fmax(-1.0, arg));
-    double th = acos(arg) / 3.0;
-    double y0 = 2 * rr * cos(th), y1 = 2 * rr * cos(th - 2.0943951023931955),
+    R rr = sqrt(fmax(-P / 3.0, 0.0));
+    R arg = (rr > 0) ? (-Qc / 2.0) / (rr * rr * rr) : R(0.0);
+    arg = fmin(fmax(arg, -1.0), 1.0);
+    R th = acos(arg) / 3.0;
+    R y0 = 2 * rr * cos(th), y1 = 2 * rr * cos(th - 2.0943951023931955),
            y2 = 2 * rr * cos(th + 2.0943951023931955);
-    double l0 = y0 + m, l1 = y1 + m, l2 = y2 + m;
+    R l0 = y0 + m, l1 = y1 + m, l2 = y2 + m;
     best = l0;
     if (fabs(l1) > fabs(best)) best = l1;
     if (fabs(l2) > fabs(best)) best = l2;
     for (int it = 0; it < 3; it++) {
-      double f = ((best - c2) * best + c1) * best - c0;
-      double fp = (3 * best - 2 * c2) * best + c1;
+      R f = ((best - c2) * best + c1) * best - c0;
+      R fp = (3 * best - 2 * c2) * best + c1;
       if (fp == 0.0) break;
-      double nb = best - f / fp;
+      R nb = best - f / fp;
       if (!(fabs(nb - best) > 0)) { best = nb; break; }
       best = nb;
     }
     return fabs(best);
   }
-  double sq = sqrt(disc);
-  double u = cbrt(-Qc / 2.0 + sq), v = cbrt(-Qc / 2.0 - sq);
-  double y = u + v;
-  double lr = y + m;
+  R sq = sqrt(disc);
+  R u = cbrt(-Qc / 2.0 + sq), v = cbrt(-Qc / 2.0 - sq);
+  R y = u + v;
+  R lr = y + m;
   for (int it = 0; it < 3; it++) {
-    double f = ((lr - c2) * lr + c1) * lr - c0;
-    double fp = (3 * lr - 2 * c2) * lr + c1;
+    R f = ((lr - c2) * lr + c1) * lr - c0;
+    R fp = (3 * lr - 2 * c2) * lr + c1;
     if (fp == 0.0) break;
     lr = lr - f / fp;
   }
   // complex pair: product of all roots = c0
-  double mod2 = (lr != 0.0) ? c0 / lr : 0.0;
-  double mc = sqrt(fabs(mod2));
+  R mod2 = (lr != 0.0) ? c0 / lr : R(0.0);
+  R mc = sqrt(fabs(mod2));
   return fmax(fabs(lr), mc);
 }
 
@@ -494,38 +494,38 @@ __device__ double max_abs_eig4(const double *a) {
 }
 
 // Thermo-acoustic tensor Xi = Xi1 Xi2 (eig.cpp:8-64) into X (n x n, n = 3 or 4).
-template <bool TH>
-__device__ __forceinline__ int thermo_acoustic_tensor(const double *q, const Mat &mm, int d, double *X) {
+template <bool TH, typename R = double>
+__device__ __forceinline__ int thermo_acoustic_tensor(const R *q, const Mat &mm, int d, R *X) {
   const gpr_par &p = mm.p;
-  NodeState<double, TH> s;
-  node_state<double, TH>(q, mm, s);
-  const double *A = q + 5;
-  double r = s.r, cs2 = s.cs2;
-  double dcs2 = mm.beta_zero ? 0.0 : p.beta / r * cs2;
-  double sd[3] = {s.sig[0 * 3 + d], s.sig[1 * 3 + d], s.sig[2 * 3 + d]};
-  double fac = 1.0 / r + dcs2 / cs2;
-  double dsdr[3] = {fac * sd[0], fac * sd[1], fac * sd[2]};
+  NodeState<R, TH> s;
+  node_state<R, TH>(q, mm, s);
+  const R *A = q + 5;
+  R r = s.r, cs2 = s.cs2;
+  R dcs2 = mm.beta_zero ? R(0.0) : p.beta / r * cs2;
+  R sd[3] = {s.sig[0 * 3 + d], s.sig[1 * 3 + d], s.sig[2 * 3 + d]};
+  R fac = 1.0 / r + dcs2 / cs2;
+  R dsdr[3] = {fac * sd[0], fac * sd[1], fac * sd[2]};
   // dsigma/dA (state.cpp:56-69): M = -rho cs2 (AdevG with col d doubled + A_d G_d/3 + G_dd A)^T
-  double G[9];
+  R G[9];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) G[i * 3 + j] = A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j];
-  double T9[9];
+  R T9[9];
   for (int i = 0; i < 9; i++) T9[i] = s.EA[i] / cs2;  // A dev G
   for (int i = 0; i < 3; i++) T9[i * 3 + d] *= 2.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) T9[i * 3 + j] += 1.0 / 3.0 * A[i * 3 + d] * G[d * 3 + j] + G[d * 3 + d] * A[i * 3 + j];
-  double M[9];
+  R M[9];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) M[i * 3 + j] = -r * cs2 * T9[j * 3 + i];
   // c0^2 (speeds.cpp:8-18)
-  double Gm = Gamma_MG(r, p), dGm = dGamma_MG(r, p), pr = p_ref(r, p), dpr = dp_ref(r, p), der = de_ref(r, p);
-  double dedr = der - (dpr * r * Gm + (Gm + r * dGm) * (s.p - pr)) / ((r * Gm) * (r * Gm));
-  double dC0 = (1 - p.beta / Gm) * dcs2 + p.beta / (Gm * Gm) * dGm * cs2;
-  double dEdr = dedr + dC0 / 4 * s.devGsq;
-  double dEdp = 1.0 / (r * Gm);
-  double c02 = (s.p / (r * r) - dEdr) / dEdp;
+  R Gm = Gamma_MG(r, p), dGm = dGamma_MG(r, p), pr = p_ref(r, p), dpr = dp_ref(r, p), der = de_ref(r, p);
+  R dedr = der - (dpr * r * Gm + (Gm + r * dGm) * (s.p - pr)) / ((r * Gm) * (r * Gm));
+  R dC0 = (1 - p.beta / Gm) * dcs2 + p.beta / (Gm * Gm) * dGm * cs2;
+  R dEdr = dedr + dC0 / 4 * s.devGsq;
+  R dEdp = 1.0 / (r * Gm);
+  R c02 = (s.p / (r * r) - dEdr) / dEdp;
   constexpr int n = TH ? 4 : 3;
-  double X1[4][5] = {}, X2[5][4] = {};
+  R X1[4][5] = {}, X2[5][4] = {};
   X1[0][1] = 1 / r;
   for (int i = 0; i < 3; i++) {
     X1[i][0] = -1 / r * dsdr[i];
@@ -537,15 +537,15 @@ __device__ __forceinline__ int thermo_acoustic_tensor(const double *q, const Mat
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) X2[2 + i][j] = A[i * 3 + j];
   if (TH) {
-    double dTdp = dEdp / p.cv;
+    R dTdp = dEdp / p.cv;
     X1[3][0] = -(dpr * r * Gm + (Gm + r * dGm) * (s.p - pr)) / ((r * Gm) * (r * Gm)) / p.cv / r;
     X1[3][1] = dTdp / r;
-    double ch = sqrt(p.calpha2 * s.T / p.cv) / r;  // c_h (speeds.cpp:20-25): NaN for T < 0 as there
+    R ch = sqrt(p.calpha2 * s.T / p.cv) / r;  // c_h (speeds.cpp:20-25): NaN for T < 0 as there
     X2[1][3] = r * ch * ch / dTdp;
   }
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) {
-      double acc = 0;
+      R acc = 0;
       for (int k = 0; k < 5; k++) acc += X1[i][k] * X2[k][j];
       X[i * n + j] = acc;
     }
@@ -553,13 +553,17 @@ __device__ __forceinline__ int thermo_acoustic_tensor(const double *q, const Mat
 }
 
 // max |characteristic speed| in direction d (eig.cpp:66-77)
-template <bool TH>
-__device__ __forceinline__ double max_abs_eigs(const double *q, const Mat &mm, int d) {
-  double X[16];
-  thermo_acoustic_tensor<TH>(q, mm, d, X);
-  double lam2 = TH ? max_abs_eig4(X) : max_abs_eig3(X);
-  double lam = sqrt(lam2);
-  double vd = q[2 + d] / q[0];
+template <bool TH, typename R = double>
+__device__ __forceinline__ R max_abs_eigs(const R *q, const Mat &mm, int d) {
+  R X[16];
+  thermo_acoustic_tensor<TH, R>(q, mm, d, X);
+  R lam2;
+  if constexpr (TH)
+    lam2 = max_abs_eig4(X);
+  else
+    lam2 = max_abs_eig3<R>(X);
+  R lam = sqrt(lam2);
+  R vd = q[2 + d] / q[0];
   return vd > 0 ? vd + lam : lam - vd;
 }
 

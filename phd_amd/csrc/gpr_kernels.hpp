@@ -14,6 +14,8 @@
 //   k_reset / k_finalize_dt   (iterator.cpp:54-83,145-151, utils.cpp:44-85)
 #pragma once
 
+#include <type_traits>
+
 #include "gpr_predictor.hpp"
 
 namespace gpr {
@@ -157,7 +159,7 @@ __device__ __forceinline__ void weno_point(const R *d, const Basis &K, int ncell
       for (int j = 0; j < N; j++) qv += ws[s][i] * R(K.sig[i][j]) * ws[s][j];
     tt[s] = qv + R(1e-14);
   }
-  if constexpr (sizeof(R) == sizeof(double)) {
+  if constexpr (!std::is_same<R, float>::value) {
 #pragma unroll
     for (int s = 0; s < NSTEN; s++) {
       R t2 = tt[s] * tt[s], t4 = t2 * t2;
@@ -261,53 +263,53 @@ __global__ void k_weno_y(const R *__restrict__ ux, R *__restrict__ wh, int nx, i
 // Rusanov flux D_RUS (fluxes.cpp:66-75) and the path integral Bint
 // (fluxes.cpp:9-20) between two states, accumulated as
 //   lo += k (b + f),  hi += k (b - f).
-template <bool TH>
-__device__ __forceinline__ void rusanov(const double *qL, const double *qR, int d, const Mat &m, double *f) {
+template <bool TH, typename R = double>
+__device__ __forceinline__ void rusanov(const R *qL, const R *qR, int d, const Mat &m, R *f) {
   constexpr int V = Dims<TH>::V;
-  double sL = max_abs_eigs<TH>(qL, m, d), sR = max_abs_eigs<TH>(qR, m, d);
-  double smax = (sL < sR) ? sR : sL;  // std::max(max1, max2): a NaN on the right is dropped
-  double F[V];
+  R sL = max_abs_eigs<TH, R>(qL, m, d), sR = max_abs_eigs<TH, R>(qR, m, d);
+  R smax = (sL < sR) ? sR : sL;  // std::max(max1, max2): a NaN on the right is dropped
+  R F[V];
 #pragma unroll
   for (int v = 0; v < V; v++) f[v] = smax * (qL[v] - qR[v]);
   {
-    NodeState<double, TH> st;
-    node_state<double, TH>(qL, m, st);
-    flux<double, TH>(qL, st, m, d, F);
+    NodeState<R, TH> st;
+    node_state<R, TH>(qL, m, st);
+    flux<R, TH>(qL, st, m, d, F);
 #pragma unroll
     for (int v = 0; v < V; v++) f[v] += F[v];
-    node_state<double, TH>(qR, m, st);
-    flux<double, TH>(qR, st, m, d, F);
+    node_state<R, TH>(qR, m, st);
+    flux<R, TH>(qR, st, m, d, F);
 #pragma unroll
     for (int v = 0; v < V; v++) f[v] += F[v];
   }
 }
 
-template <int N, bool TH>
-__device__ __forceinline__ void bint(const double *qL, const double *qR, int d, const Basis &K, double *b) {
+template <int N, bool TH, typename R = double>
+__device__ __forceinline__ void bint(const R *qL, const R *qR, int d, const Basis &K, R *b) {
   constexpr int V = Dims<TH>::V;
 #pragma unroll
   for (int v = 0; v < V; v++) b[v] = 0.0;
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    double xi = K.x[i];
-    double r = qL[0] + xi * (qR[0] - qL[0]);
-    double vv[3];
+    R xi = K.x[i];
+    R r = qL[0] + xi * (qR[0] - qL[0]);
+    R vv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) vv[c] = (qL[2 + c] + xi * (qR[2 + c] - qL[2 + c])) / r;
-    double dq[V];
+    R dq[V];
 #pragma unroll
     for (int v = 0; v < V; v++) dq[v] = qR[v] - qL[v];
-    bdot_sub(vv, dq, d, -K.w[i], b);
+    bdot_sub(vv, dq, d, R(-K.w[i]), b);
   }
 }
 
-template <int N, bool TH>
-__device__ __forceinline__ void riemann_accum(const double *qL, const double *qR, int d, const Mat &m,
-                                              const Basis &K, double k, double *lo, double *hi) {
+template <int N, bool TH, typename R = double>
+__device__ __forceinline__ void riemann_accum(const R *qL, const R *qR, int d, const Mat &m,
+                                              const Basis &K, double k, R *lo, R *hi) {
   constexpr int V = Dims<TH>::V;
-  double f[V], b[V];
-  rusanov<TH>(qL, qR, d, m, f);
-  bint<N, TH>(qL, qR, d, K, b);
+  R f[V], b[V];
+  rusanov<TH, R>(qL, qR, d, m, f);
+  bint<N, TH, R>(qL, qR, d, K, b);
 #pragma unroll
   for (int v = 0; v < V; v++) {
     lo[v] += k * (b[v] + f[v]);

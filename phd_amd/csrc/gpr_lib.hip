@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/gpr_mi355x.h"
+#include "gpr_adjoint.hpp"
 #include "gpr_batch.hpp"
 
 using namespace gpr;
@@ -201,6 +202,7 @@ struct gpr_ctx {
   int64_t timed_steps = 0;
   unsigned long long *d_itc = nullptr;  // Picard, fallback iterations, cells
   struct BatchHost *batch = nullptr;    // gpr_batch_* (1D contexts)
+  struct AdjHost *adj = nullptr;        // gpr_adj_* (2D, V = 14)
 };
 
 namespace {
@@ -575,6 +577,11 @@ int fv_buffers(gpr_ctx *c, FvArgs &a) {
   return GPR_OK;
 }
 
+__global__ void k_set_dt(StepState *st, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->dt = dt;
+}
+int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts);
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -633,6 +640,7 @@ int gpr_ctx_destroy(gpr_ctx *c) {
   if (c->d_itc) (void)hipFree(c->d_itc);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->batch) (void)gpr_batch_destroy(c);
+  if (c->adj) (void)gpr_adj_release(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GPR_OK;
@@ -1021,7 +1029,17 @@ int gpr_dev_begin(gpr_ctx *c, double *d_u, const int32_t *nX, const double *dX, 
   return GPR_OK;
 }
 
-int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
+int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) { return dev_steps_impl(c, nsteps, nullptr); }
+
+int gpr_dev_steps_dt(gpr_ctx *c, const double *dts, int64_t nsteps) {
+  if (!dts) return fail(GPR_E_ARG, "null time steps");
+  return dev_steps_impl(c, nsteps, dts);
+}
+
+}  // extern "C"
+
+namespace {
+int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
   if (!c || !c->run_ready) return fail(GPR_E_ARG, "gpr_dev_begin has not been called");
   int rc;
   int nX[2] = {c->run_nx, c->run_ny};
@@ -1044,6 +1062,7 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
       c->timed_steps++;
       HIP_TRY(hipEventRecord(ev[0], c->stream));
     }
+    if (dts) hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->stream, c->d_st, dts[k]);
     rc = dispatch<BoundariesOp>(c, c, (const double *)c->run_u, dub, c->run_nx, c->run_ny, c->run_bc);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
@@ -1100,6 +1119,9 @@ int gpr_dev_steps(gpr_ctx *c, int64_t nsteps) {
   }
   return GPR_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int gpr_dev_set_timing(gpr_ctx *c, int enable) {
   if (!c) return fail(GPR_E_ARG, "null context");
@@ -1525,6 +1547,248 @@ int gpr_batch_run(gpr_ctx *c, int steps_per_graph, int64_t max_graphs, int64_t *
   }
   if (graphs_run) *graphs_run = n;
   if (running) *running = run;
+  return GPR_OK;
+}
+
+}  // extern "C"
+
+// ==================================================================== adjoint
+// Forward run with checkpoints and its dt schedule, then the reverse sweep:
+// each segment of K steps is recomputed from its checkpoint, then its steps
+// are differentiated in reverse (gpr_adjoint.hpp).
+struct AdjHost {
+  int64_t nsteps = 0;
+  int K = 0;
+  std::vector<double> dts;
+  double *chk = nullptr;  // [nchk][cells][V]
+  double *seg = nullptr;  // [K][cells][V]
+  std::vector<std::pair<void *, size_t>> bufs;
+};
+
+namespace {
+
+__global__ void k_renorm_only(double *u, long ncell, double rho0) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+    double q[14];
+#pragma unroll
+    for (int v = 0; v < 14; v++) q[v] = u[c * 14 + v];
+    if (q[0] > 0) {  // as k_post
+      double cf = cbrt(q[0] / (rho0 * det3(q + 5)));
+#pragma unroll
+      for (int k = 0; k < 9; k++) q[5 + k] *= cf;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[c * 14 + 5 + k] = q[5 + k];
+  }
+}
+
+void *abuf(AdjHost *A, int i, size_t bytes, int &rc) {
+  if ((int)A->bufs.size() <= i) A->bufs.resize(i + 1, {nullptr, 0});
+  auto &e = A->bufs[i];
+  if (e.first && e.second >= bytes) return e.first;
+  if (e.first) (void)hipFree(e.first);
+  e = {nullptr, 0};
+  if (hipMalloc(&e.first, std::max<size_t>(bytes, 256)) != hipSuccess) {
+    rc = fail(GPR_E_HIP, "adjoint buffer allocation");
+    return nullptr;
+  }
+  e.second = bytes;
+  rc = GPR_OK;
+  return e.first;
+}
+
+// the stages of one step from u_n with a given dt: ub, ux, wh, qh and u_fv
+// (u_n plus the FV update, before renormalisation)
+int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt) {
+  int rc;
+  int nX[2] = {c->run_nx, c->run_ny};
+  double dX[2] = {c->run_dx, c->run_dy};
+  long ne = ext_cells(c, nX, 1), nc = ext_cells(c, nX, 0);
+  double *dub = (double *)c->bufs["ub"].first;
+  double *dwh = (double *)c->bufs["wh"].first;
+  double *dqh = (double *)c->bufs["qh"].first;
+  uint8_t *dm = (uint8_t *)c->bufs["mask"].first;
+  rc = dispatch<BoundariesOp>(c, c, u_n, dub, c->run_nx, c->run_ny, c->run_bc);
+  if (rc) return rc;
+  rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
+  if (rc) return rc;
+  PredArgs pa = pred_args(c, c->run_mat, dt, dX);
+  pa.qh = dqh;
+  pa.wh = dwh;
+  pa.mask = dm;
+  pa.ncell = ne;
+  rc = dispatch<PredictorOp>(c, c, pa);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(u_fv, u_n, sizeof(double) * nc * c->V, hipMemcpyDeviceToDevice, c->stream));
+  FvArgs fa = fv_args(c, c->run_mat, dt, dX, nX);
+  fa.u = u_fv;
+  fa.qh = dqh;
+  fa.mask = dm;
+  fa.post = 0;
+  rc = fv_buffers(c, fa);
+  if (rc) return rc;
+  return dispatch<FvOp>(c, c, fa);
+}
+
+template <int N> int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *lam, double *d_gmu) {
+  constexpr int V = 14;
+  int rc;
+  const int nx = c->run_nx, ny = c->run_ny;
+  long nc = (long)nx * ny, ne = (long)(nx + 2) * (ny + 2);
+  long nb = (long)(nx + 2 * N) * (ny + 2 * N), nux = (long)(nx + 2) * (ny + 2 * N);
+  long nfx = (long)(nx + 1) * ny, nfy = (long)nx * (ny + 1);
+  double *u_fv = (double *)abuf(A, 0, sizeof(double) * nc * V, rc);
+  if (!u_fv) return rc;
+  rc = adj_stages(c, u_n, u_fv, dt);
+  if (rc) return rc;
+  AdjArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.u_fv = u_fv;
+  a.lam1 = lam;
+  if (!(a.lam_fv = (double *)abuf(A, 1, sizeof(double) * nc * V, rc))) return rc;
+  if (!(a.lam_face_x = (double *)abuf(A, 2, sizeof(double) * nfx * N * N * 2 * V, rc))) return rc;
+  if (!(a.lam_face_y = (double *)abuf(A, 3, sizeof(double) * nfy * N * N * 2 * V, rc))) return rc;
+  if (!(a.lam_qh = (double *)abuf(A, 4, sizeof(double) * ne * N * N * N * V, rc))) return rc;
+  if (!(a.lam_wh = (double *)abuf(A, 5, sizeof(double) * ne * N * N * V, rc))) return rc;
+  if (!(a.lam_ux = (double *)abuf(A, 6, sizeof(double) * nux * N * V, rc))) return rc;
+  if (!(a.lam_ub = (double *)abuf(A, 7, sizeof(double) * nb * V, rc))) return rc;
+  if (!(a.lam0 = (double *)abuf(A, 8, sizeof(double) * nc * V, rc))) return rc;
+  a.qh = (const double *)c->bufs["qh"].first;
+  a.ux = (const double *)c->bufs["ux"].first;
+  a.ub = (const double *)c->bufs["ub"].first;
+  a.mask = (const uint8_t *)c->bufs["mask"].first;
+  a.gmu = d_gmu;
+  a.fail = c->d_fail;
+  a.dt = dt;
+  a.dx = c->run_dx;
+  a.dy = c->run_dy;
+  a.mu = c->run_mat.p.mu;
+  a.nx = nx;
+  a.ny = ny;
+  a.wh = (const double *)c->bufs["wh"].first;
+  a.it_plain = c->opts.dg_it;
+  a.it_max = 400;
+  a.tol = 1e-13;
+  a.bc = c->run_bc;
+  a.m = c->run_mat;
+  HIP_TRY(hipMemsetAsync(a.lam_wh, 0, sizeof(double) * ne * N * N * V, c->stream));
+  HIP_TRY(hipMemsetAsync(a.lam_ux, 0, sizeof(double) * nux * N * V, c->stream));
+  HIP_TRY(hipMemsetAsync(a.lam_ub, 0, sizeof(double) * nb * V, c->stream));
+  hipLaunchKernelGGL(ka_renorm, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a, nc);
+  constexpr int FPW = FaceShape<N, false, 2>::FPW;
+  hipLaunchKernelGGL((ka_faces<N, 2>), dim3((nfx + nfy + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
+  constexpr int CPW = PredShape<N, 2>::CPW;
+  hipLaunchKernelGGL((ka_cells<N, 2>), dim3((ne + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a, ne);
+  hipLaunchKernelGGL((ka_predictor<N, 2>), dim3((ne + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a, ne);
+  hipLaunchKernelGGL((ka_weno_y<N>), dim3(grid_for(ne * N * V, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL((ka_weno_x<N>), dim3(grid_for(nux * V, 256)), dim3(256), 0, c->stream, a);
+  HIP_TRY(hipMemcpyAsync(a.lam0, a.lam_fv, sizeof(double) * nc * V, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL((ka_boundaries<N>), dim3(grid_for(nb * V, 256)), dim3(256), 0, c->stream, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(lam, a.lam0, sizeof(double) * nc * V, hipMemcpyDeviceToDevice, c->stream));
+  return GPR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpr_adj_release(gpr_ctx *c) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  AdjHost *A = c->adj;
+  if (!A) return GPR_OK;
+  (void)hipStreamSynchronize(c->stream);
+  if (A->chk) (void)hipFree(A->chk);
+  if (A->seg) (void)hipFree(A->seg);
+  for (auto &b : A->bufs)
+    if (b.first) (void)hipFree(b.first);
+  delete A;
+  c->adj = nullptr;
+  return GPR_OK;
+}
+
+int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
+  if (!c || !c->run_ready) return fail(GPR_E_ARG, "gpr_dev_begin has not been called");
+  if (c->ndim != 2 || c->thermal || c->N != 3) return fail(GPR_E_UNSUPPORTED, "the adjoint is built for 2D, N = 3, V = 14");
+  if (nsteps < 1 || K < 1) return fail(GPR_E_ARG, "nsteps and K must be >= 1");
+  if (c->run_mat.p.SOLID || c->run_mat.p.POWER_LAW || c->run_mat.p.BINGHAM)
+    return fail(GPR_E_UNSUPPORTED, "the adjoint's mu derivative is that of the Newtonian relaxation");
+  for (int k = 0; k < 4; k++) {
+    int code = k == 0 ? c->run_bc.x : k == 1 ? c->run_bc.y : k == 2 ? c->run_bc.z : c->run_bc.w;
+    if (code == 4) return fail(GPR_E_UNSUPPORTED, "the adjoint has no moving-wall boundary");
+  }
+  if (c->adj) gpr_adj_release(c);
+  AdjHost *A = new AdjHost();
+  c->adj = A;
+  A->nsteps = nsteps;
+  A->K = K;
+  A->dts.resize(nsteps);
+  const long nc = (long)c->run_nx * c->run_ny;
+  const size_t sz = sizeof(double) * nc * c->V;
+  const int64_t nchk = (nsteps + K - 1) / K;
+  HIP_TRY(hipMalloc((void **)&A->chk, sz * nchk));
+  HIP_TRY(hipMalloc((void **)&A->seg, sz * K));
+  for (int64_t k = 0; k < nsteps; k++) {
+    if (k % K == 0)
+      HIP_TRY(hipMemcpyAsync(A->chk + (k / K) * nc * c->V, c->run_u, sz, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(&A->dts[k], &c->d_st->dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc = dev_steps_impl(c, 1, &A->dts[k]);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (dts_out) std::memcpy(dts_out, A->dts.data(), sizeof(double) * nsteps);
+  return GPR_OK;
+}
+
+int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *grad_mu, int32_t *unconverged) {
+  if (!c || !c->adj) return fail(GPR_E_ARG, "gpr_adj_forward has not been called");
+  AdjHost *A = c->adj;
+  int rc;
+  const long nc = (long)c->run_nx * c->run_ny;
+  const size_t sz = sizeof(double) * nc * c->V;
+  double *lam = (double *)abuf(A, 9, sz, rc);
+  if (!lam) return rc;
+  double *d_gmu = (double *)abuf(A, 10, sizeof(double), rc);
+  if (!d_gmu) return rc;
+  double *work = (double *)abuf(A, 11, sz, rc);
+  if (!work) return rc;
+  HIP_TRY(hipMemcpyAsync(lam, lam_T, sz, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(d_gmu, 0, sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  const int K = A->K;
+  const int64_t nchk = (A->nsteps + K - 1) / K;
+  int nX[2] = {c->run_nx, c->run_ny};
+  double dX[2] = {c->run_dx, c->run_dy};
+  for (int64_t sgi = nchk - 1; sgi >= 0; sgi--) {
+    const int64_t n0 = sgi * K, n1 = std::min<int64_t>(n0 + K, A->nsteps);
+    // recompute u_n0 .. u_{n1-1} from the checkpoint
+    HIP_TRY(hipMemcpyAsync(work, A->chk + sgi * nc * c->V, sz, hipMemcpyDeviceToDevice, c->stream));
+    for (int64_t n = n0; n < n1; n++) {
+      HIP_TRY(hipMemcpyAsync(A->seg + (n - n0) * nc * c->V, work, sz, hipMemcpyDeviceToDevice, c->stream));
+      if (n + 1 < n1) {
+        double *u_fv = (double *)abuf(A, 0, sz, rc);
+        if (!u_fv) return rc;
+        rc = adj_stages(c, work, u_fv, A->dts[n]);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_renorm_only, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, u_fv, nc,
+                           c->run_mat.p.rho0);
+        HIP_TRY(hipMemcpyAsync(work, u_fv, sz, hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
+    for (int64_t n = n1 - 1; n >= n0; n--) {
+      rc = adj_step_back<3>(c, A, A->seg + (n - n0) * nc * c->V, A->dts[n], lam, d_gmu);
+      if (rc) return rc;
+    }
+  }
+  (void)nX;
+  (void)dX;
+  int nf = 0;
+  HIP_TRY(hipMemcpyAsync(lam_0, lam, sz, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&nf, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (unconverged) *unconverged = nf;
   return GPR_OK;
 }
 

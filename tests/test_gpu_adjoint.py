@@ -1,0 +1,145 @@
+"""The adjoint of the ADER-WENO time loop (gpr_adj_*) against central finite
+differences of the same discrete map: a loss L = sum (u_T[comp] - target)^2
+after n steps at the forward run's time steps, its derivative with respect to
+the viscosity and along directions of the initial field.
+
+The finite differences replay the recorded time steps (gpr_dev_steps_dt), so
+both sides differentiate u_0 -> u_T at one dt schedule; the forward runs
+solve the DG predictor to 1e-13.
+
+Where a field is exactly constant, the WENO smoothness indicators of its
+stencils are 0 against their epsilon of 1e-14 (weno.cpp:4-15), and a
+perturbation of size h moves them to h^2: the map stays differentiable but is
+linear only for h well below 1e-7.  The initial-field checks therefore start
+from states with a small relative noise (1e-3) in every variable.  Along
+energy and distortion directions the map is strongly curved even so: over 10
+steps the central difference misses the adjoint by 5.7e-2, 5.3e-4 and 4.7e-6
+(relative) at h = 1e-6, 1e-7 and 1e-8, falling 100-fold per decade of h as an
+O(h^2) truncation error does, so those checks take h = 1e-8.  Along density
+alone the gap is 7e-6, 5e-8 and 3e-9 at the same h."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS, K = 10, 4  # three checkpoint segments, the last one short
+
+
+def case(n=16, mu=1e-2, seed=1, noise=1e-3):
+    from phd_amd import problems
+    u0, MP, _, dX, bcs = problems.grf_density_2d(n, seed=seed, μ=mu)
+    u0 = u0.ravel()
+    rng = np.random.default_rng(seed + 100)
+    u0 = u0 * (1 + noise * rng.standard_normal(u0.shape)) + 0.1 * noise * rng.standard_normal(u0.shape)
+    return u0, MP, [n, n], dX, bcs
+
+
+def loss(u, target, comp=0):
+    r = u.reshape(-1, 14)[:, comp] - target
+    return float(np.sum(r * r)), r
+
+
+@pytest.fixture(scope="module")
+def adj():
+    from phd_amd.adjoint import AdjointRun
+    print("adjoint case 16x16 GRF density, 10 steps, K = 4", flush=True)
+    u0, MP, nX, dX, bcs = case()
+    target = 1.0 + 0.01 * np.random.default_rng(5).standard_normal(nX[0] * nX[1])
+    run = AdjointRun(u0, MP, nX, dX, bcs)
+    uT = run.forward(NSTEPS, K)
+    L, r = loss(uT, target)
+    lamT = np.zeros_like(uT).reshape(-1, 14)
+    lamT[:, 0] = 2 * r
+    lam0, gmu = run.backward(lamT.ravel())
+    yield {"run": run, "u0": u0, "MP": MP, "target": target, "L": L, "uT": uT, "lam0": lam0,
+           "gmu": gmu}
+    run.close()
+
+
+def test_replay_reproduces_forward(adj):
+    assert np.array_equal(adj["run"].replay(), adj["uT"])
+
+
+def test_mu_gradient_against_fd():
+    """d L / d mu on a shear flow (the double shear layer at 16^2, mu = 1e-2),
+    L on the x-momentum: the viscous relaxation is what moves it."""
+    from phd_amd import problems
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, _, dX, bcs = problems.double_shear_2d(16, μ=1e-2)
+    u0 = u0.ravel()
+    target = 0.99 * u0.reshape(-1, 14)[:, 2]
+    run = AdjointRun(u0, MP, [16, 16], dX, bcs)
+    uT = run.forward(NSTEPS, K)
+    L, r = loss(uT, target, 2)
+    lamT = np.zeros_like(uT).reshape(-1, 14)
+    lamT[:, 2] = 2 * r
+    _, gmu = run.backward(lamT.ravel())
+    h = 1e-6 * MP.mu
+    Lp, _ = loss(run.replay(mu=MP.mu + h), target, 2)
+    Lm, _ = loss(run.replay(mu=MP.mu - h), target, 2)
+    fd = (Lp - Lm) / (2 * h)
+    run.close()
+    print(f"dL/dmu adjoint {gmu:.10e} fd {fd:.10e}")
+    assert abs(gmu - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("which", ["rho", "all"])
+def test_initial_field_gradient_against_fd(adj, which):
+    run, u0 = adj["run"], adj["u0"]
+    rng = np.random.default_rng(11 if which == "rho" else 12)
+    d = np.zeros_like(u0).reshape(-1, 14)
+    if which == "rho":
+        d[:, 0] = rng.standard_normal(d.shape[0])
+    else:
+        d = rng.standard_normal(d.shape) * np.maximum(np.abs(u0.reshape(-1, 14)), 1e-3)
+    d = d.ravel()
+    h = 1e-8
+    Lp, _ = loss(run.replay(u0=u0 + h * d), adj["target"])
+    Lm, _ = loss(run.replay(u0=u0 - h * d), adj["target"])
+    fd = (Lp - Lm) / (2 * h)
+    ad = float(np.dot(adj["lam0"], d))
+    print(f"{which}: <dL/du0, d> adjoint {ad:.10e} fd {fd:.10e}")
+    assert abs(ad - fd) <= 2e-5 * abs(fd)
+
+
+def test_density_gradient_smooth_field():
+    """Along density on the smooth field alone (no noise; the random density
+    field itself), h = 1e-6."""
+    from phd_amd import problems
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, _, dX, bcs = problems.grf_density_2d(16, seed=1, μ=1e-2)
+    u0 = u0.ravel()
+    target = 1.0 + 0.01 * np.random.default_rng(5).standard_normal(256)
+    run = AdjointRun(u0, MP, [16, 16], dX, bcs)
+    uT = run.forward(NSTEPS, K)
+    L, r = loss(uT, target)
+    lamT = np.zeros_like(uT).reshape(-1, 14)
+    lamT[:, 0] = 2 * r
+    lam0, _ = run.backward(lamT.ravel())
+    d = np.zeros((256, 14))
+    d[:, 0] = np.random.default_rng(11).standard_normal(256)
+    d = d.ravel()
+    h = 1e-6
+    fd = (loss(run.replay(u0=u0 + h * d), target)[0] - loss(run.replay(u0=u0 - h * d), target)[0]) / (2 * h)
+    ad = float(np.dot(lam0, d))
+    run.close()
+    print(f"smooth rho: adjoint {ad:.10e} fd {fd:.10e}")
+    assert abs(ad - fd) <= 2e-5 * abs(fd)
+
+
+def test_torch_function_gradients(adj):
+    import torch
+    from phd_amd.adjoint import StepsFunction
+    u0, MP = adj["u0"], adj["MP"]
+    spec = {"MP": MP, "nX": [16, 16], "dX": [1 / 16, 1 / 16],
+            "bcs": [1, 1, 1, 1], "nsteps": NSTEPS, "K": K}
+    u0_t = torch.tensor(u0, requires_grad=True)
+    mu_t = torch.tensor(MP.mu, dtype=torch.float64, requires_grad=True)
+    uT = StepsFunction.apply(u0_t, mu_t, spec)
+    tgt = torch.from_numpy(adj["target"])
+    L = ((uT.view(-1, 14)[:, 0] - tgt) ** 2).sum()
+    L.backward()
+    assert abs(L.item() - adj["L"]) <= 1e-14 * adj["L"]
+    assert np.allclose(u0_t.grad.numpy(), adj["lam0"], rtol=1e-12, atol=1e-14)
+    assert abs(mu_t.grad.item() - adj["gmu"]) <= 1e-12 * abs(adj["gmu"])
