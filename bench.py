@@ -23,6 +23,11 @@ import time
 
 import numpy as np
 
+
+def progress(msg):
+    """One line on stderr per finished stage (stdout carries only the result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -252,9 +257,11 @@ def adjoint_case(n, steps, K, p_amp):
     d = np.zeros((n * n, 14))
     d[:, 0] = np.random.default_rng(1).standard_normal(n * n)
     d = d.ravel()
-    hr = 1e-6
-    fd_rho = (loss_of(run.replay(u0=u0 + hr * d)) - loss_of(run.replay(u0=u0 - hr * d))) / (2 * hr)
     ad_rho = float(np.dot(lam0, d))
+    fd_rho = {}
+    for hr in (1e-6, 1e-7, 1e-8):
+        fd_rho[hr] = (loss_of(run.replay(u0=u0 + hr * d)) -
+                      loss_of(run.replay(u0=u0 - hr * d))) / (2 * hr)
     run.close()
     cu = n * n * steps
 
@@ -266,7 +273,8 @@ def adjoint_case(n, steps, K, p_amp):
             "value": cu / (fsec + bsec), "plain_value": cu / psec,
             "cost_vs_plain": (fsec + bsec) / psec,
             "dL_dmu": {"adjoint": gmu, "fd": fd_mu, "rel_err": rel(gmu, fd_mu)},
-            "dL_drho0_dir": {"adjoint": ad_rho, "fd": fd_rho, "rel_err": rel(ad_rho, fd_rho)}}
+            "dL_drho0_dir": {"adjoint": ad_rho, "fd": fd_rho[1e-6], "rel_err": rel(ad_rho, fd_rho[1e-6]),
+                             "rel_err_by_h": {f"{h:.0e}": rel(ad_rho, v) for h, v in fd_rho.items()}}}
 
 
 def adjoint_leg(n, steps, K):
@@ -282,11 +290,14 @@ def adjoint_leg(n, steps, K):
                      f"mu* = 2e-3, {steps} steps, checkpoint every {K}, FP64, P2, CFL 0.8",
            "cells": n * n, "steps": steps, "K": K,
            "unit": "cell-updates/s (forward and reverse sweep)",
-           "fd_direction_rho0": "standard normal per cell on rho (seed 1), h = 1e-6",
+           "fd_direction_rho0": "standard normal per cell on rho (seed 1); fd and rel_err at "
+                                "h = 1e-6, rel_err_by_h at 1e-6, 1e-7 and 1e-8",
            "note": "the forward sweep solves the predictor to 1e-13 (the gradient is that of "
                    "its solution); plain_s uses the plain form's default tolerances"}
     out["steady_start"] = adjoint_case(n, steps, K, 0.0)
+    progress("adjoint leg, steady start done")
     out["pressure_waves"] = adjoint_case(n, steps, K, 0.05)
+    progress("adjoint leg, pressure waves done")
     out["value"] = out["steady_start"]["value"]
     return out
 
@@ -359,6 +370,7 @@ def main():
     u_w = run.host()
 
     sec, tm = timed_run(run, args.steps)
+    progress(f"plain {n}x{n}: {args.steps} steps in {sec:.3f} s")
 
     cells = n * n
     value = cells * args.steps / sec
@@ -400,17 +412,21 @@ def main():
         sec0, _ = timed_run(run0, args.steps)
         out["bailout0_value"] = cells * args.steps / sec0
         out["bailout0_ms_per_step"] = 1e3 * sec0 / args.steps
+        progress("bailout 0 leg done")
         del run0
     if not args.no_fp32:
         out["fp32_bulk"] = fp32_bulk_leg(args.fp32_n, args.fp32_steps)
+        progress("FP32-bulk leg done")
     if not args.no_batch:
         out["batched"] = batched_leg(args.batch_count, 0, args.batch_sample)
+        progress("batched leg done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps, args.adj_k)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,
                             threads)
+        progress("CPU reference timed")
         psec = cpu_port(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps, threads,
                         args.bailout)
         pval = cells * args.cpu_steps / psec

@@ -206,6 +206,9 @@ int gpr_batch_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_
  * [cells][V]) and returns d loss / d u at the start (lam_0) and d loss / d mu,
  * with the time steps held at the forward run's values; *unconverged counts
  * the cells whose transposed predictor iteration did not converge.
+ * gpr_adj_forward fails with GPR_E_UNSUPPORTED if the global distortion reset
+ * fires during the sweep (it is not differentiated), and with
+ * GPR_E_PREDICTOR / GPR_E_NONFINITE on a failed step.
  * gpr_adj_release frees the checkpoints. */
 int gpr_dev_steps_dt(gpr_ctx *ctx, const double *dts, int64_t nsteps);
 int gpr_adj_forward(gpr_ctx *ctx, int64_t nsteps, int K, double *dts_out);

@@ -33,7 +33,7 @@ struct StepState {
   int done;
   double contorted_tol;
   int pushcount;  // snapshots taken
-  int pad_;
+  int resets;     // global distortion resets so far
 };
 
 __device__ __forceinline__ void atomic_max_pos(unsigned long long *addr, double v) {
@@ -619,6 +619,7 @@ __global__ void k_finalize_dt(StepState *st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long bits = st->contorted ? st->maxspeed2 : st->maxspeed;
   double MAX = __longlong_as_double((long long)bits);
+  if (st->contorted) st->resets += 1;
   double dt = st->cfl / MAX;
   if (st->count <= 5) dt *= 0.2;
   if (st->t + dt > st->tf) dt = st->tf - st->t;

@@ -1728,6 +1728,9 @@ int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
   const int64_t nchk = (nsteps + K - 1) / K;
   HIP_TRY(hipMalloc((void **)&A->chk, sz * nchk));
   HIP_TRY(hipMalloc((void **)&A->seg, sz * K));
+  StepState h0;
+  HIP_TRY(hipMemcpyAsync(&h0, c->d_st, sizeof h0, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   for (int64_t k = 0; k < nsteps; k++) {
     if (k % K == 0)
       HIP_TRY(hipMemcpyAsync(A->chk + (k / K) * nc * c->V, c->run_u, sz, hipMemcpyDeviceToDevice, c->stream));
@@ -1736,8 +1739,16 @@ int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
     int rc = dev_steps_impl(c, 1, &A->dts[k]);
     if (rc) return rc;
   }
+  StepState h1;
+  HIP_TRY(hipMemcpyAsync(&h1, c->d_st, sizeof h1, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (dts_out) std::memcpy(dts_out, A->dts.data(), sizeof(double) * nsteps);
+  // the reverse sweep differentiates renormalisation, not the global reset,
+  // and a failed or non-finite step has no derivative
+  if (h1.resets != h0.resets)
+    return fail(GPR_E_UNSUPPORTED, "a global distortion reset happened in the forward sweep; the adjoint does not differentiate it");
+  if (h1.pred_fail != h0.pred_fail) return fail(GPR_E_PREDICTOR, "a predictor cell failed in the forward sweep");
+  if (h1.nonfinite) return fail(GPR_E_NONFINITE, "the forward sweep became non-finite");
   return GPR_OK;
 }
 

@@ -143,3 +143,18 @@ def test_torch_function_gradients(adj):
     assert abs(L.item() - adj["L"]) <= 1e-14 * adj["L"]
     assert np.allclose(u0_t.grad.numpy(), adj["lam0"], rtol=1e-12, atol=1e-14)
     assert abs(mu_t.grad.item() - adj["gmu"]) <= 1e-12 * abs(adj["gmu"])
+
+
+def test_reset_in_forward_sweep_is_refused():
+    """The reverse sweep differentiates renormalisation but not the global
+    distortion reset: a forward sweep in which the reset happens (here forced
+    by a contortion tolerance below the noisy state's measure) fails with
+    GPR_E_UNSUPPORTED instead of giving a wrong gradient."""
+    from phd_amd._lib import GprError
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, nX, dX, bcs = case(noise=1e-2)
+    run = AdjointRun(u0, MP, nX, dX, bcs, contorted_tol=1e-7)
+    with pytest.raises(GprError) as e:
+        run.forward(3, 2)
+    assert e.value.code == -2
+    run.close()
