@@ -148,6 +148,8 @@ def fp32_bulk_leg(n, steps):
         sec, tm = timed_run(run, steps)
         out[name] = {"value": n * n * steps / sec, "ms_per_step": 1e3 * sec / steps,
                      "phase_ms_per_step": {k: v / tm["steps"] for k, v in tm["ms"].items()},
+                     "picard_iters_per_cell": tm["picard_iters"] / max(tm["cells"], 1),
+                     "fallback_iters": tm["fallback_iters"],
                      "fallback_cells": tm["fallback_cells"]}
         states[name] = run.host().reshape(-1, 14)
         run.close()
@@ -247,6 +249,7 @@ def adjoint_case(n, steps, K, p_amp):
     tic = time.perf_counter()
     lam0, gmu = run.backward(lamT.ravel())
     bsec = time.perf_counter() - tic
+    solve = run.stats()
 
     def loss_of(u):
         d = u.reshape(-1, 14)[:, 0] - target
@@ -272,6 +275,8 @@ def adjoint_case(n, steps, K, p_amp):
             "loss": L, "forward_s": fsec, "backward_s": bsec, "plain_s": psec,
             "value": cu / (fsec + bsec), "plain_value": cu / psec,
             "cost_vs_plain": (fsec + bsec) / psec,
+            "transposed_predictor": {"iters_per_cell_step": float(solve["iters_per_cell_step"]),
+                                     "preconditioned_fraction": float(solve["preconditioned_fraction"])},
             "dL_dmu": {"adjoint": gmu, "fd": fd_mu, "rel_err": rel(gmu, fd_mu)},
             "dL_drho0_dir": {"adjoint": ad_rho, "fd": fd_rho[1e-6], "rel_err": rel(ad_rho, fd_rho[1e-6]),
                              "rel_err_by_h": {f"{h:.0e}": rel(ad_rho, v) for h, v in fd_rho.items()}}}

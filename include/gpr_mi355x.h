@@ -214,6 +214,18 @@ int gpr_dev_steps_dt(gpr_ctx *ctx, const double *dts, int64_t nsteps);
 int gpr_adj_forward(gpr_ctx *ctx, int64_t nsteps, int K, double *dts_out);
 int gpr_adj_backward(gpr_ctx *ctx, const double *lam_T, double *lam_0, double *grad_mu, int32_t *unconverged);
 int gpr_adj_release(gpr_ctx *ctx);
+/* The transposed predictor solve of later gpr_adj_backward calls: with
+ * start_preconditioned = 0 each cell runs the transposed Picard iteration
+ * and switches to the transposed preconditioned form if that does not
+ * converge; with 1 every cell uses the preconditioned form from the start
+ * (both have the same solution; the default is 1, which takes about a third
+ * of the iterations).  tol: stopping tolerance on the change of the iterate,
+ * relative to 1 + |iterate| (default 1e-13). */
+int gpr_adj_set_solver(gpr_ctx *ctx, int start_preconditioned, double tol);
+/* out[3]: of the last gpr_adj_backward, transposed predictor iterations summed
+ * over cells and steps, cell-steps, and cell-steps that used the
+ * preconditioned form */
+int gpr_adj_stats(gpr_ctx *ctx, int64_t *out);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@ SYMBOLS = [
     "gpr_dev_timing", "gpr_batch_create", "gpr_batch_destroy", "gpr_batch_set_state",
     "gpr_batch_get_state", "gpr_batch_set_block", "gpr_batch_steps", "gpr_batch_run",
     "gpr_batch_status", "gpr_dev_steps_dt", "gpr_adj_forward", "gpr_adj_backward",
-    "gpr_adj_release",
+    "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats",
 ]
 
 
@@ -106,6 +106,8 @@ ARGTYPES = {
     "gpr_adj_forward": [_P, _l, _i, _Dp],
     "gpr_adj_backward": [_P, _Dp, _Dp, _Dp, _Ip],
     "gpr_adj_release": [_P],
+    "gpr_adj_set_solver": [_P, _i, _d],
+    "gpr_adj_stats": [_P, _Lp],
 }
 
 _lib = None

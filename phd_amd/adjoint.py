@@ -29,6 +29,7 @@ from ._lib import check, dptr
 from .runner import DeviceRun
 
 TIGHT = dict(dg_tol=1e-13, stiff_tol=1e-13, stiff_it=400)
+START_PRECONDITIONED = 1
 
 
 class AdjointRun:
@@ -41,12 +42,25 @@ class AdjointRun:
         self.L = _lib.load()
         self.dts = None
 
-    def forward(self, nsteps, K):
+    def forward(self, nsteps, K, start_preconditioned=None, tol=1e-13):
         """Advance nsteps steps (CFL time steps), keeping a checkpoint every K;
-        returns u_T."""
+        returns u_T.  start_preconditioned and tol set the transposed
+        predictor solve of the reverse sweep (gpr_adj_set_solver; by default
+        the preconditioned form from the start)."""
         self.dts = np.zeros(int(nsteps))
         check(self.L.gpr_adj_forward(self.run.ctx.h, int(nsteps), int(K), dptr(self.dts)))
+        sp = START_PRECONDITIONED if start_preconditioned is None else int(start_preconditioned)
+        check(self.L.gpr_adj_set_solver(self.run.ctx.h, sp, float(tol)))
         return self.run.host()
+
+    def stats(self):
+        """The last reverse sweep's transposed predictor iterations per
+        cell-step and the fraction of cell-steps in the preconditioned form."""
+        out = np.zeros(3, np.int64)
+        check(self.L.gpr_adj_stats(self.run.ctx.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        n = max(int(out[1]), 1)
+        return {"iters_per_cell_step": out[0] / n, "cell_steps": int(out[1]),
+                "preconditioned_fraction": out[2] / n}
 
     def backward(self, lam_T):
         """d loss / d u_0 and d loss / d mu from d loss / d u_T."""

@@ -42,9 +42,11 @@ struct AdjArgs {
   const uint8_t *mask;  // [(nx+2)(ny+2)]
   double *gmu;          // d loss / d mu, accumulated
   int *fail;            // cells whose transposed Picard iteration did not converge
+  unsigned long long *itc;  // [0] transposed predictor iterations, [1] cells, [2] cells in the preconditioned form
   double dt, dx, dy, mu;
   int nx, ny;
   int it_plain;  // transposed Picard iterations before the preconditioned form
+  int start_pre; // 1: every cell starts in the preconditioned form
   int it_max;    // iterations of the preconditioned form
   double tol;
   int4 bc;
@@ -82,6 +84,55 @@ __global__ void ka_renorm(AdjArgs a, long ncell) {
     }
 #pragma unroll
     for (int v = 0; v < V; v++) a.lam_fv[c * V + v] = out[v];
+  }
+}
+
+// The transposed Riemann terms of one face node in direction D.  With
+// f = smax (qL - qR) + F(qL) + F(qR), smax = max(s(qL), s(qR)) (rusanov) and
+// b = Bint(qL, qR), out[j] = sum_v ls[v] db_v/dx_j + ld[v] df_v/dx_j for the
+// 2V inputs x = (qL, qR).  A seed on one side moves only that side's flux and
+// wave speed, and smax only through the side the maximum takes (a tie takes
+// the left one, as rusanov does), so each side's flux is differentiated with
+// its own V seeds and one wave speed with V seeds.
+template <int N, int D>
+__device__ __forceinline__ void riemann_T(const double *qL, const double *qR, const double *ls, const double *ld,
+                                          const Mat &m, const Basis &K, double *out) {
+  constexpr int V = 14;
+  const double sL = max_abs_eigs<false, double>(qL, m, D), sR = max_abs_eigs<false, double>(qR, m, D);
+  const bool Lmax = !(sL < sR);
+  const double smax = Lmax ? sL : sR;
+  double dd = 0;  // ld . (qL - qR), the weight of d smax
+#pragma unroll
+  for (int v = 0; v < V; v++) dd += ld[v] * (qL[v] - qR[v]);
+  for (int side = 0; side < 2; side++) {
+    const double *qs = side == 0 ? qL : qR;
+    const bool with_speed = side == 0 ? Lmax : !Lmax;
+    for (int j = 0; j < V; j++) {
+      Dual x[V], F[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) x[v] = Dual(qs[v], v == j ? 1.0 : 0.0);
+      NodeState<Dual, false> st;
+      node_state<Dual, false>(x, m, st);
+      flux<Dual, false>(x, st, m, D, F);
+      double g = side == 0 ? smax * ld[j] : -smax * ld[j];
+#pragma unroll
+      for (int v = 0; v < V; v++) g += ld[v] * F[v].d;
+      if (with_speed) g += dd * max_abs_eigs<false, Dual>(x, m, D).d;
+      out[side * V + j] = g;
+    }
+  }
+  for (int j = 0; j < 2 * V; j++) {
+    Dual xL[V], xR[V], b[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      xL[v] = Dual(qL[v], j == v ? 1.0 : 0.0);
+      xR[v] = Dual(qR[v], j == V + v ? 1.0 : 0.0);
+    }
+    bint<N, false, Dual>(xL, xR, D, K, b);
+    double g = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) g += ls[v] * b[v].d;
+    out[j] += g;
   }
 }
 
@@ -153,25 +204,12 @@ __global__ __launch_bounds__(64) void ka_faces(AdjArgs a) {
     ls[v] = -kk * (lL + lR);  // weight of db
     ld[v] = -kk * (lL - lR);  // weight of df
   }
-  for (int j = 0; j < 2 * V; j++) {
-    Dual xL[V], xR[V], f[V], b[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      xL[v] = Dual(qL[v], j == v ? 1.0 : 0.0);
-      xR[v] = Dual(qR[v], j == V + v ? 1.0 : 0.0);
-    }
-    if (d == 0) {
-      rusanov<false, Dual>(xL, xR, 0, a.m, f);
-      bint<N, false, Dual>(xL, xR, 0, K, b);
-    } else {
-      rusanov<false, Dual>(xL, xR, 1, a.m, f);
-      bint<N, false, Dual>(xL, xR, 1, K, b);
-    }
-    double g = 0;
-#pragma unroll
-    for (int v = 0; v < V; v++) g += ls[v] * b[v].d + ld[v] * f[v].d;
-    out[j] = g;
-  }
+  double g[2 * V];
+  if (d == 0)
+    riemann_T<N, 0>(qL, qR, ls, ld, a.m, K, g);
+  else
+    riemann_T<N, 1>(qL, qR, ls, ld, a.m, K, g);
+  for (int j = 0; j < 2 * V; j++) out[j] = g[j];
 }
 
 // The cell terms of one node, h = S(q) - B_x(q) dqx - B_y(q) dqy (dq fixed),
@@ -369,7 +407,8 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
 // the transposed Picard iteration, which converges exactly where the forward
 // one does.  A cell where it does not (NaN, five growing updates in a row, or
 // it_plain iterations) restarts with the transposed form of the forward's
-// preconditioned fallback, on g = W_s^-1 (lam_qh - (dR/dq)^T m):
+// preconditioned fallback (with start_pre, every cell starts in that form),
+// on g = W_s^-1 (lam_qh - (dR/dq)^T m):
 //   m <- m + P^-T g,  P^-1 = DG_MAT^-1 (x) (I - Pi_s) + A_s^-1 (x) Pi_s
 // on the distortion rows (DG_MAT^-1 on the others), with Pi_s and
 // A_s = DG_MAT + dt kappa_s W as stiff_precond builds them at the node's WENO
@@ -453,10 +492,10 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   for (int v = 0; v < V; v++) mu_[v] = m0[v];
   const unsigned long long cellbits = lane_ok ? (((1ull << NN) - 1ull) << (sl * NN)) : 0ull;
   const double *myPre = sPre + (sl * NS + s) * PRE;
-  int phase = active ? 0 : 2;  // 0 transposed Picard, 1 preconditioned, 2 done
-  int it = 0, grow = 0;
+  int phase = active ? (a.start_pre ? 1 : 0) : 2;  // 0 transposed Picard, 1 preconditioned, 2 done
+  int it = 0, itot = 0, grow = 0;
   double prev = 0.0;
-  bool failed = false, have_pre = false;
+  bool failed = false, have_pre = false, used_pre = phase == 1;
   while (__ballot(phase < 2) != 0ull) {
     if (!have_pre && __ballot(phase == 1) != 0ull) {
       if (active && t == 0) stiff_precond<N, false>(a.wh + (e * NS + s) * V, a.m, dt, sPre + (sl * NS + s) * PRE);
@@ -589,6 +628,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
     if (lane_ok)
       for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
     __syncthreads();
+    if (phase < 2) itot++;
     if (phase == 0) {
       it++;
       bool restart = cnan;
@@ -605,6 +645,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
       }
       if (restart) {
         phase = 1;
+        used_pre = true;
         it = 0;
 #pragma unroll
         for (int v = 0; v < V; v++) mu_[v] = m0[v];
@@ -622,6 +663,11 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
     }
   }
   if (failed && node == 0) atomicAdd(a.fail, 1);
+  if (active && node == 0) {
+    atomicAdd(&a.itc[0], (unsigned long long)itot);
+    atomicAdd(&a.itc[1], 1ull);
+    if (used_pre) atomicAdd(&a.itc[2], 1ull);
+  }
   // lam_wh(s) = sum_t psi_t(0) W_s m(t, s); d rhs / d mu through S
   __syncthreads();
   if (lane_ok)

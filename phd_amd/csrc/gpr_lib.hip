@@ -1563,6 +1563,9 @@ struct AdjHost {
   double *chk = nullptr;  // [nchk][cells][V]
   double *seg = nullptr;  // [K][cells][V]
   std::vector<std::pair<void *, size_t>> bufs;
+  int start_pre = 1;      // transposed predictor solve: 1 = preconditioned form from the start
+  double tol = 1e-13;     // its stopping tolerance
+  int64_t stats[3] = {0, 0, 0};  // the last reverse sweep's iterations, cells, preconditioned cells
 };
 
 namespace {
@@ -1667,8 +1670,10 @@ template <int N> int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, do
   a.ny = ny;
   a.wh = (const double *)c->bufs["wh"].first;
   a.it_plain = c->opts.dg_it;
+  a.start_pre = A->start_pre;
   a.it_max = 400;
-  a.tol = 1e-13;
+  a.tol = A->tol;
+  if (!(a.itc = (unsigned long long *)abuf(A, 12, 3 * sizeof(unsigned long long), rc))) return rc;
   a.bc = c->run_bc;
   a.m = c->run_mat;
   HIP_TRY(hipMemsetAsync(a.lam_wh, 0, sizeof(double) * ne * N * N * V, c->stream));
@@ -1767,6 +1772,9 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   HIP_TRY(hipMemcpyAsync(lam, lam_T, sz, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(d_gmu, 0, sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  unsigned long long *d_itc = (unsigned long long *)abuf(A, 12, 3 * sizeof(unsigned long long), rc);
+  if (!d_itc) return rc;
+  HIP_TRY(hipMemsetAsync(d_itc, 0, 3 * sizeof(unsigned long long), c->stream));
   const int K = A->K;
   const int64_t nchk = (A->nsteps + K - 1) / K;
   int nX[2] = {c->run_nx, c->run_ny};
@@ -1798,8 +1806,26 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   HIP_TRY(hipMemcpyAsync(lam_0, lam, sz, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(&nf, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long itc[3];
+  HIP_TRY(hipMemcpyAsync(itc, d_itc, sizeof itc, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 3; k++) A->stats[k] = (int64_t)itc[k];
   if (unconverged) *unconverged = nf;
+  return GPR_OK;
+}
+
+int gpr_adj_set_solver(gpr_ctx *c, int start_preconditioned, double tol) {
+  if (!c || !c->adj) return fail(GPR_E_ARG, "gpr_adj_forward has not been called");
+  if (start_preconditioned < 0 || start_preconditioned > 1 || !(tol > 0) || !(tol < 1e-3))
+    return fail(GPR_E_ARG, "start_preconditioned must be 0 or 1 and 0 < tol < 1e-3");
+  c->adj->start_pre = start_preconditioned;
+  c->adj->tol = tol;
+  return GPR_OK;
+}
+
+int gpr_adj_stats(gpr_ctx *c, int64_t *out) {
+  if (!c || !c->adj || !out) return fail(GPR_E_ARG, "gpr_adj_forward has not been called");
+  for (int k = 0; k < 3; k++) out[k] = c->adj->stats[k];
   return GPR_OK;
 }
 

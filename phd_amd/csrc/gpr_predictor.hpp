@@ -101,32 +101,52 @@ __device__ __forceinline__ void stiff_precond(const double *ws, const Mat &m, do
     rJ = m.p.calpha2 * s.T / (m.p.kappa * s.r);
     if (!(rJ >= 0.0) || !(rJ < 1e300)) rJ = 0.0;
   }
+#pragma unroll
   for (int which = 0; which < (TH ? 2 : 1); which++) {
     double rate = which == 0 ? kap : rJ;
+    // Gauss-Jordan with partial pivoting.  Every index is a compile-time
+    // constant (the pivot row is swapped in by predicated exchanges), so the
+    // matrix stays in registers rather than in scratch memory.
     double M[N][2 * N];
+#pragma unroll
     for (int i = 0; i < N; i++)
+#pragma unroll
       for (int j = 0; j < 2 * N; j++)
         M[i][j] = j < N ? K.dgmat[i][j] + (i == j ? dt * rate * K.w[i] : 0.0) : (j - N == i ? 1.0 : 0.0);
+#pragma unroll
     for (int c = 0; c < N; c++) {
       int piv = c;
+      double best = fabs(M[c][c]);
+#pragma unroll
       for (int r = c + 1; r < N; r++)
-        if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
-      if (piv != c)
-        for (int j = 0; j < 2 * N; j++) {
-          double tt = M[c][j];
-          M[c][j] = M[piv][j];
-          M[piv][j] = tt;
+        if (fabs(M[r][c]) > best) {
+          best = fabs(M[r][c]);
+          piv = r;
         }
+#pragma unroll
+      for (int r = c + 1; r < N; r++)
+        if (piv == r)
+#pragma unroll
+          for (int j = 0; j < 2 * N; j++) {
+            double tt = M[c][j];
+            M[c][j] = M[r][j];
+            M[r][j] = tt;
+          }
       double dg = M[c][c];
+#pragma unroll
       for (int j = 0; j < 2 * N; j++) M[c][j] /= dg;
+#pragma unroll
       for (int r = 0; r < N; r++) {
         if (r == c) continue;
         double f = M[r][c];
+#pragma unroll
         for (int j = 0; j < 2 * N; j++) M[r][j] -= f * M[c][j];
       }
     }
     double *dst = which == 0 ? out : out + 18;
+#pragma unroll
     for (int i = 0; i < N; i++)
+#pragma unroll
       for (int j = 0; j < N; j++) dst[i * N + j] = M[i][N + j];
   }
   double X[9];

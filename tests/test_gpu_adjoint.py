@@ -158,3 +158,29 @@ def test_reset_in_forward_sweep_is_refused():
         run.forward(3, 2)
     assert e.value.code == -2
     run.close()
+
+
+def test_transposed_solver_forms_agree():
+    """The transposed predictor solve started in the transposed Picard form
+    and started in the preconditioned form reach the same fixed point: the
+    gradients agree to round-off, and the stats report which form ran."""
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, nX, dX, bcs = case()
+    target = 1.0 + 0.01 * np.random.default_rng(5).standard_normal(nX[0] * nX[1])
+    out = {}
+    for sp in (0, 1):
+        run = AdjointRun(u0, MP, nX, dX, bcs)
+        uT = run.forward(NSTEPS, K, start_preconditioned=sp)
+        _, r = loss(uT, target)
+        lamT = np.zeros_like(uT).reshape(-1, 14)
+        lamT[:, 0] = 2 * r
+        lam0, gmu = run.backward(lamT.ravel())
+        out[sp] = (lam0, gmu, run.stats())
+        run.close()
+    (l0, g0, s0), (l1, g1, s1) = out[0], out[1]
+    print(f"iterations per cell-step: Picard first {s0['iters_per_cell_step']:.2f}, "
+          f"preconditioned {s1['iters_per_cell_step']:.2f}")
+    assert s1["preconditioned_fraction"] == 1.0
+    assert s0["cell_steps"] == s1["cell_steps"] == NSTEPS * (nX[0] + 2) * (nX[1] + 2) - 4 * NSTEPS
+    assert np.max(np.abs(l1 - l0)) <= 1e-11 * np.max(np.abs(l0))
+    assert abs(g1 - g0) <= 1e-11 * abs(g0)
