@@ -220,8 +220,11 @@ int gpr_adj_release(gpr_ctx *ctx);
  * converge; with 1 every cell uses the preconditioned form from the start
  * (both have the same solution; the default is 1, which takes about a third
  * of the iterations).  tol: stopping tolerance on the change of the iterate,
- * relative to 1 + |iterate| (default 1e-13). */
-int gpr_adj_set_solver(gpr_ctx *ctx, int start_preconditioned, double tol);
+ * relative to 1 + |iterate| (default 1e-13).  dual_local = 1 takes the local
+ * derivatives of the node physics by forward-mode dual numbers, one seed per
+ * input, instead of the hand-written reverse mode (default 0; same values to
+ * round-off, several times slower). */
+int gpr_adj_set_solver(gpr_ctx *ctx, int start_preconditioned, double tol, int dual_local);
 /* out[3]: of the last gpr_adj_backward, transposed predictor iterations summed
  * over cells and steps, cell-steps, and cell-steps that used the
  * preconditioned form */

@@ -106,7 +106,7 @@ ARGTYPES = {
     "gpr_adj_forward": [_P, _l, _i, _Dp],
     "gpr_adj_backward": [_P, _Dp, _Dp, _Dp, _Ip],
     "gpr_adj_release": [_P],
-    "gpr_adj_set_solver": [_P, _i, _d],
+    "gpr_adj_set_solver": [_P, _i, _d, _i],
     "gpr_adj_stats": [_P, _Lp],
 }
 

@@ -42,15 +42,16 @@ class AdjointRun:
         self.L = _lib.load()
         self.dts = None
 
-    def forward(self, nsteps, K, start_preconditioned=None, tol=1e-13):
+    def forward(self, nsteps, K, start_preconditioned=None, tol=1e-13, dual_local=False):
         """Advance nsteps steps (CFL time steps), keeping a checkpoint every K;
         returns u_T.  start_preconditioned and tol set the transposed
         predictor solve of the reverse sweep (gpr_adj_set_solver; by default
-        the preconditioned form from the start)."""
+        the preconditioned form from the start); dual_local takes the local
+        derivatives by dual numbers instead of the hand-written reverse mode."""
         self.dts = np.zeros(int(nsteps))
         check(self.L.gpr_adj_forward(self.run.ctx.h, int(nsteps), int(K), dptr(self.dts)))
         sp = START_PRECONDITIONED if start_preconditioned is None else int(start_preconditioned)
-        check(self.L.gpr_adj_set_solver(self.run.ctx.h, sp, float(tol)))
+        check(self.L.gpr_adj_set_solver(self.run.ctx.h, sp, float(tol), int(bool(dual_local))))
         return self.run.host()
 
     def stats(self):

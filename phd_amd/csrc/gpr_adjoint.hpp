@@ -53,6 +53,213 @@ struct AdjArgs {
   Mat m;
 };
 
+// Reverse-mode derivative of one node's terms for the fluid the adjoint
+// covers (V = 14, no thermal terms, Newtonian theta1inv, any EOS):
+//   out[j] = d/dq_j [ ah . (S(q) - B_x(v) dqx - B_y(v) dqy)     (H)
+//                   + aF . F_x(q)                               (FX)
+//                   + aG . F_y(q) ]                             (FY)
+// with the weights and dq held fixed.  It recomputes the forward quantities
+// of node_state / source / flux (gpr_device.hpp) and runs them backwards;
+// the EOS scalars of rho (c_s^2, Gamma, e_ref, p_ref) take their derivatives
+// from one dual-number evaluation.  The stress and dev G are symmetric
+// functions of q, so their full 3x3 formulas are differentiated (the forward
+// code fills one triangle and mirrors it, which is the same function of q).
+template <bool H, bool FX, bool FY>
+__device__ __forceinline__ void node_vjp(const double *q, const double *dqx, const double *dqy, const Mat &m,
+                                         const double *ah, const double *aF, const double *aG, double *out) {
+  const gpr_par &p = m.p;
+  // ---- forward
+  const double r = q[0], ir = 1.0 / r;
+  const double v[3] = {q[2] * ir, q[3] * ir, q[4] * ir};
+  const double *A = q + 5;
+  double G[9], D[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) G[i * 3 + j] = A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j];
+  const double tr3 = (G[0] + G[4] + G[8]) / 3.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) D[k] = G[k] - (k % 4 == 0 ? tr3 : 0.0);
+  double devGsq = 0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) devGsq += D[k] * D[k];
+  const Dual rd(r, 1.0);
+  const Dual cs2d = c_s2(rd, m), Gmd = Gamma_MG(rd, p), erd = e_ref(rd, p), prd = p_ref(rd, p);
+  const double cs2 = cs2d.v, Gm = Gmd.v;
+  double AD[9], EA[9], sig[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      AD[i * 3 + j] = A[i * 3 + 0] * D[0 * 3 + j] + A[i * 3 + 1] * D[1 * 3 + j] + A[i * 3 + 2] * D[2 * 3 + j];
+      EA[i * 3 + j] = cs2 * AD[i * 3 + j];
+    }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      sig[i * 3 + j] = -r * (EA[0 * 3 + i] * A[0 * 3 + j] + EA[1 * 3 + i] * A[1 * 3 + j] + EA[2 * 3 + i] * A[2 * 3 + j]);
+  const double C0 = (1.0 - p.beta / Gm) * cs2;
+  const double e = q[1] * ir - 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - 0.25 * C0 * devGsq;
+  const double pr = (e - erd.v) * r * Gm + prd.v;
+  // ---- reverse
+  double qb[14], vb[3] = {0, 0, 0}, sigb[9], EAb[9], Ab[9];
+#pragma unroll
+  for (int k = 0; k < 14; k++) qb[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    sigb[k] = 0.0;
+    EAb[k] = 0.0;
+    Ab[k] = 0.0;
+  }
+  double pb = 0, irb = 0, rb = 0, thb = 0;
+  if constexpr (H) {
+    // S_A = -EA theta1inv; theta1inv = det(A)^(5/3) mu_fac
+    const double dA = det3(A), c13 = cbrt(dA);
+    const double th = dA * c13 * c13 * m.mu_fac;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      EAb[k] -= ah[5 + k] * th;
+      thb -= ah[5 + k] * EA[k];
+    }
+    const double f = thb * m.mu_fac * (5.0 / 3.0) * c13 * c13;  // d theta / d det(A)
+    Ab[0] += f * (A[4] * A[8] - A[5] * A[7]);
+    Ab[1] += f * (A[5] * A[6] - A[3] * A[8]);
+    Ab[2] += f * (A[3] * A[7] - A[4] * A[6]);
+    Ab[3] += f * (A[2] * A[7] - A[1] * A[8]);
+    Ab[4] += f * (A[0] * A[8] - A[2] * A[6]);
+    Ab[5] += f * (A[1] * A[6] - A[0] * A[7]);
+    Ab[6] += f * (A[1] * A[5] - A[2] * A[4]);
+    Ab[7] += f * (A[2] * A[3] - A[0] * A[5]);
+    Ab[8] += f * (A[0] * A[4] - A[1] * A[3]);
+    // -B_d(v) dq_d is linear in v: its v_l coefficient is -B_d(e_l) dq_d
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      const double *x = d == 0 ? dqx : dqy;
+      if (x == nullptr) continue;
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        const double el[3] = {l == 0 ? 1.0 : 0.0, l == 1 ? 1.0 : 0.0, l == 2 ? 1.0 : 0.0};
+        double o[14];
+#pragma unroll
+        for (int k = 0; k < 14; k++) o[k] = 0.0;
+        bdot_sub(el, x, d, 1.0, o);
+        double acc = 0;
+#pragma unroll
+        for (int k = 5; k < 14; k++) acc += ah[k] * o[k];
+        vb[l] += acc;
+      }
+    }
+  }
+  // fluxes (flux in gpr_device.hpp)
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    if ((d == 0 && !FX) || (d == 1 && !FY)) continue;
+    const double *a = d == 0 ? aF : aG;
+    const double vd = v[d];
+    qb[2 + d] += a[0];
+    // F1 = vd (q1 + p) - (sum_i sig_id q_{2+i}) / rho
+    double sq = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) sq += sig[i * 3 + d] * q[2 + i];
+    vb[d] += a[1] * (q[1] + pr);
+    qb[1] += a[1] * vd;
+    pb += a[1] * vd;
+    irb -= a[1] * sq;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      sigb[i * 3 + d] -= a[1] * q[2 + i] * ir;
+      qb[2 + i] -= a[1] * sig[i * 3 + d] * ir;
+      // F_{2+i} = vd q_{2+i} - sig_id (+ p on i = d)
+      vb[d] += a[2 + i] * q[2 + i];
+      qb[2 + i] += a[2 + i] * vd;
+      sigb[i * 3 + d] -= a[2 + i];
+    }
+    pb += a[2 + d];
+    // F_{5+d+3k} = A row k . v
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double w = a[5 + d + 3 * k];
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        Ab[3 * k + l] += w * v[l];
+        vb[l] += w * A[3 * k + l];
+      }
+    }
+  }
+  // p = (e - e_ref) rho Gamma + p_ref
+  const double eb = pb * r * Gm;
+  double Gmb = pb * (e - erd.v) * r;
+  rb += pb * ((e - erd.v) * Gm - erd.d * r * Gm + prd.d);
+  // e = q1 / rho - |v|^2 / 2 - C0 devGsq / 4,  C0 = (1 - beta / Gamma) cs2
+  qb[1] += eb * ir;
+  irb += eb * q[1];
+#pragma unroll
+  for (int i = 0; i < 3; i++) vb[i] -= eb * v[i];
+  const double C0b = -0.25 * eb * devGsq, devGsqb = -0.25 * eb * C0;
+  Gmb += C0b * p.beta / (Gm * Gm) * cs2;
+  double cs2b = C0b * (1.0 - p.beta / Gm);
+  // sig = -rho EA^T A
+  double Db[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    rb += sigb[k] * sig[k] * ir;
+    Db[k] = 2.0 * devGsqb * D[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double e1 = 0, a1 = 0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        e1 += A[k * 3 + j] * sigb[i * 3 + j];
+        a1 += EA[k * 3 + j] * sigb[j * 3 + i];
+      }
+      EAb[k * 3 + i] -= r * e1;  // d/dEA_ki of -rho sum_j EA_ki A_kj sigb_ij
+      Ab[k * 3 + i] -= r * a1;   // d/dA_ki of -rho sum_j EA_kj A_ki sigb_ji
+    }
+  // EA = cs2 A D
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      cs2b += EAb[i * 3 + j] * AD[i * 3 + j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        Ab[i * 3 + k] += cs2 * EAb[i * 3 + j] * D[k * 3 + j];
+        Db[k * 3 + j] += cs2 * A[i * 3 + k] * EAb[i * 3 + j];
+      }
+    }
+  // D = G - tr(G)/3 I, G = A^T A
+  const double tb = (Db[0] + Db[4] + Db[8]) / 3.0;
+  double Gb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) Gb[k] = Db[k] - (k % 4 == 0 ? tb : 0.0);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) acc += (Gb[i * 3 + j] + Gb[j * 3 + i]) * A[k * 3 + j];
+      Ab[k * 3 + i] += acc;
+    }
+  rb += cs2b * cs2d.d + Gmb * Gmd.d;
+  // v = (rho v) / rho, 1 / rho
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    qb[2 + i] += vb[i] * ir;
+    irb += vb[i] * q[2 + i];
+  }
+  rb -= irb * ir * ir;
+  qb[0] += rb;
+#pragma unroll
+  for (int k = 0; k < 9; k++) qb[5 + k] += Ab[k];
+#pragma unroll
+  for (int k = 0; k < 14; k++) out[k] = qb[k];
+}
+
 // Renormalisation of A (k_post / utils.cpp:6-23) on a dual state
 template <typename R> __device__ __forceinline__ void renorm_state(R *q, double rho0) {
   if (q[0] > 0) {
@@ -87,6 +294,25 @@ __global__ void ka_renorm(AdjArgs a, long ncell) {
   }
 }
 
+// (B_d(v))^T w on the distortion rows: out[j] = w . (B_d e_j), j = 5..13
+__device__ __forceinline__ void bdot_T(const double *vel, const double *w, int d, double *out) {
+  for (int j = 0; j < 14; j++) out[j] = 0.0;
+#pragma unroll
+  for (int j = 5; j < 14; j++) {
+    double e[14], o[14];
+#pragma unroll
+    for (int v = 0; v < 14; v++) {
+      e[v] = v == j ? 1.0 : 0.0;
+      o[v] = 0.0;
+    }
+    bdot_sub(vel, e, d, -1.0, o);  // o = B_d e_j
+    double s = 0;
+#pragma unroll
+    for (int v = 0; v < 14; v++) s += w[v] * o[v];
+    out[j] = s;
+  }
+}
+
 // The transposed Riemann terms of one face node in direction D.  With
 // f = smax (qL - qR) + F(qL) + F(qR), smax = max(s(qL), s(qR)) (rusanov) and
 // b = Bint(qL, qR), out[j] = sum_v ls[v] db_v/dx_j + ld[v] df_v/dx_j for the
@@ -94,7 +320,7 @@ __global__ void ka_renorm(AdjArgs a, long ncell) {
 // wave speed, and smax only through the side the maximum takes (a tie takes
 // the left one, as rusanov does), so each side's flux is differentiated with
 // its own V seeds and one wave speed with V seeds.
-template <int N, int D>
+template <int N, int D, bool DUAL>
 __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, const double *ls, const double *ld,
                                           const Mat &m, const Basis &K, double *out) {
   constexpr int V = 14;
@@ -107,39 +333,93 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
   for (int side = 0; side < 2; side++) {
     const double *qs = side == 0 ? qL : qR;
     const bool with_speed = side == 0 ? Lmax : !Lmax;
+    double fl[V];  // ld . dF(qs)/dq_j
+    if constexpr (!DUAL) node_vjp<false, D == 0, D == 1>(qs, nullptr, nullptr, m, nullptr, ld, ld, fl);
     for (int j = 0; j < V; j++) {
-      Dual x[V], F[V];
+      Dual x[V];
 #pragma unroll
       for (int v = 0; v < V; v++) x[v] = Dual(qs[v], v == j ? 1.0 : 0.0);
-      NodeState<Dual, false> st;
-      node_state<Dual, false>(x, m, st);
-      flux<Dual, false>(x, st, m, D, F);
       double g = side == 0 ? smax * ld[j] : -smax * ld[j];
+      if constexpr (DUAL) {
+        Dual F[V];
+        NodeState<Dual, false> st;
+        node_state<Dual, false>(x, m, st);
+        flux<Dual, false>(x, st, m, D, F);
 #pragma unroll
-      for (int v = 0; v < V; v++) g += ld[v] * F[v].d;
+        for (int v = 0; v < V; v++) g += ld[v] * F[v].d;
+      } else {
+        g += fl[j];
+      }
       if (with_speed) g += dd * max_abs_eigs<false, Dual>(x, m, D).d;
       out[side * V + j] = g;
     }
   }
-  for (int j = 0; j < 2 * V; j++) {
-    Dual xL[V], xR[V], b[V];
+  if constexpr (DUAL) {
+    for (int j = 0; j < 2 * V; j++) {
+      Dual xL[V], xR[V], b[V];
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      xL[v] = Dual(qL[v], j == v ? 1.0 : 0.0);
-      xR[v] = Dual(qR[v], j == V + v ? 1.0 : 0.0);
+      for (int v = 0; v < V; v++) {
+        xL[v] = Dual(qL[v], j == v ? 1.0 : 0.0);
+        xR[v] = Dual(qR[v], j == V + v ? 1.0 : 0.0);
+      }
+      bint<N, false, Dual>(xL, xR, D, K, b);
+      double g = 0;
+#pragma unroll
+      for (int v = 0; v < V; v++) g += ls[v] * b[v].d;
+      out[j] += g;
     }
-    bint<N, false, Dual>(xL, xR, D, K, b);
-    double g = 0;
+  } else {
+    // b = sum_i w_i B_D(v_i) dq with dq = qR - qL and v_i = m_i / r_i on the
+    // straight path (bint): linear in dq, and linear in v_i, whose momentum
+    // and density are (1 - x_i) qL + x_i qR
+    double dq[V];
 #pragma unroll
-    for (int v = 0; v < V; v++) g += ls[v] * b[v].d;
-    out[j] += g;
+    for (int v = 0; v < V; v++) dq[v] = qR[v] - qL[v];
+    double cv[3];  // ls . B_D(e_c) dq
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double ec[3] = {c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, c == 2 ? 1.0 : 0.0};
+      double o[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) o[v] = 0.0;
+      bdot_sub(ec, dq, D, -1.0, o);
+      double acc = 0;
+#pragma unroll
+      for (int v = 5; v < V; v++) acc += ls[v] * o[v];
+      cv[c] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const double xi = K.x[i], wi = K.w[i];
+      const double r = qL[0] + xi * (qR[0] - qL[0]);
+      double vv[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) vv[c] = (qL[2 + c] + xi * (qR[2 + c] - qL[2 + c])) / r;
+      double gdq[V];  // w_i B_D(v_i)^T ls
+      bdot_T(vv, ls, D, gdq);
+      double rbar = 0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double mbar = wi * cv[c] / r;
+        out[2 + c] += (1.0 - xi) * mbar;
+        out[V + 2 + c] += xi * mbar;
+        rbar -= mbar * vv[c];
+      }
+      out[0] += (1.0 - xi) * rbar;
+      out[V] += xi * rbar;
+#pragma unroll
+      for (int v = 5; v < V; v++) {
+        out[v] -= wi * gdq[v];
+        out[V + v] += wi * gdq[v];
+      }
+    }
   }
 }
 
 // Faces: one lane per (face, time node, face node), as k_fv_faces.  For the
 // face's lo = k (b + f) (taken from its left cell) and hi = k (b - f) (from its
 // right cell), the adjoints of qL and qR.
-template <int N, int ND>
+template <int N, int ND, bool DUAL>
 __global__ __launch_bounds__(64) void ka_faces(AdjArgs a) {
   constexpr int V = 14;
   constexpr int NS = ND == 1 ? N : N * N;
@@ -206,9 +486,9 @@ __global__ __launch_bounds__(64) void ka_faces(AdjArgs a) {
   }
   double g[2 * V];
   if (d == 0)
-    riemann_T<N, 0>(qL, qR, ls, ld, a.m, K, g);
+    riemann_T<N, 0, DUAL>(qL, qR, ls, ld, a.m, K, g);
   else
-    riemann_T<N, 1>(qL, qR, ls, ld, a.m, K, g);
+    riemann_T<N, 1, DUAL>(qL, qR, ls, ld, a.m, K, g);
   for (int j = 0; j < 2 * V; j++) out[j] = g[j];
 }
 
@@ -230,29 +510,11 @@ __device__ __forceinline__ void node_h(const R *q, const double *dqx, const doub
   }
 }
 
-// (B_d(v))^T w on the distortion rows: out[j] = w . (B_d e_j), j = 5..13
-__device__ __forceinline__ void bdot_T(const double *vel, const double *w, int d, double *out) {
-  for (int j = 0; j < 14; j++) out[j] = 0.0;
-#pragma unroll
-  for (int j = 5; j < 14; j++) {
-    double e[14], o[14];
-#pragma unroll
-    for (int v = 0; v < 14; v++) {
-      e[v] = v == j ? 1.0 : 0.0;
-      o[v] = 0.0;
-    }
-    bdot_sub(vel, e, d, -1.0, o);  // o = B_d e_j
-    double s = 0;
-#pragma unroll
-    for (int v = 0; v < 14; v++) s += w[v] * o[v];
-    out[j] = s;
-  }
-}
 
 // Cells: one lane per space-time node of every extended cell, as k_predictor's
 // layout.  lam_qh = (cell terms)^T lam_fv for real cells + face extrapolation
 // adjoints; the mu-derivative of the cell terms into gmu.
-template <int N, int ND>
+template <int N, int ND, bool DUAL>
 __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
   constexpr int V = 14;
   constexpr int NS = PredShape<N, ND>::NS;
@@ -311,15 +573,19 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
     const double wgt = ND == 2 ? a.dt * K.w[t] * K.w[ia] * K.w[ib] : a.dt * K.w[t] * K.w[ia];
 #pragma unroll
     for (int v = 0; v < V; v++) wl[v] = wgt * a.lam_fv[c * V + v];
-    for (int j = 0; j < V; j++) {
-      Dual x[V], h[V];
+    if constexpr (DUAL) {
+      for (int j = 0; j < V; j++) {
+        Dual x[V], h[V];
 #pragma unroll
-      for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
-      node_h<Dual, ND>(x, dqx, dqy, a.m, h);
-      double s2 = 0;
+        for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
+        node_h<Dual, ND>(x, dqx, dqy, a.m, h);
+        double s2 = 0;
 #pragma unroll
-      for (int v = 0; v < V; v++) s2 += wl[v] * h[v].d;
-      g[j] = s2;
+        for (int v = 0; v < V; v++) s2 += wl[v] * h[v].d;
+        g[j] = s2;
+      }
+    } else {
+      node_vjp<true, false, false>(q, dqx, ND == 2 ? dqy : nullptr, a.m, wl, nullptr, nullptr, g);
     }
     // the dq inputs: h depends on dqx through -B_x(v) dqx
     NodeState<double, false> st;
@@ -415,7 +681,7 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
 // state; Pi_s is an orthogonal projection, so Pi_s^T = Pi_s.  Both forms have
 // the same fixed point.  Then lam_wh = (d rhs / d w)^T m =
 // sum_t psi_t(0) W_s m(t, s), and gmu += (d rhs / d mu)^T m.
-template <int N, int ND>
+template <int N, int ND, bool DUAL>
 __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   constexpr int V = 14;
   constexpr int NS = PredShape<N, ND>::NS;
@@ -424,6 +690,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   __shared__ double sQ[CPW * NN * V];  // q (for dq), then the rhs adjoint pieces
   __shared__ double sM[CPW * NN * V];  // m
   __shared__ double sA[CPW * NN * V];  // scratch: a_F, a_G, dq adjoints
+  __shared__ double sM0[CPW * NN * V];  // the starting iterate, for a restart
   __shared__ double sRat[WAVE];
   constexpr int PRE = 18;
   __shared__ double sPre[CPW * NS * PRE];
@@ -467,10 +734,9 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
     dqx[v] = ax / a.dx;
     dqy[v] = ND == 2 ? ay / a.dy : 0.0;
   }
-  NodeState<double, false> st;
-  node_state<double, false>(q, a.m, st);
-  double S[V];
-  source<double, false>(q, st, a.m, S);
+  double vel[3];  // the node's velocity, for the transposed B products
+#pragma unroll
+  for (int k = 0; k < 3; k++) vel[k] = q[2 + k] / q[0];
   // m0 = DG_U2^-T lam_qh: m(t, s) = sum_k DG_MAT^-1[k][t] x(k, s) / W_s
   auto solveT = [&](const double *x, double *out) {
     __syncthreads();
@@ -486,10 +752,13 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
       out[v] = acc / Ws;
     }
   };
-  double m0[V], mu_[V];
-  solveT(lq, m0);
+  // m0 is kept in LDS for a restart
+  double mu_[V];
+  solveT(lq, mu_);
+  double *myM0 = sM0 + sl * NN * V;
+  if (lane_ok)
 #pragma unroll
-  for (int v = 0; v < V; v++) mu_[v] = m0[v];
+    for (int v = 0; v < V; v++) myM0[node * V + v] = mu_[v];
   const unsigned long long cellbits = lane_ok ? (((1ull << NN) - 1ull) << (sl * NN)) : 0ull;
   const double *myPre = sPre + (sl * NS + s) * PRE;
   int phase = active ? (a.start_pre ? 1 : 0) : 2;  // 0 transposed Picard, 1 preconditioned, 2 done
@@ -524,7 +793,8 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
       aG[v] = ND == 2 ? -dt * Wt * K.w[ia] / a.dy * sG2 : 0.0;
     }
     double g[V];
-    for (int j = 0; j < V; j++) {
+    if constexpr (!DUAL) node_vjp<true, true, ND == 2>(q, dqx, ND == 2 ? dqy : nullptr, a.m, ah, aF, aG, g);
+    for (int j = 0; j < (DUAL ? V : 0); j++) {
       Dual x[V], h[V], F[V], G[V];
 #pragma unroll
       for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
@@ -552,8 +822,8 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
     }
     // the dq paths: h -= B_d(v) dq_d, dq_x at node n = sum_k der[a_n][k] q_{t,k,b} / dx
     double tx[V], ty[V];
-    bdot_T(st.v, ah, 0, tx);
-    if (ND == 2) bdot_T(st.v, ah, 1, ty);
+    bdot_T(vel, ah, 0, tx);
+    if (ND == 2) bdot_T(vel, ah, 1, ty);
     __syncthreads();
     if (lane_ok)
 #pragma unroll
@@ -648,7 +918,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
         used_pre = true;
         it = 0;
 #pragma unroll
-        for (int v = 0; v < V; v++) mu_[v] = m0[v];
+        for (int v = 0; v < V; v++) mu_[v] = myM0[node * V + v];
       }
     } else if (phase == 1) {
       it++;
@@ -684,6 +954,10 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
         a.lam_wh[(e * NS + s) * V + v] = acc;
       }
     }
+    NodeState<double, false> st;
+    node_state<double, false>(q, a.m, st);
+    double S[V];
+    source<double, false>(q, st, a.m, S);
     double gm = 0;
 #pragma unroll
     for (int k = 5; k < 14; k++) gm += dt * Wt * Ws * mu_[k] * (-S[k] / a.mu);

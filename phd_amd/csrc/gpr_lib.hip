@@ -1564,6 +1564,7 @@ struct AdjHost {
   double *seg = nullptr;  // [K][cells][V]
   std::vector<std::pair<void *, size_t>> bufs;
   int start_pre = 1;      // transposed predictor solve: 1 = preconditioned form from the start
+  int dual_local = 0;     // 1: local derivatives by dual numbers instead of node_vjp
   double tol = 1e-13;     // its stopping tolerance
   int64_t stats[3] = {0, 0, 0};  // the last reverse sweep's iterations, cells, preconditioned cells
 };
@@ -1681,10 +1682,17 @@ template <int N> int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, do
   HIP_TRY(hipMemsetAsync(a.lam_ub, 0, sizeof(double) * nb * V, c->stream));
   hipLaunchKernelGGL(ka_renorm, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a, nc);
   constexpr int FPW = FaceShape<N, false, 2>::FPW;
-  hipLaunchKernelGGL((ka_faces<N, 2>), dim3((nfx + nfy + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
   constexpr int CPW = PredShape<N, 2>::CPW;
-  hipLaunchKernelGGL((ka_cells<N, 2>), dim3((ne + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a, ne);
-  hipLaunchKernelGGL((ka_predictor<N, 2>), dim3((ne + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a, ne);
+  const dim3 gf((nfx + nfy + FPW - 1) / FPW), gc((ne + CPW - 1) / CPW);
+  if (A->dual_local) {
+    hipLaunchKernelGGL((ka_faces<N, 2, true>), gf, dim3(WAVE), 0, c->stream, a);
+    hipLaunchKernelGGL((ka_cells<N, 2, true>), gc, dim3(WAVE), 0, c->stream, a, ne);
+    hipLaunchKernelGGL((ka_predictor<N, 2, true>), gc, dim3(WAVE), 0, c->stream, a, ne);
+  } else {
+    hipLaunchKernelGGL((ka_faces<N, 2, false>), gf, dim3(WAVE), 0, c->stream, a);
+    hipLaunchKernelGGL((ka_cells<N, 2, false>), gc, dim3(WAVE), 0, c->stream, a, ne);
+    hipLaunchKernelGGL((ka_predictor<N, 2, false>), gc, dim3(WAVE), 0, c->stream, a, ne);
+  }
   hipLaunchKernelGGL((ka_weno_y<N>), dim3(grid_for(ne * N * V, 256)), dim3(256), 0, c->stream, a);
   hipLaunchKernelGGL((ka_weno_x<N>), dim3(grid_for(nux * V, 256)), dim3(256), 0, c->stream, a);
   HIP_TRY(hipMemcpyAsync(a.lam0, a.lam_fv, sizeof(double) * nc * V, hipMemcpyDeviceToDevice, c->stream));
@@ -1814,12 +1822,14 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   return GPR_OK;
 }
 
-int gpr_adj_set_solver(gpr_ctx *c, int start_preconditioned, double tol) {
+int gpr_adj_set_solver(gpr_ctx *c, int start_preconditioned, double tol, int dual_local) {
   if (!c || !c->adj) return fail(GPR_E_ARG, "gpr_adj_forward has not been called");
-  if (start_preconditioned < 0 || start_preconditioned > 1 || !(tol > 0) || !(tol < 1e-3))
-    return fail(GPR_E_ARG, "start_preconditioned must be 0 or 1 and 0 < tol < 1e-3");
+  if (start_preconditioned < 0 || start_preconditioned > 1 || !(tol > 0) || !(tol < 1e-3) || dual_local < 0 ||
+      dual_local > 1)
+    return fail(GPR_E_ARG, "start_preconditioned and dual_local must be 0 or 1, and 0 < tol < 1e-3");
   c->adj->start_pre = start_preconditioned;
   c->adj->tol = tol;
+  c->adj->dual_local = dual_local;
   return GPR_OK;
 }
 
