@@ -580,6 +580,10 @@ int fv_buffers(gpr_ctx *c, FvArgs &a) {
 __global__ void k_set_dt(StepState *st, double dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) st->dt = dt;
 }
+// *dst = the dt the coming step takes
+__global__ void k_record_dt(const StepState *st, double *dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = st->dt;
+}
 int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts);
 
 }  // namespace
@@ -1562,6 +1566,8 @@ struct AdjHost {
   std::vector<double> dts;
   double *chk = nullptr;  // [nchk][cells][V]
   double *seg = nullptr;  // [K][cells][V]
+  double *qhs = nullptr;  // [K][predictor cells][NN][V], the segment's predictor results (or none)
+  size_t qh_bytes = 0;
   std::vector<std::pair<void *, size_t>> bufs;
   int start_pre = 1;      // transposed predictor solve: 1 = preconditioned form from the start
   int dual_local = 0;     // 1: local derivatives by dual numbers instead of node_vjp
@@ -1602,8 +1608,10 @@ void *abuf(AdjHost *A, int i, size_t bytes, int &rc) {
 }
 
 // the stages of one step from u_n with a given dt: ub, ux, wh, qh and u_fv
-// (u_n plus the FV update, before renormalisation)
-int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt) {
+// (u_n plus the FV update, before renormalisation).  With qh_in, the
+// predictor is not run: qh_in holds its result for this step and the FV
+// update reads it.
+int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt, const double *qh_in = nullptr) {
   int rc;
   int nX[2] = {c->run_nx, c->run_ny};
   double dX[2] = {c->run_dx, c->run_dy};
@@ -1616,17 +1624,19 @@ int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt) {
   if (rc) return rc;
   rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
   if (rc) return rc;
-  PredArgs pa = pred_args(c, c->run_mat, dt, dX);
-  pa.qh = dqh;
-  pa.wh = dwh;
-  pa.mask = dm;
-  pa.ncell = ne;
-  rc = dispatch<PredictorOp>(c, c, pa);
-  if (rc) return rc;
+  if (!qh_in) {
+    PredArgs pa = pred_args(c, c->run_mat, dt, dX);
+    pa.qh = dqh;
+    pa.wh = dwh;
+    pa.mask = dm;
+    pa.ncell = ne;
+    rc = dispatch<PredictorOp>(c, c, pa);
+    if (rc) return rc;
+  }
   HIP_TRY(hipMemcpyAsync(u_fv, u_n, sizeof(double) * nc * c->V, hipMemcpyDeviceToDevice, c->stream));
   FvArgs fa = fv_args(c, c->run_mat, dt, dX, nX);
   fa.u = u_fv;
-  fa.qh = dqh;
+  fa.qh = qh_in ? qh_in : dqh;
   fa.mask = dm;
   fa.post = 0;
   rc = fv_buffers(c, fa);
@@ -1634,7 +1644,9 @@ int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt) {
   return dispatch<FvOp>(c, c, fa);
 }
 
-template <int N> int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *lam, double *d_gmu) {
+template <int N>
+int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *lam, double *d_gmu,
+                  const double *qh_saved) {
   constexpr int V = 14;
   int rc;
   const int nx = c->run_nx, ny = c->run_ny;
@@ -1643,7 +1655,7 @@ template <int N> int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, do
   long nfx = (long)(nx + 1) * ny, nfy = (long)nx * (ny + 1);
   double *u_fv = (double *)abuf(A, 0, sizeof(double) * nc * V, rc);
   if (!u_fv) return rc;
-  rc = adj_stages(c, u_n, u_fv, dt);
+  rc = adj_stages(c, u_n, u_fv, dt, qh_saved);
   if (rc) return rc;
   AdjArgs a;
   std::memset(&a, 0, sizeof a);
@@ -1657,7 +1669,7 @@ template <int N> int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, do
   if (!(a.lam_ux = (double *)abuf(A, 6, sizeof(double) * nux * N * V, rc))) return rc;
   if (!(a.lam_ub = (double *)abuf(A, 7, sizeof(double) * nb * V, rc))) return rc;
   if (!(a.lam0 = (double *)abuf(A, 8, sizeof(double) * nc * V, rc))) return rc;
-  a.qh = (const double *)c->bufs["qh"].first;
+  a.qh = qh_saved ? qh_saved : (const double *)c->bufs["qh"].first;
   a.ux = (const double *)c->bufs["ux"].first;
   a.ub = (const double *)c->bufs["ub"].first;
   a.mask = (const uint8_t *)c->bufs["mask"].first;
@@ -1713,6 +1725,7 @@ int gpr_adj_release(gpr_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   if (A->chk) (void)hipFree(A->chk);
   if (A->seg) (void)hipFree(A->seg);
+  if (A->qhs) (void)hipFree(A->qhs);
   for (auto &b : A->bufs)
     if (b.first) (void)hipFree(b.first);
   delete A;
@@ -1744,14 +1757,19 @@ int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
   StepState h0;
   HIP_TRY(hipMemcpyAsync(&h0, c->d_st, sizeof h0, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  // the steps' dt are recorded on the device as they are taken (no host
+  // synchronisation per step)
+  int rc = GPR_OK;
+  double *d_dts = (double *)abuf(A, 13, sizeof(double) * nsteps, rc);
+  if (!d_dts) return rc;
   for (int64_t k = 0; k < nsteps; k++) {
     if (k % K == 0)
       HIP_TRY(hipMemcpyAsync(A->chk + (k / K) * nc * c->V, c->run_u, sz, hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(&A->dts[k], &c->d_st->dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    int rc = dev_steps_impl(c, 1, &A->dts[k]);
+    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(1), 0, c->stream, c->d_st, d_dts + k);
+    rc = dev_steps_impl(c, 1, nullptr);
     if (rc) return rc;
   }
+  HIP_TRY(hipMemcpyAsync(A->dts.data(), d_dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
   StepState h1;
   HIP_TRY(hipMemcpyAsync(&h1, c->d_st, sizeof h1, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1787,24 +1805,43 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   const int64_t nchk = (A->nsteps + K - 1) / K;
   int nX[2] = {c->run_nx, c->run_ny};
   double dX[2] = {c->run_dx, c->run_dy};
+  // Keep each recomputed step's predictor result for its transposed step
+  // (K of them; 5 GB at 256^2 with K = 25), so the predictor runs once per
+  // step in the reverse sweep.  Without the memory, it runs twice.
+  const size_t qhb = sizeof(double) * (size_t)(c->run_nx + 2) * (c->run_ny + 2) * 27 * c->V;  // N = 3, 2D
+  if (!A->qhs || A->qh_bytes != qhb) {
+    if (A->qhs) (void)hipFree(A->qhs);
+    A->qhs = nullptr;
+    A->qh_bytes = qhb;
+    if (hipMalloc((void **)&A->qhs, qhb * K) != hipSuccess) {
+      (void)hipGetLastError();
+      A->qhs = nullptr;
+    }
+  }
   for (int64_t sgi = nchk - 1; sgi >= 0; sgi--) {
     const int64_t n0 = sgi * K, n1 = std::min<int64_t>(n0 + K, A->nsteps);
     // recompute u_n0 .. u_{n1-1} from the checkpoint
     HIP_TRY(hipMemcpyAsync(work, A->chk + sgi * nc * c->V, sz, hipMemcpyDeviceToDevice, c->stream));
     for (int64_t n = n0; n < n1; n++) {
       HIP_TRY(hipMemcpyAsync(A->seg + (n - n0) * nc * c->V, work, sz, hipMemcpyDeviceToDevice, c->stream));
-      if (n + 1 < n1) {
+      if (n + 1 < n1 || A->qhs) {
         double *u_fv = (double *)abuf(A, 0, sz, rc);
         if (!u_fv) return rc;
         rc = adj_stages(c, work, u_fv, A->dts[n]);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_renorm_only, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, u_fv, nc,
-                           c->run_mat.p.rho0);
-        HIP_TRY(hipMemcpyAsync(work, u_fv, sz, hipMemcpyDeviceToDevice, c->stream));
+        if (A->qhs)
+          HIP_TRY(hipMemcpyAsync((char *)A->qhs + (n - n0) * qhb, c->bufs["qh"].first, qhb, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        if (n + 1 < n1) {
+          hipLaunchKernelGGL(k_renorm_only, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, u_fv, nc,
+                             c->run_mat.p.rho0);
+          HIP_TRY(hipMemcpyAsync(work, u_fv, sz, hipMemcpyDeviceToDevice, c->stream));
+        }
       }
     }
     for (int64_t n = n1 - 1; n >= n0; n--) {
-      rc = adj_step_back<3>(c, A, A->seg + (n - n0) * nc * c->V, A->dts[n], lam, d_gmu);
+      const double *qsave = A->qhs ? (const double *)((char *)A->qhs + (n - n0) * qhb) : nullptr;
+      rc = adj_step_back<3>(c, A, A->seg + (n - n0) * nc * c->V, A->dts[n], lam, d_gmu, qsave);
       if (rc) return rc;
     }
   }
