@@ -21,6 +21,8 @@ alone the gap is 7e-6, 5e-8 and 3e-9 at the same h."""
 import numpy as np
 import pytest
 
+from util import relerr
+
 pytestmark = pytest.mark.gpu
 
 NSTEPS, K = 10, 4  # three checkpoint segments, the last one short
@@ -213,3 +215,45 @@ def test_reverse_mode_local_derivatives_equal_dual_numbers(mk):
     print(f"{mk}: max |lam0 difference| / max |lam0| {err:.2e}, dL/dmu {g0:.12e} vs {g1:.12e}")
     assert err <= 1e-12
     assert abs(g1 - g0) <= 1e-12 * abs(g1)
+
+
+def oracle_steps_dt(O, u, nX, dX, bcs, dts, MP):
+    """The oracle's time step at prescribed dt: boundaries, WENO, predictor,
+    FV update and renormalisation (orc_step with dt given instead of taken
+    from the CFL rule)."""
+    mask = O.extend_mask(np.ones(nX[0] * nX[1], np.uint8), nX)
+    for dt in dts:
+        ub = O.boundaries(u, nX, bcs)
+        wh = O.weno_launcher(ub, nX)
+        qh = O.predictor(wh, dt, dX, nX, MP, mask)
+        u = O.fv_launcher(u, qh, nX, dt, dX, MP, mask)
+        u = O.renorm_distortion(u, MP)
+    return u
+
+
+@pytest.mark.parametrize("which", ["rho", "all"])
+def test_gradient_against_oracle_finite_differences(adj, which):
+    """The FP64 reference gradient: central differences of the oracle's own
+    map (the CPU restatement, predictor solved to 1e-13 as the forward sweep
+    solves it) at the GPU run's dt schedule, against the GPU adjoint.  The
+    oracle's final state first reproduces the GPU's."""
+    from orcbind import Oracle
+    run, u0, MP = adj["run"], adj["u0"], adj["MP"]
+    O = Oracle(3, False, 2, dg_tol=1e-13, stiff_tol=1e-13, stiff_it=400)
+    nX, dX, bcs = [16, 16], [1 / 16, 1 / 16], [1, 1, 1, 1]
+    uT = oracle_steps_dt(O, u0, nX, dX, bcs, run.dts, MP)
+    assert relerr(uT, adj["uT"]) < 1e-11
+    rng = np.random.default_rng(21 if which == "rho" else 22)
+    d = np.zeros_like(u0).reshape(-1, 14)
+    if which == "rho":
+        d[:, 0] = rng.standard_normal(d.shape[0])
+    else:
+        d = rng.standard_normal(d.shape) * np.maximum(np.abs(u0.reshape(-1, 14)), 1e-3)
+    d = d.ravel()
+    h = 1e-8
+    Lp, _ = loss(oracle_steps_dt(O, u0 + h * d, nX, dX, bcs, run.dts, MP), adj["target"])
+    Lm, _ = loss(oracle_steps_dt(O, u0 - h * d, nX, dX, bcs, run.dts, MP), adj["target"])
+    fd = (Lp - Lm) / (2 * h)
+    ad = float(np.dot(adj["lam0"], d))
+    print(f"{which}: adjoint {ad:.10e} oracle fd {fd:.10e}")
+    assert abs(ad - fd) <= 2e-5 * abs(fd)
