@@ -197,14 +197,17 @@ int gpr_batch_run(gpr_ctx *ctx, int steps_per_graph, int64_t max_graphs, int64_t
 int gpr_batch_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t *done, int32_t *nonfinite,
                      int32_t *pred_fail);
 
-/* --- adjoint of the device loop (2D, N = 3, V = 14, Newtonian) --------------
+/* --- adjoint of the device loop (2D, N = 3, V = 14; Newtonian fluids and
+ * elastic or power-law solids) ------------------------------------------------
  * No reference counterpart.  After gpr_dev_begin: gpr_dev_steps_dt advances
  * with prescribed time steps (dts[k] for step k) instead of the CFL rule;
  * gpr_adj_forward advances nsteps steps as gpr_dev_steps does, keeping a
  * checkpoint of u every K steps and the time steps taken (dts_out, may be
  * null); gpr_adj_backward takes lam_T = d loss / d u at the end (host,
- * [cells][V]) and returns d loss / d u at the start (lam_0) and d loss / d mu,
- * with the time steps held at the forward run's values; *unconverged counts
+ * [cells][V]) and returns d loss / d u at the start (lam_0) and the
+ * derivative with respect to the material's relaxation parameter in *grad_mu:
+ * mu for a Newtonian fluid, sigmaY for a power-law solid (0 for an elastic
+ * solid), with the time steps held at the forward run's values; *unconverged counts
  * the cells whose transposed predictor iteration did not converge.
  * gpr_adj_forward fails with GPR_E_UNSUPPORTED if the global distortion reset
  * fires during the sweep (it is not differentiated), and with

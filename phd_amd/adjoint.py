@@ -8,7 +8,9 @@ then held): the gradient is that of the map u_0 -> u_T at that dt schedule,
 and finite differences are taken with the same schedule
 (AdjointRun.replay).  The predictor is differentiated at its solution of the
 DG equations, so the forward run solves them tightly (dg_tol = stiff_tol =
-1e-13 by default here).  2D, N = 3, V = 14, Newtonian viscosity.
+1e-13 by default here).  2D, N = 3, V = 14; the material's relaxation
+parameter is mu for a Newtonian fluid and sigmaY for a power-law solid
+(`param_name`), and "dmu" below is the derivative with respect to it.
 
     from phd_amd.adjoint import AdjointRun, StepsFunction
     run = AdjointRun(u0, MP, nX, dX, bcs)
@@ -76,10 +78,12 @@ class AdjointRun:
                                     f"{unconv.value} cell-steps")
         return lam_0, gmu.value
 
-    def replay(self, u0=None, mu=None):
+    def replay(self, u0=None, mu=None, param=None):
         """The forward map at the recorded time steps, from u0 (default: the
-        initial state) with viscosity mu (default: the run's): u_T."""
-        MP = self.MP if mu is None else _with_mu(self.MP, float(mu))
+        initial state) with the relaxation parameter `param` (mu or sigmaY;
+        `mu` is an alias; default: the run's): u_T."""
+        value = param if param is not None else mu
+        MP = self.MP if value is None else _with_param(self.MP, float(value))
         u = self.u0 if u0 is None else np.ascontiguousarray(u0, np.float64).ravel()
         r = DeviceRun(u, MP, self.nX, self.dX, self.bcs, CFL=self.CFL, **self.opts)
         check(self.L.gpr_dev_steps_dt(r.ctx.h, dptr(self.dts), len(self.dts)))
@@ -92,26 +96,42 @@ class AdjointRun:
         self.run.close()
 
 
-def _with_mu(MP, mu):
-    """A copy of the material with viscosity mu (and the heat conductivity
-    material_params derives from it left as it was)."""
+def param_name(MP):
+    """The relaxation parameter the adjoint differentiates: "mu" for a
+    Newtonian fluid, "sigmaY" for a power-law solid, None for an elastic
+    solid (no parameter)."""
+    if MP.SOLID:
+        return "sigmaY" if MP.POWER_LAW else None
+    return "mu"
+
+
+def _with_param(MP, value):
+    """A copy of the material with its relaxation parameter set to value (a
+    heat conductivity material_params derived from mu is left as it was)."""
     from .params import Par
     P = Par()
     ctypes.memmove(ctypes.byref(P), ctypes.byref(MP), ctypes.sizeof(Par))
-    P.mu = mu
+    name = param_name(MP)
+    if name is None:
+        raise ValueError("an elastic solid has no relaxation parameter")
+    setattr(P, name, value)
     return P
+
+
 
 
 try:
     import torch
 
     class StepsFunction(torch.autograd.Function):
-        """u_T = Steps(u_0, mu) over `spec` (MP, nX, dX, bcs, nsteps, K; CPU
-        tensors in and out, the steps and their adjoint on the GPU)."""
+        """u_T = Steps(u_0, p) over `spec` (MP, nX, dX, bcs, nsteps, K; CPU
+        tensors in and out, the steps and their adjoint on the GPU); p is the
+        material's relaxation parameter (mu, or sigmaY for a power-law
+        solid)."""
 
         @staticmethod
         def forward(ctx, u0, mu, spec):
-            MP = _with_mu(spec["MP"], float(mu))
+            MP = _with_param(spec["MP"], float(mu))
             run = AdjointRun(u0.detach().cpu().numpy(), MP, spec["nX"], spec["dX"], spec["bcs"])
             uT = run.forward(spec["nsteps"], spec["K"])
             ctx.run = run

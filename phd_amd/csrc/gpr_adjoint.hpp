@@ -2,12 +2,13 @@
 // conduction), with the step's dt held fixed (the time steps of the forward
 // run).  With lambda the gradient of a loss with respect to u_{n+1}, the
 // kernels below give its gradient with respect to u_n and accumulate its
-// derivative with respect to the viscosity mu.  Order, the reverse of the
+// derivative with respect to the material's relaxation parameter (mu of a
+// Newtonian fluid, sigmaY of a power-law solid).  Order, the reverse of the
 // step:
 //   ka_renorm   u_{n+1} = renorm(u_fv)                    (utils.cpp:6-23)
 //   ka_faces    u_fv -= face terms of qh                  (fv.cpp:162-268)
 //   ka_cells    u_fv += cell terms of qh; gathers faces  (fv.cpp:56-109)
-//   ka_predictor  qh solves the DG equations R(qh; wh, mu) = 0 (dg2.cpp):
+//   ka_predictor  qh solves the DG equations R(qh; wh, parameter) = 0 (dg2.cpp):
 //               (dR/dq)^T m = lambda_qh by the transposed Picard
 //               iteration, then lambda_wh = -(dR/dwh)^T m
 //   ka_weno_y, ka_weno_x   wh = WENO_y(WENO_x(ub))         (weno.cpp:83-124)
@@ -40,10 +41,11 @@ struct AdjArgs {
   double *lam_ub;       // like ub
   double *lam0;         // [cells][V] d loss / d u_n (output)
   const uint8_t *mask;  // [(nx+2)(ny+2)]
-  double *gmu;          // d loss / d mu, accumulated
+  double *gmu;          // d loss / d (the material parameter), accumulated
   int *fail;            // cells whose transposed Picard iteration did not converge
   unsigned long long *itc;  // [0] transposed predictor iterations, [1] cells, [2] cells in the preconditioned form
-  double dt, dx, dy, mu;
+  double dt, dx, dy;
+  double par_fac;       // d S_A / d parameter = par_fac S_A: -1/mu (Newtonian), -n/sigmaY (power-law solid)
   int nx, ny;
   int it_plain;  // transposed Picard iterations before the preconditioned form
   int start_pre; // 1: every cell starts in the preconditioned form
@@ -53,8 +55,9 @@ struct AdjArgs {
   Mat m;
 };
 
-// Reverse-mode derivative of one node's terms for the fluid the adjoint
-// covers (V = 14, no thermal terms, Newtonian theta1inv, any EOS):
+// Reverse-mode derivative of one node's terms for the materials the adjoint
+// covers (V = 14, no thermal terms, any EOS; theta1inv of a Newtonian fluid
+// or of a solid, elastic or power-law plastic):
 //   out[j] = d/dq_j [ ah . (S(q) - B_x(v) dqx - B_y(v) dqy)     (H)
 //                   + aF . F_x(q)                               (FX)
 //                   + aG . F_y(q) ]                             (FY)
@@ -112,17 +115,46 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
     EAb[k] = 0.0;
     Ab[k] = 0.0;
   }
-  double pb = 0, irb = 0, rb = 0, thb = 0;
+  double pb = 0, irb = 0, rb = 0, thb = 0, cs2b = 0;
   if constexpr (H) {
-    // S_A = -EA theta1inv; theta1inv = det(A)^(5/3) mu_fac
-    const double dA = det3(A), c13 = cbrt(dA);
-    const double th = dA * c13 * c13 * m.mu_fac;
+    // S_A = -EA theta1inv (sources.cpp:13-81):
+    //   Newtonian fluid   theta1inv = det(A)^(5/3) mu_fac
+    //   power-law solid   theta1inv = 3 det(A)^(5/3) / (cs2 tau0) (min(|sigma'|, 1e8) / sigmaY)^n
+    //   elastic solid     theta1inv = 0
+    const double dA = det3(A), c13 = cbrt(dA), A53 = dA * c13 * c13;
+    double th = 0, th_A53 = 0, th_cs2 = 0, th_sn = 0, sn = 0;
+    if (p.SOLID) {
+      if (p.POWER_LAW) {
+        const double a1 = sig[0] - sig[4], b1 = sig[4] - sig[8], c1 = sig[8] - sig[0];
+        sn = sqrt(0.5 * (a1 * a1 + b1 * b1 + c1 * c1) + 3.0 * (sig[1] * sig[1] + sig[5] * sig[5] + sig[6] * sig[6]));
+        const double base = 3.0 / (cs2 * p.tau0), pw = pow(fmin(sn, 1e8) / p.sigmaY, p.n);
+        th = A53 * base * pw;
+        th_A53 = base * pw;
+        th_cs2 = -th / cs2;
+        if (sn < 1e8) th_sn = sn > 0 ? p.n * th / sn : (p.n == 1.0 ? A53 * base / p.sigmaY : 0.0);
+      }
+    } else {
+      th = A53 * m.mu_fac;
+      th_A53 = m.mu_fac;
+    }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       EAb[k] -= ah[5 + k] * th;
       thb -= ah[5 + k] * EA[k];
     }
-    const double f = thb * m.mu_fac * (5.0 / 3.0) * c13 * c13;  // d theta / d det(A)
+    cs2b += thb * th_cs2;
+    if (th_sn != 0.0 && sn > 0) {
+      // |sigma'|^2 = ((s00 - s11)^2 + (s11 - s22)^2 + (s22 - s00)^2) / 2 + 3 (s01^2 + s12^2 + s20^2)
+      const double g = thb * th_sn / (2.0 * sn);
+      const double a1 = sig[0] - sig[4], b1 = sig[4] - sig[8], c1 = sig[8] - sig[0];
+      sigb[0] += g * (a1 - c1);
+      sigb[4] += g * (b1 - a1);
+      sigb[8] += g * (c1 - b1);
+      sigb[1] += g * 6.0 * sig[1];
+      sigb[5] += g * 6.0 * sig[5];
+      sigb[6] += g * 6.0 * sig[6];
+    }
+    const double f = thb * th_A53 * (5.0 / 3.0) * c13 * c13;  // d theta / d det(A)
     Ab[0] += f * (A[4] * A[8] - A[5] * A[7]);
     Ab[1] += f * (A[5] * A[6] - A[3] * A[8]);
     Ab[2] += f * (A[3] * A[7] - A[4] * A[6]);
@@ -198,7 +230,7 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
   for (int i = 0; i < 3; i++) vb[i] -= eb * v[i];
   const double C0b = -0.25 * eb * devGsq, devGsqb = -0.25 * eb * C0;
   Gmb += C0b * p.beta / (Gm * Gm) * cs2;
-  double cs2b = C0b * (1.0 - p.beta / Gm);
+  cs2b += C0b * (1.0 - p.beta / Gm);
   // sig = -rho EA^T A
   double Db[9];
 #pragma unroll
@@ -513,7 +545,7 @@ __device__ __forceinline__ void node_h(const R *q, const double *dqx, const doub
 
 // Cells: one lane per space-time node of every extended cell, as k_predictor's
 // layout.  lam_qh = (cell terms)^T lam_fv for real cells + face extrapolation
-// adjoints; the mu-derivative of the cell terms into gmu.
+// adjoints; the parameter derivative of the cell terms into gmu.
 template <int N, int ND, bool DUAL>
 __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
   constexpr int V = 14;
@@ -599,12 +631,12 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
 #pragma unroll
       for (int v = 0; v < V; v++) gy[v] = -tmp[v] / a.dy;
     }
-    // d h / d mu = -S_A / mu for the Newtonian relaxation (theta1^-1 ~ 1/mu)
+    // d h / d parameter = par_fac S_A (theta1^-1 ~ 1/mu, or ~ sigmaY^-n)
     double S[V];
     source<double, false>(q, st, a.m, S);
     double gm = 0;
 #pragma unroll
-    for (int k = 5; k < 14; k++) gm += wl[k] * (-S[k] / a.mu);
+    for (int k = 5; k < 14; k++) gm += wl[k] * S[k] * a.par_fac;
     atomicAdd(a.gmu, gm);
   }
   __syncthreads();
@@ -680,7 +712,7 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
 // A_s = DG_MAT + dt kappa_s W as stiff_precond builds them at the node's WENO
 // state; Pi_s is an orthogonal projection, so Pi_s^T = Pi_s.  Both forms have
 // the same fixed point.  Then lam_wh = (d rhs / d w)^T m =
-// sum_t psi_t(0) W_s m(t, s), and gmu += (d rhs / d mu)^T m.
+// sum_t psi_t(0) W_s m(t, s), and gmu += (d rhs / d parameter)^T m.
 template <int N, int ND, bool DUAL>
 __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   constexpr int V = 14;
@@ -960,7 +992,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
     source<double, false>(q, st, a.m, S);
     double gm = 0;
 #pragma unroll
-    for (int k = 5; k < 14; k++) gm += dt * Wt * Ws * mu_[k] * (-S[k] / a.mu);
+    for (int k = 5; k < 14; k++) gm += dt * Wt * Ws * mu_[k] * S[k] * a.par_fac;
     atomicAdd(a.gmu, gm);
   }
 }

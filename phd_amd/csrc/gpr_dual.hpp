@@ -57,9 +57,12 @@ GPR_DUAL_CMP(==)
 GPR_DUAL_CMP(!=)
 #undef GPR_DUAL_CMP
 
+// At a zero argument with a zero derivative (sqrt of a sum of squares at
+// its minimum, e.g. the stress norm of a body at rest) the derivative is
+// taken as 0, the value of the one-sided limits' average.
 __device__ __forceinline__ Dual sqrt(const Dual &a) {
   const double s = ::sqrt(a.v);
-  return Dual(s, a.d / (2.0 * s));
+  return Dual(s, (s == 0.0 && a.d == 0.0) ? 0.0 : a.d / (2.0 * s));
 }
 __device__ __forceinline__ Dual cbrt(const Dual &a) {
   const double c = ::cbrt(a.v);

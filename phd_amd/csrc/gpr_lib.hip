@@ -1678,7 +1678,10 @@ int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *
   a.dt = dt;
   a.dx = c->run_dx;
   a.dy = c->run_dy;
-  a.mu = c->run_mat.p.mu;
+  {
+    const gpr_par &P = c->run_mat.p;
+    a.par_fac = P.SOLID ? (P.POWER_LAW ? -P.n / P.sigmaY : 0.0) : -1.0 / P.mu;
+  }
   a.nx = nx;
   a.ny = ny;
   a.wh = (const double *)c->bufs["wh"].first;
@@ -1737,8 +1740,9 @@ int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
   if (!c || !c->run_ready) return fail(GPR_E_ARG, "gpr_dev_begin has not been called");
   if (c->ndim != 2 || c->thermal || c->N != 3) return fail(GPR_E_UNSUPPORTED, "the adjoint is built for 2D, N = 3, V = 14");
   if (nsteps < 1 || K < 1) return fail(GPR_E_ARG, "nsteps and K must be >= 1");
-  if (c->run_mat.p.SOLID || c->run_mat.p.POWER_LAW || c->run_mat.p.BINGHAM)
-    return fail(GPR_E_UNSUPPORTED, "the adjoint's mu derivative is that of the Newtonian relaxation");
+  if (c->run_mat.p.BINGHAM || (c->run_mat.p.POWER_LAW && !c->run_mat.p.SOLID))
+    return fail(GPR_E_UNSUPPORTED,
+                "the adjoint covers Newtonian fluids and elastic or power-law solids, not Bingham or power-law fluids");
   for (int k = 0; k < 4; k++) {
     int code = k == 0 ? c->run_bc.x : k == 1 ? c->run_bc.y : k == 2 ? c->run_bc.z : c->run_bc.w;
     if (code == 4) return fail(GPR_E_UNSUPPORTED, "the adjoint has no moving-wall boundary");

@@ -257,3 +257,65 @@ def test_gradient_against_oracle_finite_differences(adj, which):
     ad = float(np.dot(adj["lam0"], d))
     print(f"{which}: adjoint {ad:.10e} oracle fd {fd:.10e}")
     assert abs(ad - fd) <= 2e-5 * abs(fd)
+
+
+def plastic_case(amp=0.2, n_pow=4):
+    """The elastoplastic pulse on 16^2 (Godunov-Romenski copper, power-law
+    plasticity with sigmaY = 0.4, tau0 = 1) with a velocity pulse of the given
+    amplitude and exponent n: at amplitude 0.2 and n = 4 the forward steps
+    converge at this resolution and 54 cell-steps of ten take the stiff
+    fallback (the n = 10 pulse of amplitude 0.5 does not converge at 16^2)."""
+    from phd_amd import problems
+    from phd_amd.params import material_params
+    u0, _, _, dX, bcs = problems.plastic_pulse_2d(16)
+    MP = material_params("gr", 8.9, cv=4e-4, γ=2.0, c0=3.909, α=1.0, β=3.0, b0=2.1, σY=0.4,
+                         τ0=1.0, n=n_pow)
+    u0 = u0.reshape(16, 16, 14).copy()
+    r = u0[..., 0]
+    v = u0[..., 2:5] / r[..., None]
+    u0[..., 1] -= 0.5 * r * (v ** 2).sum(-1) * (1 - (amp / 0.5) ** 2)
+    u0[..., 2:5] *= amp / 0.5
+    return u0.ravel().copy(), MP, [16, 16], dX, bcs
+
+
+def test_sigmaY_gradient_against_fd():
+    """d L / d sigmaY of a power-law solid (L on the distortion A_00) against
+    central differences at the recorded dt schedule."""
+    from phd_amd.adjoint import AdjointRun, param_name
+    u0, MP, nX, dX, bcs = plastic_case()
+    assert param_name(MP) == "sigmaY"
+    run = AdjointRun(u0, MP, nX, dX, bcs)
+    uT = run.forward(NSTEPS, K)
+    L, r = loss(uT, 1.0, 5)
+    lamT = np.zeros_like(uT).reshape(-1, 14)
+    lamT[:, 5] = 2 * r
+    _, g = run.backward(lamT.ravel())
+    h = 1e-6 * MP.sigmaY
+    fd = (loss(run.replay(param=MP.sigmaY + h), 1.0, 5)[0] -
+          loss(run.replay(param=MP.sigmaY - h), 1.0, 5)[0]) / (2 * h)
+    st = run.stats()
+    run.close()
+    print(f"dL/dsigmaY adjoint {g:.10e} fd {fd:.10e}; transposed iterations per cell-step "
+          f"{st['iters_per_cell_step']:.2f}")
+    assert abs(g - fd) <= 1e-5 * abs(fd)
+
+
+def test_plastic_reverse_mode_equals_dual_numbers():
+    """The power-law branch of theta1inv in the hand-written reverse mode
+    against the dual numbers, on the plastic pulse."""
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, nX, dX, bcs = plastic_case()
+    out = {}
+    for dual in (False, True):
+        run = AdjointRun(u0, MP, nX, dX, bcs)
+        uT = run.forward(NSTEPS, K, dual_local=dual)
+        lamT = np.zeros_like(uT).reshape(-1, 14)
+        lamT[:, 5] = 2 * (uT.reshape(-1, 14)[:, 5] - 1.0)
+        lamT[:, 2] = uT.reshape(-1, 14)[:, 2]
+        out[dual] = run.backward(lamT.ravel())
+        run.close()
+    (l0, g0), (l1, g1) = out[False], out[True]
+    err = np.max(np.abs(l1 - l0)) / np.max(np.abs(l1))
+    print(f"plastic: max |lam0 difference| / max |lam0| {err:.2e}, dL/dsigmaY {g0:.12e} vs {g1:.12e}")
+    assert err <= 1e-12
+    assert abs(g1 - g0) <= 1e-12 * abs(g1)
