@@ -24,6 +24,12 @@ struct BatchArgs {
   double *wh;                   // [sum (nx + 2)][N][V]
   double *qh;                   // [sum (nx + 2)][N^2][V]
   double *fx;                   // [sum (nx + 1)][2][V]
+  // the view: the instances the element-wise launches cover (all, or the
+  // running ones), vinst[j] for j < vcount, with their cumulative cell,
+  // ghost-extended row and extended cell counts (vcount + 1 entries each)
+  const int *vinst;
+  int vcount;
+  const long *vu0s, *vb0s, *ve0s;
 };
 
 // index k with starts[k] <= x < starts[k + 1]
@@ -42,19 +48,20 @@ __device__ __forceinline__ int find_inst(const long *starts, int count, long x) 
 // boundaries1 (grid.cpp:12-145) per instance: one thread per (row of ub, variable)
 template <int N, int V>
 __global__ void kb_boundaries(BatchArgs b) {
-  const long total = b.b0s[b.count] * V;
+  const long total = b.vb0s[b.vcount] * V;
   for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
     const int v = id % V;
-    const long row = id / V;
-    const int k = find_inst(b.b0s, b.count, row);
+    const long vrow = id / V;
+    const int j = find_inst(b.vb0s, b.vcount, vrow);
+    const int k = b.vinst[j];
     const BatchInst &I = b.inst[k];
     if (b.st[k].done) continue;
-    const int Irow = (int)(row - I.b0);
+    const int Irow = (int)(vrow - b.vb0s[j]);
     bool refl;
     const int code = Irow < N ? I.bc0 : I.bc1;
     const int i = bc_src(Irow, I.nx, N, code, refl);
     const double *src = b.u + (I.u0 + i) * V;
-    b.ub[id] = refl ? bc_value(v, code, 0, [&](int m) { return src[m]; }) : src[v];
+    b.ub[(I.b0 + Irow) * V + v] = refl ? bc_value(v, code, 0, [&](int m) { return src[m]; }) : src[v];
   }
 }
 
@@ -62,14 +69,16 @@ __global__ void kb_boundaries(BatchArgs b) {
 template <int N, int V>
 __global__ void kb_weno(BatchArgs b) {
   const Basis &K = basis_dev<N>();
-  const long total = b.e0s[b.count] * V;
+  const long total = b.ve0s[b.vcount] * V;
   for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
     const int v = id % V;
-    const long e = id / V;
-    const int k = find_inst(b.e0s, b.count, e);
+    const long ve = id / V;
+    const int j = find_inst(b.ve0s, b.vcount, ve);
+    const int k = b.vinst[j];
     const BatchInst &I = b.inst[k];
     if (b.st[k].done) continue;
-    const int i = (int)(e - I.e0);
+    const int i = (int)(ve - b.ve0s[j]);
+    const long e = I.e0 + i;
     double d[2 * N - 1], o[N];
 #pragma unroll
     for (int r = 0; r < 2 * N - 1; r++) d[r] = b.ub[(I.b0 + i + r) * V + v];
@@ -220,11 +229,13 @@ __device__ __forceinline__ void batch_max(BatchState *st, int k, bool work, doub
 template <bool TH>
 __global__ void kb_post(BatchArgs b, int cfl_only) {
   constexpr int V = Dims<TH>::V;
-  const long total = b.u0s[b.count];
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const bool in = c < total;
-  const int k = find_inst(b.u0s, b.count, in ? c : total - 1);
+  const long total = b.vu0s[b.vcount];
+  const long vc = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool in = vc < total;
+  const int j = find_inst(b.vu0s, b.vcount, in ? vc : total - 1);
+  const int k = b.vinst[j];
   const BatchInst &I = b.inst[k];
+  const long c = I.u0 + (in ? vc : total - 1) - b.vu0s[j];
   BatchState &S = b.st[k];
   const bool work = in && !S.done;
   double sp = 0.0;
@@ -262,11 +273,13 @@ __global__ void kb_post(BatchArgs b, int cfl_only) {
 template <bool TH>
 __global__ void kb_reset(BatchArgs b) {
   constexpr int V = Dims<TH>::V;
-  const long total = b.u0s[b.count];
-  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const bool in = c < total;
-  const int k = find_inst(b.u0s, b.count, in ? c : total - 1);
+  const long total = b.vu0s[b.vcount];
+  const long vc = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool in = vc < total;
+  const int j = find_inst(b.vu0s, b.vcount, in ? vc : total - 1);
+  const int k = b.vinst[j];
   const BatchInst &I = b.inst[k];
+  const long c = I.u0 + (in ? vc : total - 1) - b.vu0s[j];
   BatchState &S = b.st[k];
   const bool work = in && !S.done && S.contorted;
   double sp = 0.0;

@@ -1267,6 +1267,8 @@ struct BatchHost {
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
   bool ready = false;
+  int view_count = 0;  // instances the launches cover
+  long vnu = 0, vnb = 0, vne = 0;  // their cells, ghost-extended rows, extended cells
 };
 
 namespace {
@@ -1276,8 +1278,8 @@ template <int N, bool TH> struct BatchStep {
     constexpr int V = Dims<TH>::V;
     const BatchArgs &b = B->args;
     const int blk = B->block;
-    hipLaunchKernelGGL((kb_boundaries<N, V>), dim3(grid_for(B->nb * V, blk)), dim3(blk), 0, c->stream, b);
-    hipLaunchKernelGGL((kb_weno<N, V>), dim3(grid_for(B->ne * V, blk)), dim3(blk), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_boundaries<N, V>), dim3(grid_for(B->vnb * V, blk)), dim3(blk), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_weno<N, V>), dim3(grid_for(B->vne * V, blk)), dim3(blk), 0, c->stream, b);
     PredArgs pa = pred_args(c, B->inst[0].m, 0.0, &B->inst[0].dx);
     pa.qh = b.qh;
     pa.wh = b.wh;
@@ -1289,9 +1291,9 @@ template <int N, bool TH> struct BatchStep {
                        pa);
     hipLaunchKernelGGL((kb_faces<N, TH>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
     hipLaunchKernelGGL((kb_cells<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
-    hipLaunchKernelGGL((kb_post<TH>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
+    hipLaunchKernelGGL((kb_post<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
     hipLaunchKernelGGL(kb_advance, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
-    hipLaunchKernelGGL((kb_reset<TH>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, b);
+    hipLaunchKernelGGL((kb_reset<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b);
     hipLaunchKernelGGL(kb_finalize, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
     HIP_TRY(hipGetLastError());
     return GPR_OK;
@@ -1308,9 +1310,9 @@ int batch_step(gpr_ctx *c, BatchHost *B) {
 int batch_first_dt(gpr_ctx *c, BatchHost *B) {
   const int blk = B->block;
   if (c->thermal)
-    hipLaunchKernelGGL((kb_post<true>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, B->args, 1);
+    hipLaunchKernelGGL((kb_post<true>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, B->args, 1);
   else
-    hipLaunchKernelGGL((kb_post<false>), dim3((B->nu + blk - 1) / blk), dim3(blk), 0, c->stream, B->args, 1);
+    hipLaunchKernelGGL((kb_post<false>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, B->args, 1);
   hipLaunchKernelGGL(kb_finalize, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, B->args);
   HIP_TRY(hipGetLastError());
   return GPR_OK;
@@ -1320,6 +1322,73 @@ void batch_drop_graph(BatchHost *B) {
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   B->exec = nullptr;
   B->graph_steps = 0;
+}
+
+// The block -> instance maps of the predictor, face and cell launches, over
+// the instances with on[k] set: each such instance gets its blocks (and its
+// blk_p / blk_c / blk_f), the others none, so a launch covers running
+// instances only.  A map that would be empty gets one block of instance 0,
+// whose work the kernels skip once it is done.  Waits for the stream.
+int batch_build_view(gpr_ctx *c, BatchHost *B, const std::vector<char> &on) {
+  const int N = c->N, NN = N * N;
+  const int CPW = WAVE / NN, FPW = WAVE / N;
+  std::vector<int> pblk, cblk, fblk;
+  int n_on = 0;
+  for (int k = 0; k < B->count; k++) {
+    if (!on[k]) continue;
+    n_on++;
+    BatchInst &I = B->inst[k];
+    I.blk_p = (int)pblk.size();
+    for (int b = 0; b < (I.nx + 2 + CPW - 1) / CPW; b++) pblk.push_back(k);
+    I.blk_c = (int)cblk.size();
+    for (int b = 0; b < (I.nx + CPW - 1) / CPW; b++) cblk.push_back(k);
+    I.blk_f = (int)fblk.size();
+    for (int b = 0; b < (I.nx + 1 + FPW - 1) / FPW; b++) fblk.push_back(k);
+  }
+  if (pblk.empty()) {
+    B->inst[0].blk_p = B->inst[0].blk_c = B->inst[0].blk_f = 0;
+    pblk.push_back(0);
+    cblk.push_back(0);
+    fblk.push_back(0);
+  }
+  // the element-wise launches' index space: the covered instances' cells,
+  // rows and extended cells back to back (instance 0 alone if none)
+  std::vector<int> vinst;
+  for (int k = 0; k < B->count; k++)
+    if (on[k]) vinst.push_back(k);
+  if (vinst.empty()) vinst.push_back(0);
+  const int nv = (int)vinst.size();
+  std::vector<long> voff(3 * (B->count + 1), 0);
+  long *vu = voff.data(), *vb = vu + (B->count + 1), *ve = vb + (B->count + 1);
+  for (int j = 0; j < nv; j++) {
+    const BatchInst &I = B->inst[vinst[j]];
+    vu[j + 1] = vu[j] + I.nx;
+    vb[j + 1] = vb[j] + I.nx + 2 * N;
+    ve[j + 1] = ve[j] + I.nx + 2;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[11], vinst.data(), sizeof(int) * nv, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[12], voff.data(), sizeof(long) * voff.size(), hipMemcpyHostToDevice, c->stream));
+  B->args.vinst = (const int *)B->allocs[11];
+  B->args.vcount = nv;
+  B->args.vu0s = (const long *)B->allocs[12];
+  B->args.vb0s = B->args.vu0s + (B->count + 1);
+  B->args.ve0s = B->args.vb0s + (B->count + 1);
+  B->vnu = vu[nv];
+  B->vnb = vb[nv];
+  B->vne = ve[nv];
+  HIP_TRY(hipMemcpyAsync(B->allocs[0], B->inst.data(), sizeof(BatchInst) * B->count, hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[3], pblk.data(), sizeof(int) * pblk.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[4], cblk.data(), sizeof(int) * cblk.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(B->allocs[5], fblk.data(), sizeof(int) * fblk.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  batch_drop_graph(B);  // launch sizes and the view count are in the captured launches
+  B->nblk_p = (int)pblk.size();
+  B->nblk_c = (int)cblk.size();
+  B->nblk_f = (int)fblk.size();
+  B->view_count = n_on;
+  return GPR_OK;
 }
 
 }  // namespace
@@ -1397,8 +1466,10 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   B->nblk_p = (int)pblk.size();
   B->nblk_c = (int)cblk.size();
   B->nblk_f = (int)fblk.size();
-  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 ub, 8 wh, 9 qh, 10 fx
-  const size_t sizes[11] = {sizeof(BatchInst) * count,
+  B->view_count = count;
+  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 ub, 8 wh, 9 qh,
+  // 10 fx, 11 view instances, 12 view offsets (3 x (count + 1))
+  const size_t sizes[13] = {sizeof(BatchInst) * count,
                             sizeof(BatchState) * count,
                             sizeof(long) * 3 * (count + 1),
                             sizeof(int) * pblk.size(),
@@ -1408,10 +1479,12 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
                             sizeof(double) * B->nb * V,
                             sizeof(double) * B->ne * N * V,
                             sizeof(double) * B->ne * NN * V,
-                            sizeof(double) * B->nf * 2 * V};
-  B->allocs.assign(11, nullptr);
+                            sizeof(double) * B->nf * 2 * V,
+                            sizeof(int) * count,
+                            sizeof(long) * 3 * (count + 1)};
+  B->allocs.assign(13, nullptr);
   c->batch = B;
-  for (int i = 0; i < 11; i++) HIP_TRY(hipMalloc(&B->allocs[i], std::max<size_t>(sizes[i], 256)));
+  for (int i = 0; i < 13; i++) HIP_TRY(hipMalloc(&B->allocs[i], std::max<size_t>(sizes[i], 256)));
   std::vector<long> offs;
   offs.insert(offs.end(), B->u0s.begin(), B->u0s.end());
   offs.insert(offs.end(), B->b0s.begin(), B->b0s.end());
@@ -1445,7 +1518,7 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   a.wh = (double *)B->allocs[8];
   a.qh = (double *)B->allocs[9];
   a.fx = (double *)B->allocs[10];
-  return GPR_OK;
+  return batch_build_view(c, B, std::vector<char>(count, 1));
 }
 
 int gpr_batch_set_state(gpr_ctx *c, const double *u) {
@@ -1465,6 +1538,10 @@ int gpr_batch_set_state(gpr_ctx *c, const double *u) {
   }
   HIP_TRY(hipMemcpyAsync(B->args.st, st.data(), sizeof(BatchState) * B->count, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
+  if (B->view_count != B->count) {
+    int rv = batch_build_view(c, B, std::vector<char>(B->count, 1));
+    if (rv) return rv;
+  }
   int rc = batch_first_dt(c, B);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1522,32 +1599,40 @@ int gpr_batch_run(gpr_ctx *c, int steps_per_graph, int64_t max_graphs, int64_t *
   if (steps_per_graph < 1 || max_graphs < 1) return fail(GPR_E_ARG, "steps_per_graph and max_graphs must be >= 1");
   BatchHost *B = c->batch;
   if (B->exec && B->graph_steps != steps_per_graph) batch_drop_graph(B);
-  if (!B->exec) {
-    hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = GPR_OK;
-    for (int k = 0; k < steps_per_graph && rc == GPR_OK; k++) rc = batch_step(c, B);
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    B->graph_steps = steps_per_graph;
-  }
   std::vector<BatchState> st(B->count);
   int64_t n = 0;
   int run = B->count;
   while (n < max_graphs && run > 0) {
+    if (!B->exec) {
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = GPR_OK;
+      for (int k = 0; k < steps_per_graph && rc == GPR_OK; k++) rc = batch_step(c, B);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      B->graph_steps = steps_per_graph;
+    }
     HIP_TRY(hipGraphLaunch(B->exec, c->stream));
     n++;
     HIP_TRY(hipMemcpyAsync(st.data(), B->args.st, sizeof(BatchState) * B->count, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     run = 0;
     for (auto &s : st) run += s.done ? 0 : 1;
+    // Once half of the covered instances are done, the block-mapped launches
+    // are rebuilt over the running ones (and the graph captured again).
+    if (run > 0 && run <= B->view_count / 2) {
+      std::vector<char> on(B->count);
+      for (int k = 0; k < B->count; k++) on[k] = st[k].done ? 0 : 1;
+      int rv = batch_build_view(c, B, on);
+      if (rv) return rv;
+    }
   }
   if (graphs_run) *graphs_run = n;
   if (running) *running = run;
