@@ -38,8 +38,8 @@ METRIC = "ADER-WENO cell-updates/s, 2D 512² P2: plain FP64 vs batched/FP32/adjo
 # fused multiply-add = 2) of one DG-predictor iteration at one space-time
 # node, 2D, V=14, Newtonian ideal gas, counted from the algorithm's formulas
 # (dg2.cpp:8-71 right-hand side, the DG_U2 solve in its factorised form,
-# the stopping test dg2.cpp:280); the table and its derivation are in
-# DESIGN.md.
+# the stopping test dg2.cpp:280).  Each entry is the count of the named part
+# as the formulas state it, not of the compiled instructions.
 FLOPS_NODE = {
     "state (pressure, A^T A, dev G, dE/dA, stress)": 160,
     "source (theta1^-1, S_A)": 30,
