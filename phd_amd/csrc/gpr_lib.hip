@@ -1892,8 +1892,6 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   HIP_TRY(hipMemsetAsync(d_itc, 0, 3 * sizeof(unsigned long long), c->stream));
   const int K = A->K;
   const int64_t nchk = (A->nsteps + K - 1) / K;
-  int nX[2] = {c->run_nx, c->run_ny};
-  double dX[2] = {c->run_dx, c->run_dy};
   // Keep each recomputed step's predictor result for its transposed step
   // (K of them; 5 GB at 256^2 with K = 25), so the predictor runs once per
   // step in the reverse sweep.  Without the memory, it runs twice.
@@ -1934,8 +1932,6 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
       if (rc) return rc;
     }
   }
-  (void)nX;
-  (void)dX;
   int nf = 0;
   HIP_TRY(hipMemcpyAsync(lam_0, lam, sz, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(double), hipMemcpyDeviceToHost, c->stream));
