@@ -197,8 +197,8 @@ int gpr_batch_run(gpr_ctx *ctx, int steps_per_graph, int64_t max_graphs, int64_t
 int gpr_batch_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t *done, int32_t *nonfinite,
                      int32_t *pred_fail);
 
-/* --- adjoint of the device loop (2D, N = 3, V = 14; Newtonian fluids and
- * elastic or power-law solids) ------------------------------------------------
+/* --- adjoint of the device loop (2D, N = 3, V = 14 or 17; Newtonian fluids
+ * and elastic or power-law solids) ---------------------------------------------
  * No reference counterpart.  After gpr_dev_begin: gpr_dev_steps_dt advances
  * with prescribed time steps (dts[k] for step k) instead of the CFL rule;
  * gpr_adj_forward advances nsteps steps as gpr_dev_steps does, keeping a
@@ -228,6 +228,9 @@ int gpr_adj_release(gpr_ctx *ctx);
  * input, instead of the hand-written reverse mode (default 0; same values to
  * round-off, several times slower). */
 int gpr_adj_set_solver(gpr_ctx *ctx, int start_preconditioned, double tol, int dual_local);
+/* d loss / d kappa (the heat conductivity) of the last gpr_adj_backward; 0 for
+ * a context without heat conduction */
+int gpr_adj_grad_kappa(gpr_ctx *ctx, double *out);
 /* out[3]: of the last gpr_adj_backward, transposed predictor iterations summed
  * over cells and steps, cell-steps, and cell-steps that used the
  * preconditioned form */

@@ -32,7 +32,7 @@ SYMBOLS = [
     "gpr_dev_timing", "gpr_batch_create", "gpr_batch_destroy", "gpr_batch_set_state",
     "gpr_batch_get_state", "gpr_batch_set_block", "gpr_batch_steps", "gpr_batch_run",
     "gpr_batch_status", "gpr_dev_steps_dt", "gpr_adj_forward", "gpr_adj_backward",
-    "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats",
+    "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats", "gpr_adj_grad_kappa",
 ]
 
 
@@ -108,6 +108,7 @@ ARGTYPES = {
     "gpr_adj_release": [_P],
     "gpr_adj_set_solver": [_P, _i, _d, _i],
     "gpr_adj_stats": [_P, _Lp],
+    "gpr_adj_grad_kappa": [_P, _Dp],
 }
 
 _lib = None

@@ -8,9 +8,10 @@ then held): the gradient is that of the map u_0 -> u_T at that dt schedule,
 and finite differences are taken with the same schedule
 (AdjointRun.replay).  The predictor is differentiated at its solution of the
 DG equations, so the forward run solves them tightly (dg_tol = stiff_tol =
-1e-13 by default here).  2D, N = 3, V = 14; the material's relaxation
-parameter is mu for a Newtonian fluid and sigmaY for a power-law solid
-(`param_name`), and "dmu" below is the derivative with respect to it.
+1e-13 by default here).  2D, N = 3, V = 14 or 17 (heat conduction); the
+material's relaxation parameter is mu for a Newtonian fluid and sigmaY for a
+power-law solid (`param_name`), and "dmu" below is the derivative with
+respect to it; `grad_kappa` gives the heat conductivity's.
 
     from phd_amd.adjoint import AdjointRun, StepsFunction
     run = AdjointRun(u0, MP, nX, dX, bcs)
@@ -56,6 +57,13 @@ class AdjointRun:
         check(self.L.gpr_adj_set_solver(self.run.ctx.h, sp, float(tol), int(bool(dual_local))))
         return self.run.host()
 
+    def grad_kappa(self):
+        """d loss / d kappa of the last backward (heat-conducting materials;
+        0 otherwise)."""
+        out = ctypes.c_double()
+        check(self.L.gpr_adj_grad_kappa(self.run.ctx.h, ctypes.byref(out)))
+        return out.value
+
     def stats(self):
         """The last reverse sweep's transposed predictor iterations per
         cell-step and the fraction of cell-steps in the preconditioned form."""
@@ -78,12 +86,15 @@ class AdjointRun:
                                     f"{unconv.value} cell-steps")
         return lam_0, gmu.value
 
-    def replay(self, u0=None, mu=None, param=None):
+    def replay(self, u0=None, mu=None, param=None, kappa=None):
         """The forward map at the recorded time steps, from u0 (default: the
         initial state) with the relaxation parameter `param` (mu or sigmaY;
-        `mu` is an alias; default: the run's): u_T."""
+        `mu` is an alias) and heat conductivity `kappa` (defaults: the
+        run's): u_T."""
         value = param if param is not None else mu
         MP = self.MP if value is None else _with_param(self.MP, float(value))
+        if kappa is not None:
+            MP = _with_kappa(MP, float(kappa))
         u = self.u0 if u0 is None else np.ascontiguousarray(u0, np.float64).ravel()
         r = DeviceRun(u, MP, self.nX, self.dX, self.bcs, CFL=self.CFL, **self.opts)
         check(self.L.gpr_dev_steps_dt(r.ctx.h, dptr(self.dts), len(self.dts)))
@@ -103,6 +114,15 @@ def param_name(MP):
     if MP.SOLID:
         return "sigmaY" if MP.POWER_LAW else None
     return "mu"
+
+
+def _with_kappa(MP, kappa):
+    """A copy of the material with heat conductivity kappa."""
+    from .params import Par
+    P = Par()
+    ctypes.memmove(ctypes.byref(P), ctypes.byref(MP), ctypes.sizeof(Par))
+    P.kappa = kappa
+    return P
 
 
 def _with_param(MP, value):

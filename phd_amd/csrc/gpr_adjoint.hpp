@@ -42,10 +42,12 @@ struct AdjArgs {
   double *lam0;         // [cells][V] d loss / d u_n (output)
   const uint8_t *mask;  // [(nx+2)(ny+2)]
   double *gmu;          // d loss / d (the material parameter), accumulated
+  double *gkap;         // d loss / d kappa (heat conduction), accumulated
   int *fail;            // cells whose transposed Picard iteration did not converge
   unsigned long long *itc;  // [0] transposed predictor iterations, [1] cells, [2] cells in the preconditioned form
   double dt, dx, dy;
   double par_fac;       // d S_A / d parameter = par_fac S_A: -1/mu (Newtonian), -n/sigmaY (power-law solid)
+  double kappa;         // heat conductivity: d S_J / d kappa = -S_J / kappa
   int nx, ny;
   int it_plain;  // transposed Picard iterations before the preconditioned form
   int start_pre; // 1: every cell starts in the preconditioned form
@@ -67,9 +69,10 @@ struct AdjArgs {
 // from one dual-number evaluation.  The stress and dev G are symmetric
 // functions of q, so their full 3x3 formulas are differentiated (the forward
 // code fills one triangle and mirrors it, which is the same function of q).
-template <bool H, bool FX, bool FY>
+template <bool TH, bool H, bool FX, bool FY>
 __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, const double *dqy, const Mat &m,
                                          const double *ah, const double *aF, const double *aG, double *out) {
+  constexpr int V = Dims<TH>::V;
   const gpr_par &p = m.p;
   // ---- forward
   const double r = q[0], ir = 1.0 / r;
@@ -103,12 +106,25 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
     for (int j = 0; j < 3; j++)
       sig[i * 3 + j] = -r * (EA[0 * 3 + i] * A[0 * 3 + j] + EA[1 * 3 + i] * A[1 * 3 + j] + EA[2 * 3 + i] * A[2 * 3 + j]);
   const double C0 = (1.0 - p.beta / Gm) * cs2;
-  const double e = q[1] * ir - 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - 0.25 * C0 * devGsq;
-  const double pr = (e - erd.v) * r * Gm + prd.v;
-  // ---- reverse
-  double qb[14], vb[3] = {0, 0, 0}, sigb[9], EAb[9], Ab[9];
+  double Jv[3] = {0.0, 0.0, 0.0};  // J = (rho J) / rho
+  if constexpr (TH)
 #pragma unroll
-  for (int k = 0; k < 14; k++) qb[k] = 0.0;
+    for (int i = 0; i < 3; i++) Jv[i] = q[14 + i] * ir;
+  const double e = q[1] * ir - 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - 0.25 * C0 * devGsq -
+                   (TH ? 0.5 * p.calpha2 * (Jv[0] * Jv[0] + Jv[1] * Jv[1] + Jv[2] * Jv[2]) : 0.0);
+  const double pr = (e - erd.v) * r * Gm + prd.v;
+  // temperature T(rho, p) (mg.cpp) and its partial derivatives
+  double Tt = 0, dTdr = 0, dTdp = 0, Tb = 0;
+  if constexpr (TH) {
+    const Dual Td = temperature_mg(rd, Dual(pr, 0.0), p);
+    Tt = Td.v;
+    dTdr = Td.d;
+    dTdp = 1.0 / (r * Gm * p.cv);
+  }
+  // ---- reverse
+  double qb[V], vb[3] = {0, 0, 0}, sigb[9], EAb[9], Ab[9];
+#pragma unroll
+  for (int k = 0; k < V; k++) qb[k] = 0.0;
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     sigb[k] = 0.0;
@@ -183,6 +199,16 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
       }
     }
   }
+  if constexpr (H && TH) {
+    // S_J = -rho (c_alpha^2 J) T / (kappa rho) = -(c_alpha^2 / kappa) (rho J) T / rho
+    const double c = -p.calpha2 / p.kappa;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      qb[14 + i] += ah[14 + i] * c * Tt * ir;
+      Tb += ah[14 + i] * c * q[14 + i] * ir;
+      irb += ah[14 + i] * c * q[14 + i] * Tt;
+    }
+  }
   // fluxes (flux in gpr_device.hpp)
 #pragma unroll
   for (int d = 0; d < 2; d++) {
@@ -218,6 +244,23 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
         vb[l] += w * A[3 * k + l];
       }
     }
+    if constexpr (TH) {
+      // F1 += c_alpha^2 T (rho J_d) / rho; F_{14+i} = v_d (rho J_i); F_{14+d} += T
+      Tb += a[1] * p.calpha2 * q[14 + d] * ir;
+      qb[14 + d] += a[1] * p.calpha2 * Tt * ir;
+      irb += a[1] * p.calpha2 * Tt * q[14 + d];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        vb[d] += a[14 + i] * q[14 + i];
+        qb[14 + i] += a[14 + i] * vd;
+      }
+      Tb += a[14 + d];
+    }
+  }
+  // T = T(rho, p)
+  if constexpr (TH) {
+    pb += Tb * dTdp;
+    rb += Tb * dTdr;
   }
   // p = (e - e_ref) rho Gamma + p_ref
   const double eb = pb * r * Gm;
@@ -228,6 +271,12 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
   irb += eb * q[1];
 #pragma unroll
   for (int i = 0; i < 3; i++) vb[i] -= eb * v[i];
+  if constexpr (TH)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      qb[14 + i] -= eb * p.calpha2 * Jv[i] * ir;
+      irb -= eb * p.calpha2 * Jv[i] * q[14 + i];
+    }
   const double C0b = -0.25 * eb * devGsq, devGsqb = -0.25 * eb * C0;
   Gmb += C0b * p.beta / (Gm * Gm) * cs2;
   cs2b += C0b * (1.0 - p.beta / Gm);
@@ -289,7 +338,7 @@ __device__ __forceinline__ void node_vjp(const double *q, const double *dqx, con
 #pragma unroll
   for (int k = 0; k < 9; k++) qb[5 + k] += Ab[k];
 #pragma unroll
-  for (int k = 0; k < 14; k++) out[k] = qb[k];
+  for (int k = 0; k < V; k++) out[k] = qb[k];
 }
 
 // Renormalisation of A (k_post / utils.cpp:6-23) on a dual state
@@ -302,8 +351,9 @@ template <typename R> __device__ __forceinline__ void renorm_state(R *q, double 
 }
 
 // one thread per cell: lam_fv = (d renorm / d u_fv)^T lam1
+template <bool TH>
 __global__ void ka_renorm(AdjArgs a, long ncell) {
-  constexpr int V = 14;
+  constexpr int V = Dims<TH>::V;
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
     const double *u = a.u_fv + c * V;
     const double *l = a.lam1 + c * V;
@@ -327,8 +377,9 @@ __global__ void ka_renorm(AdjArgs a, long ncell) {
 }
 
 // (B_d(v))^T w on the distortion rows: out[j] = w . (B_d e_j), j = 5..13
+template <int V = 14>
 __device__ __forceinline__ void bdot_T(const double *vel, const double *w, int d, double *out) {
-  for (int j = 0; j < 14; j++) out[j] = 0.0;
+  for (int j = 0; j < V; j++) out[j] = 0.0;
 #pragma unroll
   for (int j = 5; j < 14; j++) {
     double e[14], o[14];
@@ -337,11 +388,82 @@ __device__ __forceinline__ void bdot_T(const double *vel, const double *w, int d
       e[v] = v == j ? 1.0 : 0.0;
       o[v] = 0.0;
     }
-    bdot_sub(vel, e, d, -1.0, o);  // o = B_d e_j
+    bdot_sub(vel, e, d, -1.0, o);  // o = B_d e_j (distortion rows only)
     double s = 0;
 #pragma unroll
-    for (int v = 0; v < 14; v++) s += w[v] * o[v];
+    for (int v = 5; v < 14; v++) s += w[v] * o[v];
     out[j] = s;
+  }
+}
+
+// adj(M) of a complex 4x4 matrix (row-major):
+// adj(M)_rc = (-1)^(r+c) det(M without row c, column r)
+__device__ __forceinline__ cplx cdet3(const cplx *a) {
+  const cplx m0 = csub(cmul(a[4], a[8]), cmul(a[5], a[7]));
+  const cplx m1 = csub(cmul(a[3], a[8]), cmul(a[5], a[6]));
+  const cplx m2 = csub(cmul(a[3], a[7]), cmul(a[4], a[6]));
+  return cadd(csub(cmul(a[0], m0), cmul(a[1], m1)), cmul(a[2], m2));
+}
+__device__ __forceinline__ void adjugate4(const cplx *M, cplx *adj) {
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      cplx sub[9];
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (i == c) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (j == r) continue;
+          sub[k++] = M[i * 4 + j];
+        }
+      }
+      const cplx d = cdet3(sub);
+      adj[r * 4 + c] = ((r + c) % 2) ? cplx{-d.re, -d.im} : d;
+    }
+}
+
+// Gradient of max_abs_eigs<true> (the heat-conducting 4x4 acoustic tensor X,
+// whose eigenvalue of largest modulus max_abs_eig4 finds by QR iteration).
+// That eigenvalue lam (real, or one of a complex pair: the y tensor has such
+// pairs) is a simple root of det(lam I - X), so by Jacobi's formula
+//   d lam = sum_ik adj(lam I - X)_ki dX_ik / tr adj(lam I - X)
+// in complex arithmetic, and d|lam| = Re(conj(lam) d lam) / |lam|, with dX
+// from dual numbers through thermo_acoustic_tensor.  The speed is
+// sqrt(|lam|) plus |v_D| (eig.cpp:66-77).
+template <int D>
+__device__ __forceinline__ void speed_grad_th(const double *q, const Mat &m, double *out) {
+  constexpr int V = 17;
+  double X[16];
+  thermo_acoustic_tensor<true, double>(q, m, D, X);
+  cplx lam;
+  const double alam = max_abs_eig4(X, &lam);
+  cplx M[16], adj[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) M[k] = {(k % 5 == 0 ? lam.re : 0.0) - X[k], k % 5 == 0 ? lam.im : 0.0};
+  adjugate4(M, adj);
+  const cplx tr = cadd(cadd(adj[0], adj[5]), cadd(adj[10], adj[15]));
+  const double tr2 = cabs2(tr);
+  const double sq = sqrt(alam), vd = q[2 + D] / q[0];
+  for (int j = 0; j < V; j++) {
+    Dual x[V], Xd[16];
+#pragma unroll
+    for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
+    thermo_acoustic_tensor<true, Dual>(x, m, D, Xd);
+    cplx num = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        num.re += adj[k * 4 + i].re * Xd[i * 4 + k].d;
+        num.im += adj[k * 4 + i].im * Xd[i * 4 + k].d;
+      }
+    const cplx dl = cdivr(cmul(num, cconj(tr)), tr2);           // d lam
+    const double dabs = (lam.re * dl.re + lam.im * dl.im) / alam;  // d |lam|
+    const double dv = (j == 2 + D ? 1.0 / q[0] : 0.0) - (j == 0 ? vd / q[0] : 0.0);
+    out[j] = dabs / (2.0 * sq) + (vd > 0 ? dv : -dv);
   }
 }
 
@@ -352,11 +474,11 @@ __device__ __forceinline__ void bdot_T(const double *vel, const double *w, int d
 // wave speed, and smax only through the side the maximum takes (a tie takes
 // the left one, as rusanov does), so each side's flux is differentiated with
 // its own V seeds and one wave speed with V seeds.
-template <int N, int D, bool DUAL>
+template <int N, int D, bool DUAL, bool TH>
 __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, const double *ls, const double *ld,
                                           const Mat &m, const Basis &K, double *out) {
-  constexpr int V = 14;
-  const double sL = max_abs_eigs<false, double>(qL, m, D), sR = max_abs_eigs<false, double>(qR, m, D);
+  constexpr int V = Dims<TH>::V;
+  const double sL = max_abs_eigs<TH, double>(qL, m, D), sR = max_abs_eigs<TH, double>(qR, m, D);
   const bool Lmax = !(sL < sR);
   const double smax = Lmax ? sL : sR;
   double dd = 0;  // ld . (qL - qR), the weight of d smax
@@ -366,7 +488,11 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
     const double *qs = side == 0 ? qL : qR;
     const bool with_speed = side == 0 ? Lmax : !Lmax;
     double fl[V];  // ld . dF(qs)/dq_j
-    if constexpr (!DUAL) node_vjp<false, D == 0, D == 1>(qs, nullptr, nullptr, m, nullptr, ld, ld, fl);
+    if constexpr (!DUAL) node_vjp<TH, false, D == 0, D == 1>(qs, nullptr, nullptr, m, nullptr, ld, ld, fl);
+    double sg[V];  // the wave speed's gradient, thermal case
+    if constexpr (TH) {
+      if (with_speed) speed_grad_th<D>(qs, m, sg);
+    }
     for (int j = 0; j < V; j++) {
       Dual x[V];
 #pragma unroll
@@ -374,15 +500,20 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
       double g = side == 0 ? smax * ld[j] : -smax * ld[j];
       if constexpr (DUAL) {
         Dual F[V];
-        NodeState<Dual, false> st;
-        node_state<Dual, false>(x, m, st);
-        flux<Dual, false>(x, st, m, D, F);
+        NodeState<Dual, TH> st;
+        node_state<Dual, TH>(x, m, st);
+        flux<Dual, TH>(x, st, m, D, F);
 #pragma unroll
         for (int v = 0; v < V; v++) g += ld[v] * F[v].d;
       } else {
         g += fl[j];
       }
-      if (with_speed) g += dd * max_abs_eigs<false, Dual>(x, m, D).d;
+      if (with_speed) {
+        if constexpr (TH)
+          g += dd * sg[j];
+        else
+          g += dd * max_abs_eigs<false, Dual>(x, m, D).d;
+      }
       out[side * V + j] = g;
     }
   }
@@ -394,7 +525,7 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
         xL[v] = Dual(qL[v], j == v ? 1.0 : 0.0);
         xR[v] = Dual(qR[v], j == V + v ? 1.0 : 0.0);
       }
-      bint<N, false, Dual>(xL, xR, D, K, b);
+      bint<N, TH, Dual>(xL, xR, D, K, b);
       double g = 0;
 #pragma unroll
       for (int v = 0; v < V; v++) g += ls[v] * b[v].d;
@@ -428,7 +559,7 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
 #pragma unroll
       for (int c = 0; c < 3; c++) vv[c] = (qL[2 + c] + xi * (qR[2 + c] - qL[2 + c])) / r;
       double gdq[V];  // w_i B_D(v_i)^T ls
-      bdot_T(vv, ls, D, gdq);
+      bdot_T<V>(vv, ls, D, gdq);
       double rbar = 0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -451,9 +582,9 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
 // Faces: one lane per (face, time node, face node), as k_fv_faces.  For the
 // face's lo = k (b + f) (taken from its left cell) and hi = k (b - f) (from its
 // right cell), the adjoints of qL and qR.
-template <int N, int ND, bool DUAL>
+template <int N, int ND, bool DUAL, bool TH>
 __global__ __launch_bounds__(64) void ka_faces(AdjArgs a) {
-  constexpr int V = 14;
+  constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
   constexpr int NN = NS * N;
   constexpr int NSF = ND == 2 ? N : 1;
@@ -518,26 +649,27 @@ __global__ __launch_bounds__(64) void ka_faces(AdjArgs a) {
   }
   double g[2 * V];
   if (d == 0)
-    riemann_T<N, 0, DUAL>(qL, qR, ls, ld, a.m, K, g);
+    riemann_T<N, 0, DUAL, TH>(qL, qR, ls, ld, a.m, K, g);
   else
-    riemann_T<N, 1, DUAL>(qL, qR, ls, ld, a.m, K, g);
+    riemann_T<N, 1, DUAL, TH>(qL, qR, ls, ld, a.m, K, g);
   for (int j = 0; j < 2 * V; j++) out[j] = g[j];
 }
 
 // The cell terms of one node, h = S(q) - B_x(q) dqx - B_y(q) dqy (dq fixed),
 // as k_fv_cells and the predictor's right-hand side form them.
-template <typename R, int ND>
+template <typename R, int ND, bool TH>
 __device__ __forceinline__ void node_h(const R *q, const double *dqx, const double *dqy, const Mat &m, R *h) {
-  NodeState<R, false> st;
-  node_state<R, false>(q, m, st);
-  source<R, false>(q, st, m, h);
-  R dq[14];
+  constexpr int V = Dims<TH>::V;
+  NodeState<R, TH> st;
+  node_state<R, TH>(q, m, st);
+  source<R, TH>(q, st, m, h);
+  R dq[V];
 #pragma unroll
-  for (int v = 0; v < 14; v++) dq[v] = R(dqx[v]);
+  for (int v = 0; v < V; v++) dq[v] = R(dqx[v]);
   bdot_sub(st.v, dq, 0, R(1.0), h);
   if (ND == 2) {
 #pragma unroll
-    for (int v = 0; v < 14; v++) dq[v] = R(dqy[v]);
+    for (int v = 0; v < V; v++) dq[v] = R(dqy[v]);
     bdot_sub(st.v, dq, 1, R(1.0), h);
   }
 }
@@ -546,9 +678,9 @@ __device__ __forceinline__ void node_h(const R *q, const double *dqx, const doub
 // Cells: one lane per space-time node of every extended cell, as k_predictor's
 // layout.  lam_qh = (cell terms)^T lam_fv for real cells + face extrapolation
 // adjoints; the parameter derivative of the cell terms into gmu.
-template <int N, int ND, bool DUAL>
+template <int N, int ND, bool DUAL, bool TH>
 __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
-  constexpr int V = 14;
+  constexpr int V = Dims<TH>::V;
   constexpr int NS = PredShape<N, ND>::NS;
   constexpr int NN = PredShape<N, ND>::NN;
   constexpr int CPW = PredShape<N, ND>::CPW;
@@ -610,34 +742,40 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
         Dual x[V], h[V];
 #pragma unroll
         for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
-        node_h<Dual, ND>(x, dqx, dqy, a.m, h);
+        node_h<Dual, ND, TH>(x, dqx, dqy, a.m, h);
         double s2 = 0;
 #pragma unroll
         for (int v = 0; v < V; v++) s2 += wl[v] * h[v].d;
         g[j] = s2;
       }
     } else {
-      node_vjp<true, false, false>(q, dqx, ND == 2 ? dqy : nullptr, a.m, wl, nullptr, nullptr, g);
+      node_vjp<TH, true, false, false>(q, dqx, ND == 2 ? dqy : nullptr, a.m, wl, nullptr, nullptr, g);
     }
     // the dq inputs: h depends on dqx through -B_x(v) dqx
-    NodeState<double, false> st;
-    node_state<double, false>(q, a.m, st);
+    NodeState<double, TH> st;
+    node_state<double, TH>(q, a.m, st);
     double tmp[V];
-    bdot_T(st.v, wl, 0, tmp);
+    bdot_T<V>(st.v, wl, 0, tmp);
 #pragma unroll
     for (int v = 0; v < V; v++) gx[v] = -tmp[v] / a.dx;
     if (ND == 2) {
-      bdot_T(st.v, wl, 1, tmp);
+      bdot_T<V>(st.v, wl, 1, tmp);
 #pragma unroll
       for (int v = 0; v < V; v++) gy[v] = -tmp[v] / a.dy;
     }
     // d h / d parameter = par_fac S_A (theta1^-1 ~ 1/mu, or ~ sigmaY^-n)
     double S[V];
-    source<double, false>(q, st, a.m, S);
+    source<double, TH>(q, st, a.m, S);
     double gm = 0;
 #pragma unroll
     for (int k = 5; k < 14; k++) gm += wl[k] * S[k] * a.par_fac;
     atomicAdd(a.gmu, gm);
+    if constexpr (TH) {
+      double gk = 0;
+#pragma unroll
+      for (int k = 14; k < 17; k++) gk -= wl[k] * S[k] / a.kappa;
+      atomicAdd(a.gkap, gk);
+    }
   }
   __syncthreads();
   // scatter the dq adjoints along the lines: q_{t,k,ib} gets der[ia][k] gx(node)
@@ -713,9 +851,9 @@ __global__ __launch_bounds__(64) void ka_cells(AdjArgs a, long ncell_ext) {
 // state; Pi_s is an orthogonal projection, so Pi_s^T = Pi_s.  Both forms have
 // the same fixed point.  Then lam_wh = (d rhs / d w)^T m =
 // sum_t psi_t(0) W_s m(t, s), and gmu += (d rhs / d parameter)^T m.
-template <int N, int ND, bool DUAL>
+template <int N, int ND, bool DUAL, bool TH>
 __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
-  constexpr int V = 14;
+  constexpr int V = Dims<TH>::V;
   constexpr int NS = PredShape<N, ND>::NS;
   constexpr int NN = PredShape<N, ND>::NN;
   constexpr int CPW = PredShape<N, ND>::CPW;
@@ -724,7 +862,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   __shared__ double sA[CPW * NN * V];  // scratch: a_F, a_G, dq adjoints
   __shared__ double sM0[CPW * NN * V];  // the starting iterate, for a restart
   __shared__ double sRat[WAVE];
-  constexpr int PRE = 18;
+  constexpr int PRE = TH ? 27 : 18;
   __shared__ double sPre[CPW * NS * PRE];
   const Basis &K = basis_dev<N>();
   const int lane = threadIdx.x;
@@ -799,7 +937,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   bool failed = false, have_pre = false, used_pre = phase == 1;
   while (__ballot(phase < 2) != 0ull) {
     if (!have_pre && __ballot(phase == 1) != 0ull) {
-      if (active && t == 0) stiff_precond<N, false>(a.wh + (e * NS + s) * V, a.m, dt, sPre + (sl * NS + s) * PRE);
+      if (active && t == 0) stiff_precond<N, TH>(a.wh + (e * NS + s) * V, a.m, dt, sPre + (sl * NS + s) * PRE);
       have_pre = true;
       __syncthreads();
     }
@@ -825,25 +963,25 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
       aG[v] = ND == 2 ? -dt * Wt * K.w[ia] / a.dy * sG2 : 0.0;
     }
     double g[V];
-    if constexpr (!DUAL) node_vjp<true, true, ND == 2>(q, dqx, ND == 2 ? dqy : nullptr, a.m, ah, aF, aG, g);
+    if constexpr (!DUAL) node_vjp<TH, true, true, ND == 2>(q, dqx, ND == 2 ? dqy : nullptr, a.m, ah, aF, aG, g);
     for (int j = 0; j < (DUAL ? V : 0); j++) {
       Dual x[V], h[V], F[V], G[V];
 #pragma unroll
       for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
-      NodeState<Dual, false> sd;
-      node_state<Dual, false>(x, a.m, sd);
-      source<Dual, false>(x, sd, a.m, h);
-      Dual dq[14];
+      NodeState<Dual, TH> sd;
+      node_state<Dual, TH>(x, a.m, sd);
+      source<Dual, TH>(x, sd, a.m, h);
+      Dual dq[V];
 #pragma unroll
-      for (int v = 0; v < 14; v++) dq[v] = Dual(dqx[v]);
+      for (int v = 0; v < V; v++) dq[v] = Dual(dqx[v]);
       bdot_sub(sd.v, dq, 0, Dual(1.0), h);
       if (ND == 2) {
 #pragma unroll
-        for (int v = 0; v < 14; v++) dq[v] = Dual(dqy[v]);
+        for (int v = 0; v < V; v++) dq[v] = Dual(dqy[v]);
         bdot_sub(sd.v, dq, 1, Dual(1.0), h);
       }
-      flux<Dual, false>(x, sd, a.m, 0, F);
-      if (ND == 2) flux<Dual, false>(x, sd, a.m, 1, G);
+      flux<Dual, TH>(x, sd, a.m, 0, F);
+      if (ND == 2) flux<Dual, TH>(x, sd, a.m, 1, G);
       double acc = 0;
 #pragma unroll
       for (int v = 0; v < V; v++) {
@@ -854,8 +992,8 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
     }
     // the dq paths: h -= B_d(v) dq_d, dq_x at node n = sum_k der[a_n][k] q_{t,k,b} / dx
     double tx[V], ty[V];
-    bdot_T(vel, ah, 0, tx);
-    if (ND == 2) bdot_T(vel, ah, 1, ty);
+    bdot_T<V>(vel, ah, 0, tx);
+    if (ND == 2) bdot_T<V>(vel, ah, 1, ty);
     __syncthreads();
     if (lane_ok)
 #pragma unroll
@@ -907,9 +1045,12 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
       double acc = 0;
 #pragma unroll
       for (int k = 0; k < N; k++) acc += K.dgmatinv[k][t] * myA[(k * NS + s) * V + v];
-      if (phase == 1 && v >= 5)
+      if (phase == 1 && v >= 5 && v < 14)
 #pragma unroll
         for (int k = 0; k < N; k++) acc += (myPre[k * N + t] - K.dgmatinv[k][t]) * myQ[(k * NS + s) * V + v];
+      if (TH && phase == 1 && v >= 14)  // J rows: (DG_MAT + dt rJ W)^-T
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += (myPre[18 + k * N + t] - K.dgmatinv[k][t]) * myA[(k * NS + s) * V + v];
       mn[v] = phase == 1 ? mu_[v] + acc : acc;
     }
     // stopping rule over the cell's block, as the forward iteration's
@@ -986,22 +1127,27 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
         a.lam_wh[(e * NS + s) * V + v] = acc;
       }
     }
-    NodeState<double, false> st;
-    node_state<double, false>(q, a.m, st);
+    NodeState<double, TH> st;
+    node_state<double, TH>(q, a.m, st);
     double S[V];
-    source<double, false>(q, st, a.m, S);
+    source<double, TH>(q, st, a.m, S);
     double gm = 0;
 #pragma unroll
     for (int k = 5; k < 14; k++) gm += dt * Wt * Ws * mu_[k] * S[k] * a.par_fac;
     atomicAdd(a.gmu, gm);
+    if constexpr (TH) {
+      double gk = 0;
+#pragma unroll
+      for (int k = 14; k < 17; k++) gk -= dt * Wt * Ws * mu_[k] * S[k] / a.kappa;
+      atomicAdd(a.gkap, gk);
+    }
   }
 }
 
 // WENO y-pass transposed: lam_ux[i][j + r][a][v] += d wh[i][j][a][.][v] / d ux[i][j + r][a][v]
 // . lam_wh.  One thread per line (i, j, a, v), as k_weno_y.
-template <int N>
+template <int N, int V>
 __global__ void ka_weno_y(AdjArgs a) {
-  constexpr int V = 14;
   const Basis &K = basis_dev<N>();
   const int nx = a.nx, ny = a.ny;
   const int NX = nx + 2, NY = ny + 2, ncol = ny + 2 * N;
@@ -1030,9 +1176,8 @@ __global__ void ka_weno_y(AdjArgs a) {
 }
 
 // WENO x-pass transposed: lam_ub[i + r][j][v] += d ux[i][j][.][v] / d ub[i + r][j][v] . lam_ux
-template <int N>
+template <int N, int V>
 __global__ void ka_weno_x(AdjArgs a) {
-  constexpr int V = 14;
   const Basis &K = basis_dev<N>();
   const int nx = a.nx, ny = a.ny;
   const int NX = nx + 2, ncol = ny + 2 * N;
@@ -1061,9 +1206,8 @@ __global__ void ka_weno_x(AdjArgs a) {
 // Boundaries transposed (codes 0, 1, 2, 3, 5: linear copies with signs) and
 // the identity path: lam0 = lam_fv + sum over the ghost-extended positions
 // that copy each cell.  One thread per (ghost-extended cell, variable).
-template <int N>
+template <int N, int V>
 __global__ void ka_boundaries(AdjArgs a) {
-  constexpr int V = 14;
   const int nx = a.nx, ny = a.ny;
   const long nby = ny + 2 * N;
   const long total = (long)(nx + 2 * N) * nby * V;

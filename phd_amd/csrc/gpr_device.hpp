@@ -432,16 +432,22 @@ __device__ __forceinline__ cplx csqrt_(cplx z) {
   return {sre, z.im < 0 ? -sim : sim};
 }
 
-__device__ double max_abs_eig4(const double *a) {
+// With lam_out, the eigenvalue of largest modulus itself (one of a complex pair).
+__device__ double max_abs_eig4(const double *a, cplx *lam_out = nullptr) {
   cplx H[4][4];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) H[i][j] = {a[i * 4 + j], 0.0};
   double best = 0;
+  cplx lbest = {0.0, 0.0};
   int m = 4, its = 0;
   while (m > 0) {
-    if (m == 1) { best = fmax(best, cabs_(H[0][0])); break; }
+    if (m == 1) {
+      if (cabs_(H[0][0]) > best) lbest = H[0][0];
+      best = fmax(best, cabs_(H[0][0]));
+      break;
+    }
     double s = cabs_(H[m - 1][m - 1]) + cabs_(H[m - 2][m - 2]);
     double off = 0;
     for (int j = 0; j < m - 1; j++) off += cabs_(H[m - 1][j]);
@@ -449,6 +455,7 @@ __device__ double max_abs_eig4(const double *a) {
       for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) s += cabs_(H[i][j]);
     }
     if (off <= 1e-17 * s || off == 0.0) {
+      if (cabs_(H[m - 1][m - 1]) > best) lbest = H[m - 1][m - 1];
       best = fmax(best, cabs_(H[m - 1][m - 1]));
       m--;
       its = 0;
@@ -490,6 +497,7 @@ __device__ double max_abs_eig4(const double *a) {
       }
     for (int i = 0; i < m; i++) H[i][i] = cadd(H[i][i], mu);
   }
+  if (lam_out) *lam_out = lbest;
   return best;
 }
 

@@ -1658,22 +1658,24 @@ struct AdjHost {
   int dual_local = 0;     // 1: local derivatives by dual numbers instead of node_vjp
   double tol = 1e-13;     // its stopping tolerance
   int64_t stats[3] = {0, 0, 0};  // the last reverse sweep's iterations, cells, preconditioned cells
+  double grad_kappa = 0.0;        // the last reverse sweep's d loss / d kappa
 };
 
 namespace {
 
+template <int V>
 __global__ void k_renorm_only(double *u, long ncell, double rho0) {
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
     double q[14];
 #pragma unroll
-    for (int v = 0; v < 14; v++) q[v] = u[c * 14 + v];
+    for (int v = 0; v < 14; v++) q[v] = u[c * V + v];
     if (q[0] > 0) {  // as k_post
       double cf = cbrt(q[0] / (rho0 * det3(q + 5)));
 #pragma unroll
       for (int k = 0; k < 9; k++) q[5 + k] *= cf;
     }
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[c * 14 + 5 + k] = q[5 + k];
+    for (int k = 0; k < 9; k++) u[c * V + 5 + k] = q[5 + k];
   }
 }
 
@@ -1729,10 +1731,10 @@ int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt, const dou
   return dispatch<FvOp>(c, c, fa);
 }
 
-template <int N>
+template <int N, bool TH>
 int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *lam, double *d_gmu,
-                  const double *qh_saved) {
-  constexpr int V = 14;
+                  double *d_gkap, const double *qh_saved) {
+  constexpr int V = Dims<TH>::V;
   int rc;
   const int nx = c->run_nx, ny = c->run_ny;
   long nc = (long)nx * ny, ne = (long)(nx + 2) * (ny + 2);
@@ -1759,6 +1761,8 @@ int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *
   a.ub = (const double *)c->bufs["ub"].first;
   a.mask = (const uint8_t *)c->bufs["mask"].first;
   a.gmu = d_gmu;
+  a.gkap = d_gkap;
+  a.kappa = c->run_mat.p.kappa;
   a.fail = c->d_fail;
   a.dt = dt;
   a.dx = c->run_dx;
@@ -1780,23 +1784,23 @@ int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *
   HIP_TRY(hipMemsetAsync(a.lam_wh, 0, sizeof(double) * ne * N * N * V, c->stream));
   HIP_TRY(hipMemsetAsync(a.lam_ux, 0, sizeof(double) * nux * N * V, c->stream));
   HIP_TRY(hipMemsetAsync(a.lam_ub, 0, sizeof(double) * nb * V, c->stream));
-  hipLaunchKernelGGL(ka_renorm, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a, nc);
-  constexpr int FPW = FaceShape<N, false, 2>::FPW;
+  hipLaunchKernelGGL((ka_renorm<TH>), dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a, nc);
+  constexpr int FPW = FaceShape<N, TH, 2>::FPW;
   constexpr int CPW = PredShape<N, 2>::CPW;
   const dim3 gf((nfx + nfy + FPW - 1) / FPW), gc((ne + CPW - 1) / CPW);
   if (A->dual_local) {
-    hipLaunchKernelGGL((ka_faces<N, 2, true>), gf, dim3(WAVE), 0, c->stream, a);
-    hipLaunchKernelGGL((ka_cells<N, 2, true>), gc, dim3(WAVE), 0, c->stream, a, ne);
-    hipLaunchKernelGGL((ka_predictor<N, 2, true>), gc, dim3(WAVE), 0, c->stream, a, ne);
+    hipLaunchKernelGGL((ka_faces<N, 2, true, TH>), gf, dim3(WAVE), 0, c->stream, a);
+    hipLaunchKernelGGL((ka_cells<N, 2, true, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
+    hipLaunchKernelGGL((ka_predictor<N, 2, true, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
   } else {
-    hipLaunchKernelGGL((ka_faces<N, 2, false>), gf, dim3(WAVE), 0, c->stream, a);
-    hipLaunchKernelGGL((ka_cells<N, 2, false>), gc, dim3(WAVE), 0, c->stream, a, ne);
-    hipLaunchKernelGGL((ka_predictor<N, 2, false>), gc, dim3(WAVE), 0, c->stream, a, ne);
+    hipLaunchKernelGGL((ka_faces<N, 2, false, TH>), gf, dim3(WAVE), 0, c->stream, a);
+    hipLaunchKernelGGL((ka_cells<N, 2, false, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
+    hipLaunchKernelGGL((ka_predictor<N, 2, false, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
   }
-  hipLaunchKernelGGL((ka_weno_y<N>), dim3(grid_for(ne * N * V, 256)), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL((ka_weno_x<N>), dim3(grid_for(nux * V, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL((ka_weno_y<N, V>), dim3(grid_for(ne * N * V, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL((ka_weno_x<N, V>), dim3(grid_for(nux * V, 256)), dim3(256), 0, c->stream, a);
   HIP_TRY(hipMemcpyAsync(a.lam0, a.lam_fv, sizeof(double) * nc * V, hipMemcpyDeviceToDevice, c->stream));
-  hipLaunchKernelGGL((ka_boundaries<N>), dim3(grid_for(nb * V, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL((ka_boundaries<N, V>), dim3(grid_for(nb * V, 256)), dim3(256), 0, c->stream, a);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(lam, a.lam0, sizeof(double) * nc * V, hipMemcpyDeviceToDevice, c->stream));
   return GPR_OK;
@@ -1823,7 +1827,7 @@ int gpr_adj_release(gpr_ctx *c) {
 
 int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
   if (!c || !c->run_ready) return fail(GPR_E_ARG, "gpr_dev_begin has not been called");
-  if (c->ndim != 2 || c->thermal || c->N != 3) return fail(GPR_E_UNSUPPORTED, "the adjoint is built for 2D, N = 3, V = 14");
+  if (c->ndim != 2 || c->N != 3) return fail(GPR_E_UNSUPPORTED, "the adjoint is built for 2D, N = 3");
   if (nsteps < 1 || K < 1) return fail(GPR_E_ARG, "nsteps and K must be >= 1");
   if (c->run_mat.p.BINGHAM || (c->run_mat.p.POWER_LAW && !c->run_mat.p.SOLID))
     return fail(GPR_E_UNSUPPORTED,
@@ -1882,6 +1886,9 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   if (!lam) return rc;
   double *d_gmu = (double *)abuf(A, 10, sizeof(double), rc);
   if (!d_gmu) return rc;
+  double *d_gkap = (double *)abuf(A, 14, sizeof(double), rc);
+  if (!d_gkap) return rc;
+  HIP_TRY(hipMemsetAsync(d_gkap, 0, sizeof(double), c->stream));
   double *work = (double *)abuf(A, 11, sz, rc);
   if (!work) return rc;
   HIP_TRY(hipMemcpyAsync(lam, lam_T, sz, hipMemcpyHostToDevice, c->stream));
@@ -1920,7 +1927,11 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
           HIP_TRY(hipMemcpyAsync((char *)A->qhs + (n - n0) * qhb, c->bufs["qh"].first, qhb, hipMemcpyDeviceToDevice,
                                  c->stream));
         if (n + 1 < n1) {
-          hipLaunchKernelGGL(k_renorm_only, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, u_fv, nc,
+          if (c->thermal)
+            hipLaunchKernelGGL(k_renorm_only<17>, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, u_fv, nc,
+                               c->run_mat.p.rho0);
+          else
+            hipLaunchKernelGGL(k_renorm_only<14>, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, u_fv, nc,
                              c->run_mat.p.rho0);
           HIP_TRY(hipMemcpyAsync(work, u_fv, sz, hipMemcpyDeviceToDevice, c->stream));
         }
@@ -1928,7 +1939,9 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
     }
     for (int64_t n = n1 - 1; n >= n0; n--) {
       const double *qsave = A->qhs ? (const double *)((char *)A->qhs + (n - n0) * qhb) : nullptr;
-      rc = adj_step_back<3>(c, A, A->seg + (n - n0) * nc * c->V, A->dts[n], lam, d_gmu, qsave);
+      rc = c->thermal ? adj_step_back<3, true>(c, A, A->seg + (n - n0) * nc * c->V, A->dts[n], lam, d_gmu, d_gkap, qsave)
+                      : adj_step_back<3, false>(c, A, A->seg + (n - n0) * nc * c->V, A->dts[n], lam, d_gmu, d_gkap,
+                                                qsave);
       if (rc) return rc;
     }
   }
@@ -1938,6 +1951,7 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   HIP_TRY(hipMemcpyAsync(&nf, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   unsigned long long itc[3];
   HIP_TRY(hipMemcpyAsync(itc, d_itc, sizeof itc, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&A->grad_kappa, d_gkap, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int k = 0; k < 3; k++) A->stats[k] = (int64_t)itc[k];
   if (unconverged) *unconverged = nf;
@@ -1952,6 +1966,12 @@ int gpr_adj_set_solver(gpr_ctx *c, int start_preconditioned, double tol, int dua
   c->adj->start_pre = start_preconditioned;
   c->adj->tol = tol;
   c->adj->dual_local = dual_local;
+  return GPR_OK;
+}
+
+int gpr_adj_grad_kappa(gpr_ctx *c, double *out) {
+  if (!c || !c->adj || !out) return fail(GPR_E_ARG, "gpr_adj_forward has not been called");
+  *out = c->adj->grad_kappa;
   return GPR_OK;
 }
 

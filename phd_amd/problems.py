@@ -92,13 +92,18 @@ def double_shear_2d(n=512, μ=1e-3, b0=8.0):
     return u, MP, None, [dx, dx], [PERIODIC] * 4
 
 
-def grf_density_2d(n=256, seed=0, ell=0.1, μ=1e-3, p_amp=0.0):
+def grf_density_2d(n=256, seed=0, ell=0.1, μ=1e-3, p_amp=0.0, thermal=False):
     """ρ = 1 + 0.05·GRF (Gaussian spectrum of length scale ell, unit std),
     v=0, p=1, A = ρ^(1/3) I, periodic.  This start is a steady state (uniform
     pressure, no shear stress); p_amp > 0 sets p = 1 + p_amp·GRF instead, a
-    second field drawn after the first, which starts acoustic waves."""
+    second field drawn after the first, which starts acoustic waves.
+    thermal: the heat-conducting gas (V = 17, cα = 1, κ = μγcv/Pr with
+    Pr = 0.75), J = 0."""
     γ = 1.4
-    MP = material_params("sg", 1.0, cv=1.0, γ=γ, b0=1.0, μ=μ)
+    if thermal:
+        MP = material_params("sg", 1.0, cv=1.0, γ=γ, b0=1.0, μ=μ, cα=1.0, Pr=0.75)
+    else:
+        MP = material_params("sg", 1.0, cv=1.0, γ=γ, b0=1.0, μ=μ)
     rng = np.random.default_rng(seed)
     k = np.fft.fftfreq(n, d=1.0 / n)
     kx, ky = np.meshgrid(k, k, indexing="ij")
@@ -113,7 +118,7 @@ def grf_density_2d(n=256, seed=0, ell=0.1, μ=1e-3, p_amp=0.0):
     p = np.ones((n, n)) if p_amp == 0 else 1 + p_amp * field()
     v = np.zeros((n, n, 3))
     A = (ρ ** (1 / 3))[..., None, None] * np.eye(3)
-    u = cvec_field(ρ, p, v, MP, A)
+    u = cvec_field(ρ, p, v, MP, A, np.zeros((n, n, 3)) if thermal else None)
     return u, MP, None, [1.0 / n, 1.0 / n], [PERIODIC] * 4
 
 

@@ -319,3 +319,110 @@ def test_plastic_reverse_mode_equals_dual_numbers():
     print(f"plastic: max |lam0 difference| / max |lam0| {err:.2e}, dL/dsigmaY {g0:.12e} vs {g1:.12e}")
     assert err <= 1e-12
     assert abs(g1 - g0) <= 1e-12 * abs(g1)
+
+
+def thermal_case(seed=1, noise=1e-3):
+    """The random density field with a random pressure field, heat-conducting
+    (V = 17, c_alpha = 1, kappa = mu gamma cv / Pr), 16^2, mu = 1e-2, with the
+    small noise of case() in every variable (J included); 133-175 cell-steps of
+    ten take the stiff fallback."""
+    from phd_amd import problems
+    u0, MP, _, dX, bcs = problems.grf_density_2d(16, seed=seed, μ=1e-2, p_amp=0.05, thermal=True)
+    u0 = u0.ravel()
+    rng = np.random.default_rng(seed + 100)
+    u0 = u0 * (1 + noise * rng.standard_normal(u0.shape)) + 0.1 * noise * rng.standard_normal(u0.shape)
+    return u0, MP, [16, 16], dX, bcs
+
+
+def thermal_loss(u, target):
+    r = u.reshape(-1, 17)[:, 1] - target
+    return float(np.sum(r * r)), r
+
+
+@pytest.fixture(scope="module")
+def adj_th():
+    from phd_amd.adjoint import AdjointRun
+    print("thermal adjoint case 16x16, 10 steps, K = 4", flush=True)
+    u0, MP, nX, dX, bcs = thermal_case()
+    target = 0.99 * u0.reshape(-1, 17)[:, 1]
+    run = AdjointRun(u0, MP, nX, dX, bcs)
+    uT = run.forward(NSTEPS, K)
+    L, r = thermal_loss(uT, target)
+    lamT = np.zeros_like(uT).reshape(-1, 17)
+    lamT[:, 1] = 2 * r
+    lam0, gmu = run.backward(lamT.ravel())
+    yield {"run": run, "u0": u0, "MP": MP, "target": target, "L": L, "uT": uT, "lam0": lam0,
+           "gmu": gmu, "gkap": run.grad_kappa(), "lamT": lamT.ravel()}
+    run.close()
+
+
+def test_thermal_kappa_gradient_against_fd(adj_th):
+    """d L / d kappa (the heat conductivity; L on rho E) against central
+    differences at the recorded dt schedule."""
+    run, MP = adj_th["run"], adj_th["MP"]
+    h = 1e-6 * MP.kappa
+    fd = (thermal_loss(run.replay(kappa=MP.kappa + h), adj_th["target"])[0] -
+          thermal_loss(run.replay(kappa=MP.kappa - h), adj_th["target"])[0]) / (2 * h)
+    g = adj_th["gkap"]
+    print(f"dL/dkappa adjoint {g:.10e} fd {fd:.10e}")
+    assert abs(g - fd) <= 1e-5 * abs(fd)
+
+
+def test_thermal_mu_gradient_against_fd(adj_th):
+    run, MP = adj_th["run"], adj_th["MP"]
+    h = 1e-6 * MP.mu
+    fd = (thermal_loss(run.replay(mu=MP.mu + h), adj_th["target"])[0] -
+          thermal_loss(run.replay(mu=MP.mu - h), adj_th["target"])[0]) / (2 * h)
+    g = adj_th["gmu"]
+    print(f"thermal dL/dmu adjoint {g:.10e} fd {fd:.10e}")
+    assert abs(g - fd) <= 1e-5 * abs(fd)
+
+
+def test_thermal_initial_field_gradient_against_fd(adj_th):
+    run, u0 = adj_th["run"], adj_th["u0"]
+    rng = np.random.default_rng(31)
+    d = (rng.standard_normal((256, 17)) * np.maximum(np.abs(u0.reshape(-1, 17)), 1e-3)).ravel()
+    h = 1e-8
+    fd = (thermal_loss(run.replay(u0=u0 + h * d), adj_th["target"])[0] -
+          thermal_loss(run.replay(u0=u0 - h * d), adj_th["target"])[0]) / (2 * h)
+    ad = float(np.dot(adj_th["lam0"], d))
+    print(f"thermal all: adjoint {ad:.10e} fd {fd:.10e}")
+    assert abs(ad - fd) <= 2e-5 * abs(fd)
+
+
+def test_thermal_reverse_mode_equals_dual_numbers(adj_th):
+    """The heat-conduction terms of node_vjp against dual numbers through the
+    whole reverse sweep (the wave speed's derivative is shared by both)."""
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, nX, dX, bcs = thermal_case()
+    run = AdjointRun(u0, MP, nX, dX, bcs)
+    run.forward(NSTEPS, K, dual_local=True)
+    lam0, gmu = run.backward(adj_th["lamT"])
+    gk = run.grad_kappa()
+    run.close()
+    err = np.max(np.abs(lam0 - adj_th["lam0"])) / np.max(np.abs(lam0))
+    print(f"thermal: lam0 difference {err:.2e}, dL/dmu {gmu:.12e} vs {adj_th['gmu']:.12e}, "
+          f"dL/dkappa {gk:.12e} vs {adj_th['gkap']:.12e}")
+    assert err <= 1e-12
+    assert abs(gmu - adj_th["gmu"]) <= 1e-12 * abs(gmu)
+    assert abs(gk - adj_th["gkap"]) <= 1e-12 * abs(gk)
+
+
+def test_thermal_gradient_against_oracle_finite_differences(adj_th):
+    """The FP64 reference gradient for the heat-conducting case: central
+    differences of the oracle's map at the GPU run's dt schedule."""
+    from orcbind import Oracle
+    run, u0, MP = adj_th["run"], adj_th["u0"], adj_th["MP"]
+    O = Oracle(3, True, 2, dg_tol=1e-13, stiff_tol=1e-13, stiff_it=400)
+    nX, dX, bcs = [16, 16], [1 / 16, 1 / 16], [1, 1, 1, 1]
+    uT = oracle_steps_dt(O, u0, nX, dX, bcs, run.dts, MP)
+    assert relerr(uT, adj_th["uT"]) < 1e-11
+    d = np.zeros((256, 17))
+    d[:, 0] = np.random.default_rng(32).standard_normal(256)
+    d = d.ravel()
+    h = 1e-8
+    fd = (thermal_loss(oracle_steps_dt(O, u0 + h * d, nX, dX, bcs, run.dts, MP), adj_th["target"])[0] -
+          thermal_loss(oracle_steps_dt(O, u0 - h * d, nX, dX, bcs, run.dts, MP), adj_th["target"])[0]) / (2 * h)
+    ad = float(np.dot(adj_th["lam0"], d))
+    print(f"thermal rho: adjoint {ad:.10e} oracle fd {fd:.10e}")
+    assert abs(ad - fd) <= 2e-5 * abs(fd)
