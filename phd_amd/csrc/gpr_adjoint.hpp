@@ -1,23 +1,26 @@
-// Adjoint (reverse mode) of one plain FP64 ADER-WENO step, V = 14 (no heat
-// conduction), with the step's dt held fixed (the time steps of the forward
-// run).  With lambda the gradient of a loss with respect to u_{n+1}, the
-// kernels below give its gradient with respect to u_n and accumulate its
-// derivative with respect to the material's relaxation parameter (mu of a
-// Newtonian fluid, sigmaY of a power-law solid).  Order, the reverse of the
-// step:
+// Adjoint (reverse mode) of one plain FP64 ADER-WENO step in 2D, V = 14 or
+// 17 (heat conduction), with the step's dt held fixed (the time steps of the
+// forward run).  With lambda the gradient of a loss with respect to u_{n+1},
+// the kernels below give its gradient with respect to u_n and accumulate its
+// derivatives with respect to the material's relaxation parameter (mu of a
+// Newtonian fluid, sigmaY of a power-law solid) and, with heat conduction,
+// the conductivity kappa.  Order, the reverse of the step:
 //   ka_renorm   u_{n+1} = renorm(u_fv)                    (utils.cpp:6-23)
 //   ka_faces    u_fv -= face terms of qh                  (fv.cpp:162-268)
 //   ka_cells    u_fv += cell terms of qh; gathers faces  (fv.cpp:56-109)
-//   ka_predictor  qh solves the DG equations R(qh; wh, parameter) = 0 (dg2.cpp):
-//               (dR/dq)^T m = lambda_qh by the transposed Picard
-//               iteration, then lambda_wh = -(dR/dwh)^T m
+//   ka_predictor  qh solves the DG equations R(qh; wh, parameters) = 0
+//               (dg2.cpp): (dR/dq)^T m = lambda_qh by a transposed
+//               fixed-point iteration, then lambda_wh = -(dR/dwh)^T m
 //   ka_weno_y, ka_weno_x   wh = WENO_y(WENO_x(ub))         (weno.cpp:83-124)
 //   ka_boundaries          ub from u (grid.cpp:147-170), plus the identity
 //                          path u_fv = u_n + ...
-// Local derivatives come from forward-mode dual numbers (gpr_dual.hpp) on the
-// same templated physics the forward step runs.  The predictor is
-// differentiated at its solution (implicit function theorem); the forward run
-// that a gradient is checked against solves it tightly (dg_tol 1e-13).
+// Local derivatives of the node physics come from the hand-written reverse
+// mode node_vjp, or (DUAL) from forward-mode dual numbers (gpr_dual.hpp) on
+// the same templated physics the forward step runs; the wave speed's come
+// from dual numbers (heat conduction: Jacobi's formula on the 4x4 tensor's
+// eigenvalue).  The predictor is differentiated at its solution (implicit
+// function theorem); the forward run that a gradient is checked against
+// solves it tightly (dg_tol 1e-13).
 #pragma once
 
 #include "gpr_dual.hpp"
