@@ -864,6 +864,11 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   __shared__ double sM[CPW * NN * V];  // m
   __shared__ double sA[CPW * NN * V];  // scratch: a_F, a_G, dq adjoints
   __shared__ double sM0[CPW * NN * V];  // the starting iterate, for a restart
+  // each node's dq/dx, dq/dy (read by its own lane only): in LDS for V = 14,
+  // where it takes register pressure off the iteration; in registers for
+  // V = 17, where the LDS would no longer fit four wavefronts per CU
+  __shared__ double sDQ[TH ? 1 : 2 * CPW * NN * V];
+  double dq_reg[TH ? 2 * V : 1];
   __shared__ double sRat[WAVE];
   constexpr int PRE = TH ? 27 : 18;
   __shared__ double sPre[CPW * NS * PRE];
@@ -895,7 +900,8 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
 #pragma unroll
     for (int v = 0; v < V; v++) myQ[node * V + v] = q[v];
   __syncthreads();
-  double dqx[V], dqy[V];
+  double *dqx = TH ? dq_reg : sDQ + (sl * NN + node) * V;
+  double *dqy = TH ? dq_reg + V : sDQ + (CPW * NN + sl * NN + node) * V;
 #pragma unroll
   for (int v = 0; v < V; v++) {
     double ax = 0, ay = 0;
@@ -904,8 +910,10 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
       ax += K.der[ia][k] * myQ[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
       if (ND == 2) ay += K.der[ib][k] * myQ[(t * NS + ia * N + k) * V + v];
     }
-    dqx[v] = ax / a.dx;
-    dqy[v] = ND == 2 ? ay / a.dy : 0.0;
+    if (TH || lane_ok) {  // in LDS, lanes past the last whole cell read slot 0's values and write none
+      dqx[v] = ax / a.dx;
+      dqy[v] = ND == 2 ? ay / a.dy : 0.0;
+    }
   }
   double vel[3];  // the node's velocity, for the transposed B products
 #pragma unroll
