@@ -20,6 +20,7 @@ respect to it; `grad_kappa` gives the heat conductivity's.
 
     u0_t = torch.tensor(u0, requires_grad=True); mu_t = torch.tensor(1e-3, requires_grad=True)
     uT = StepsFunction.apply(u0_t, mu_t, spec)   # spec: dict of MP, nX, dX, bcs, nsteps, K
+    # heat-conducting materials: StepsFunction.apply(u0_t, mu_t, spec, kappa_t)
     loss = ((uT.view(-1, 14)[:, 0] - target) ** 2).sum(); loss.backward()
 """
 
@@ -144,23 +145,29 @@ try:
     import torch
 
     class StepsFunction(torch.autograd.Function):
-        """u_T = Steps(u_0, p) over `spec` (MP, nX, dX, bcs, nsteps, K; CPU
-        tensors in and out, the steps and their adjoint on the GPU); p is the
-        material's relaxation parameter (mu, or sigmaY for a power-law
-        solid)."""
+        """u_T = Steps(u_0, p[, kappa]) over `spec` (MP, nX, dX, bcs, nsteps,
+        K; CPU tensors in and out, the steps and their adjoint on the GPU); p
+        is the material's relaxation parameter (mu, or sigmaY for a power-law
+        solid) and the optional kappa the heat conductivity of a
+        heat-conducting material (V = 17)."""
 
         @staticmethod
-        def forward(ctx, u0, mu, spec):
+        def forward(ctx, u0, mu, spec, kappa=None):
             MP = _with_param(spec["MP"], float(mu))
+            if kappa is not None:
+                MP = _with_kappa(MP, float(kappa))
             run = AdjointRun(u0.detach().cpu().numpy(), MP, spec["nX"], spec["dX"], spec["bcs"])
             uT = run.forward(spec["nsteps"], spec["K"])
             ctx.run = run
+            ctx.has_kappa = kappa is not None
             return torch.from_numpy(uT.copy())
 
         @staticmethod
         def backward(ctx, grad_uT):
             lam0, gmu = ctx.run.backward(grad_uT.detach().cpu().numpy())
+            gkap = (torch.tensor(ctx.run.grad_kappa(), dtype=torch.float64)
+                    if ctx.has_kappa else None)
             ctx.run.close()
-            return (torch.from_numpy(lam0), torch.tensor(gmu, dtype=torch.float64), None)
+            return (torch.from_numpy(lam0), torch.tensor(gmu, dtype=torch.float64), None, gkap)
 except ImportError:  # the adjoint itself needs no torch
     StepsFunction = None

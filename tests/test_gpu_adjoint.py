@@ -378,6 +378,27 @@ def test_thermal_mu_gradient_against_fd(adj_th):
     assert abs(g - fd) <= 1e-5 * abs(fd)
 
 
+def test_thermal_torch_function_gradients(adj_th):
+    """StepsFunction with the heat conductivity as a fourth input gives the
+    adjoint run's d/du0, d/dmu and d/dkappa."""
+    import torch
+    from phd_amd.adjoint import StepsFunction
+    MP = adj_th["MP"]
+    spec = {"MP": MP, "nX": [16, 16], "dX": [1 / 16, 1 / 16],
+            "bcs": [1, 1, 1, 1], "nsteps": NSTEPS, "K": K}
+    u0_t = torch.tensor(adj_th["u0"], requires_grad=True)
+    mu_t = torch.tensor(MP.mu, dtype=torch.float64, requires_grad=True)
+    kap_t = torch.tensor(MP.kappa, dtype=torch.float64, requires_grad=True)
+    uT = StepsFunction.apply(u0_t, mu_t, spec, kap_t)
+    tgt = torch.from_numpy(adj_th["target"])
+    L = ((uT.view(-1, 17)[:, 1] - tgt) ** 2).sum()
+    L.backward()
+    assert abs(L.item() - adj_th["L"]) <= 1e-14 * adj_th["L"]
+    assert np.allclose(u0_t.grad.numpy(), adj_th["lam0"], rtol=1e-12, atol=1e-14)
+    assert abs(mu_t.grad.item() - adj_th["gmu"]) <= 1e-12 * abs(adj_th["gmu"])
+    assert abs(kap_t.grad.item() - adj_th["gkap"]) <= 1e-12 * abs(adj_th["gkap"])
+
+
 def test_thermal_initial_field_gradient_against_fd(adj_th):
     run, u0 = adj_th["run"], adj_th["u0"]
     rng = np.random.default_rng(31)
