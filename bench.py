@@ -74,7 +74,7 @@ MI355X_HBM_GBS = 8000.0
 PREDICTOR_TRAFFIC_512 = 1235.2e6
 
 
-def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads):
+def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0, half_step=0):
     """The reference's own C++ step loop (oracle/_ref, built from its sources
     by oracle/Makefile) on `nsteps` steps from the GPU's timed starting state,
     OpenMP over `threads` cores; seconds per step as the reference's loop
@@ -90,7 +90,8 @@ def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads):
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "state.npz")
         np.savez(path, u=u_start, nX=np.array(nX), dX=np.array(dX), bcs=np.array(bcs), t=t0,
-                 count=count0, nsteps=nsteps, V=14, cfl=0.8, par=json.dumps(MP.as_dict()))
+                 count=count0, nsteps=nsteps, V=14, cfl=0.8, par=json.dumps(MP.as_dict()),
+                 split=split, half_step=half_step)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         p = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "refbind.py"), path],
                            env=env, capture_output=True, text=True, timeout=600)
@@ -307,6 +308,42 @@ def adjoint_leg(n, steps, K):
     return out
 
 
+def split_leg(n, steps, warmup, threads, cpu):
+    """The split-WENO step (iterator's SPLIT, steppers.cpp:50-94) on the same
+    512^2 double shear layer, without and with HALF_STEP: warm-up steps, then
+    `steps` timed steps of the device loop; the reference's own split step
+    timed on the host cores from the same state."""
+    from phd_amd import problems
+    from phd_amd.runner import DeviceRun
+    u0, MP, _, dX, bcs = problems.double_shear_2d(n)
+    nX = [n, n]
+    out = {"config": f"2D GPR double shear layer {n}x{n} periodic, split-WENO step "
+                     f"(closed-form source ODE, WENO, FV), FP64, P2, CFL 0.8, "
+                     f"steps {warmup}-{warmup + steps}", "cells": n * n, "steps": steps}
+    for hs in (0, 1):
+        run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, split=1, half_step=hs)
+        run.steps(warmup)
+        st0 = run.status()
+        u_w = run.host()
+        sec, tm = timed_run(run, steps)
+        run.close()
+        r = {"value": n * n * steps / sec, "ms_per_step": 1e3 * sec / steps,
+             "phase_ms_per_step": {("boundaries_ode" if k == "boundaries" else
+                                    "midstepper" if k == "predictor" else
+                                    "fv_ode_post" if k == "fv" else k): v / tm["steps"]
+                                   for k, v in tm["ms"].items()}}
+        if cpu:
+            ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], 1, threads,
+                                split=1, half_step=hs)
+            if ref is not None and "error" not in ref:
+                r["cpu_reference"] = {"value": n * n * ref["steps"] / ref["sec"], "cores": threads,
+                                      "sample": "1 split step from the GPU's timed starting state, "
+                                                "the reference C++ (oracle/_ref)"}
+                r["gpu_over_cpu"] = r["value"] / r["cpu_reference"]["value"]
+        out["half_step" if hs else "plain"] = r
+    return out
+
+
 def timed_run(run, steps):
     """K steps bracketed by a synchronisation of the library's stream (all
     device work of this process runs on it); HIP-event phase timing on the
@@ -344,6 +381,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-sample", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-WENO leg")
     ap.add_argument("--adj-n", type=int, default=256)
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--adj-k", type=int, default=25)
@@ -427,6 +465,10 @@ def main():
         progress("batched leg done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps, args.adj_k)
+    if not args.no_split:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["split"] = split_leg(n, args.steps, args.warmup, threads, not args.no_cpu)
+        progress("split leg done")
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,

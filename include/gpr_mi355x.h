@@ -59,6 +59,9 @@ typedef struct gpr_solver_opts {
   int32_t precision;  /* device loop only (gpr_dev_steps): 0 FP64 throughout;
                          1 FP32 bulk: WENO and Picard in FP32, the stiff
                          fallback, FV accumulation, post-step and CFL in FP64 */
+  int32_t split;      /* device loop: 0 the ADER step; 1 the split-WENO step
+                         (steppers.cpp:50-66, iterator's SPLIT), FP64 only */
+  int32_t half_step;  /* with split: the midstepper between WENO and FV (HALF_STEP) */
 } gpr_solver_opts;
 
 const char *gpr_last_error(void);
@@ -116,7 +119,9 @@ int gpr_predictor(gpr_ctx *ctx, double *qh, const double *wh, double dt, const d
                   int64_t ncell, int stiff, int stiff_ig, const gpr_par *MP, const uint8_t *mask,
                   int32_t *iters);
 /* solvers.fv.fv_launcher(u, rec, nX, dt, dX, SOURCES, TIME, FLUX, MP, mask)
- * (fv.h:17-18); SOURCES = TIME = 1 and FLUX = 0 (Rusanov) are supported. */
+ * (fv.h:17-18), FLUX = 0 (Rusanov).  TIME = 1: rec is the predictor's qh
+ * ([cells+halo][N time][N^d space][V]); TIME = 0: one time level per cell
+ * ([cells+halo][N^d][V], the split step's wh), weighted by dt (fv.cpp:46-54). */
 int gpr_fv_launcher(gpr_ctx *ctx, double *u, const double *qh, const int32_t *nX, double dt,
                     const double *dX, int sources, int time, int flux, const gpr_par *MP,
                     const uint8_t *mask);
@@ -126,13 +131,34 @@ int gpr_timestep(gpr_ctx *ctx, double *dt_out, const double *u, const int32_t *n
 /* solvers.ader_stepper(u, ub, nX, dt, dX, STIFF, FLUX, MP, mask) (steppers.h:7-8) */
 int gpr_ader_stepper(gpr_ctx *ctx, double *u, const double *ub, const int32_t *nX, double dt,
                      const double *dX, int stiff, int flux, const gpr_par *MP, const uint8_t *mask);
+/* solvers.split_stepper(u, ub, nX, dt, dX, HALF_STEP, FLUX, MP, mask)
+ * (steppers.h:13-14, steppers.cpp:50-66): half-step source ODE on u and on ub
+ * (both written in place), WENO of ub, the midstepper if HALF_STEP, the FV
+ * update with TIME = SOURCES = 0, and the closing half-step ODE on u.
+ * solvers.split_stepper_para is the same step on a private copy of ub. */
+int gpr_split_stepper(gpr_ctx *ctx, double *u, double *ub, const int32_t *nX, double dt,
+                      const double *dX, int half_step, int flux, const gpr_par *MP,
+                      const uint8_t *mask);
+/* solvers.split.ode_launcher(u, dt, MP) (solvers/split/ode.h:6, ode.cpp:5-11):
+ * the closed-form source ODE (relaxation/analytic.cpp:154-162) over dt in each
+ * of the ncell states of u.  Reactive materials (REACTION > -1) are refused. */
+int gpr_ode_launcher(gpr_ctx *ctx, double *u, int64_t ncell, double dt, const gpr_par *MP);
+/* solvers.split.analyticSolver_distortion (part 0, analytic.cpp:72-129) and
+ * analyticSolver_thermal (part 1, analytic.cpp:131-152) of count states in place */
+int gpr_analytic_solver(gpr_ctx *ctx, double *Q, int64_t count, double dt, const gpr_par *MP, int part);
+/* solvers.split.midstepper(wh, dt, dX, MP, mask) (homogeneous.h:7,
+ * homogeneous.cpp:5-67): wh ([ncell][N^d][V]) advanced by dt/2 under the
+ * homogeneous system in each masked cell. */
+int gpr_midstepper(gpr_ctx *ctx, double *wh, double dt, const double *dX, int64_t ncell,
+                   const gpr_par *MP, const uint8_t *mask);
 /* solvers.renorm_distortion / contorted / reset_distortion (utils.cpp) */
 int gpr_renorm_distortion(gpr_ctx *ctx, double *u, int64_t ncell, const gpr_par *MP);
 int gpr_contorted(gpr_ctx *ctx, int32_t *result, const double *u, int64_t ncell, double tol);
 int gpr_reset_distortion(gpr_ctx *ctx, double *u, int64_t ncell, const gpr_par *MP);
 
 /* solvers.iterator (iterator.cpp:85-170) for one material (LSET = 0),
- * SPLIT = 0, FLUX = 0.  u is advanced in place to tf.  snapshots
+ * FLUX = 0; SPLIT = 1 takes the split-WENO step (with HALF_STEP) instead of
+ * the ADER step.  u is advanced in place to tf.  snapshots
  * ([100][ncell*V], may be null) receives u at each 1% of tf as the reference's
  * returned list; *nsnap the number filled.  On a non-finite state the loop
  * stops as at iterator.cpp:161-165: the last two snapshots hold the previous

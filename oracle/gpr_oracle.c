@@ -1559,19 +1559,29 @@ int orc_predictor(double *qh, const double *wh, double dt, const double *dX,
 void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
                      const double *dX, const gpr_par *p, const uint8_t *mask,
                      const orc_cfg *c) {
+  orc_fv_launcher_ex(u, qh, nX, dt, dX, p, mask, 1, 1, c);
+}
+
+/* fv.cpp:11-297 with fv_launcher's SOURCES and TIME flags: TIME = 0 reads one
+ * time level per cell (nt = 1) with weight dt (fv.cpp:46-54, 99-109, 151-160,
+ * 270-279) */
+void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double dt,
+                        const double *dX, const gpr_par *p, const uint8_t *mask,
+                        int sources, int time, const orc_cfg *c) {
   const consts_t *K = get_consts(c->N);
   int N = c->N, V = c->thermal ? 17 : 14, th = c->thermal;
+  int nt = time ? N : 1;
   double S[MAXV], b1[MAXV], b2[MAXV], f[MAXV], bb[MAXV];
   if (c->ndim == 1) {
     int nx = nX[0];
     double dx = dX[0];
     /* centers1 (fv.cpp:11-54) */
-    for (int t = 0; t < N; t++) {
-      double wt = dt * K->w[t];
+    for (int t = 0; t < nt; t++) {
+      double wt = time ? dt * K->w[t] : dt;
       for (int i = 0; i < nx; i++) {
         if (!mask[i + 1])
           continue;
-        const double *qc = qh + ((long)(i + 1) * N + t) * N * V;
+        const double *qc = qh + ((long)(i + 1) * nt + t) * N * V;
         for (int s = 0; s < N; s++) {
           double dq[MAXV];
           for (int v = 0; v < V; v++) {
@@ -1580,7 +1590,10 @@ void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
               acc += K->der[s][j] * qc[j * V + v];
             dq[v] = acc;
           }
-          orc_source(S, qc + s * V, p, th);
+          if (sources)
+            orc_source(S, qc + s * V, p, th);
+          else
+            memset(S, 0, sizeof S);
           orc_bdot(b1, qc + s * V, dq, 0, th);
           for (int v = 0; v < V; v++)
             u[(long)i * V + v] += wt * K->w[s] * (S[v] - b1[v] / dx);
@@ -1588,13 +1601,13 @@ void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
       }
     }
     /* interfs1 (fv.cpp:111-160) */
-    for (int t = 0; t < N; t++) {
-      double k = dt * K->w[t] / (2. * dx);
+    for (int t = 0; t < nt; t++) {
+      double k = (time ? dt * K->w[t] : dt) / (2. * dx);
       for (int i = 0; i < nx + 1; i++) {
         if (!(mask[i] && mask[i + 1]))
           continue;
-        const double *ql = qh + ((long)i * N + t) * N * V;
-        const double *qr = qh + ((long)(i + 1) * N + t) * N * V;
+        const double *ql = qh + ((long)i * nt + t) * N * V;
+        const double *qr = qh + ((long)(i + 1) * nt + t) * N * V;
         double L[MAXV], R[MAXV];
         for (int v = 0; v < V; v++) {
           double a = 0, b = 0;
@@ -1621,14 +1634,14 @@ void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
   double dx = dX[0], dy = dX[1];
   int NNV = N * N * V;
   /* centers2 (fv.cpp:56-109) */
-  for (int t = 0; t < N; t++) {
-    double wt = dt * K->w[t];
+  for (int t = 0; t < nt; t++) {
+    double wt = time ? dt * K->w[t] : dt;
     for (int i = 0; i < nx; i++)
       for (int j = 0; j < ny; j++) {
         long e = (long)(i + 1) * (ny + 2) + (j + 1);
         if (!mask[e])
           continue;
-        const double *qc = qh + (e * N + t) * NNV;
+        const double *qc = qh + (e * nt + t) * NNV;
         for (int a = 0; a < N; a++)
           for (int b = 0; b < N; b++) {
             double dqx[MAXV], dqy[MAXV];
@@ -1642,7 +1655,10 @@ void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
               dqy[v] = sy;
             }
             const double *qs = qc + (a * N + b) * V;
-            orc_source(S, qs, p, th);
+            if (sources)
+              orc_source(S, qs, p, th);
+            else
+              memset(S, 0, sizeof S);
             orc_bdot(b1, qs, dqx, 0, th);
             orc_bdot(b2, qs, dqy, 1, th);
             for (int v = 0; v < V; v++) {
@@ -1654,19 +1670,19 @@ void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
       }
   }
   /* interfs2 (fv.cpp:162-279) */
-  for (int t = 0; t < N; t++) {
-    double wts = dt * K->w[t];
+  for (int t = 0; t < nt; t++) {
+    double wts = time ? dt * K->w[t] : dt;
     for (int i = 0; i < nx + 1; i++)
       for (int j = 0; j < ny + 1; j++) {
         long e0 = (long)i * (ny + 2) + j;
         if (!mask[e0])
           continue;
-        const double *q0c = qh + (e0 * N + t) * NNV;
+        const double *q0c = qh + (e0 * nt + t) * NNV;
         for (int d = 0; d < 2; d++) {
           long e1 = d == 0 ? e0 + (ny + 2) : e0 + 1;
           if (!mask[e1] || (d == 0 ? j <= 0 : i <= 0))
             continue;
-          const double *q1c = qh + (e1 * N + t) * NNV;
+          const double *q1c = qh + (e1 * nt + t) * NNV;
           double u0[MAXV] = {0}, u1[MAXV] = {0};
           double kk = wts / (2. * (d == 0 ? dx : dy));
           for (int s = 0; s < N; s++) {
@@ -1811,4 +1827,307 @@ int orc_step(double *u, const int32_t *nX, const double *dX, double CFL,
     if (u[i] != u[i])
       return -1;
   return rc;
+}
+
+/* ========================================================= split-WENO step
+ * solvers/steppers.cpp:50-94, solvers/split/{ode,homogeneous}.cpp,
+ * system/relaxation/analytic.cpp. */
+
+/* 3x3 SVD A = U diag(s) V^T: cyclic Jacobi on the symmetric G = A^T A gives V
+ * and s^2, then U = A V / s.  s descending by selection of the largest
+ * remaining value, ties kept in column order (JacobiSVD's ordering). */
+static void orc_svd3(const double *A, double *U, double *s, double *Vm) {
+  double G[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      G[i * 3 + j] = A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j];
+  for (int k = 0; k < 9; k++)
+    Vm[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 50; sweep++) {
+    double off = G[1] * G[1] + G[2] * G[2] + G[5] * G[5];
+    double dia = G[0] * G[0] + G[4] * G[4] + G[8] * G[8];
+    if (off <= 1e-34 * dia)
+      break;
+    for (int pr = 0; pr < 3; pr++) {
+      int p = pr == 2 ? 1 : 0, q = pr == 0 ? 1 : 2;
+      double apq = G[p * 3 + q];
+      if (apq == 0.0)
+        continue;
+      double th = (G[q * 3 + q] - G[p * 3 + p]) / (2 * apq);
+      double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+      double c = 1 / sqrt(t * t + 1), sn = t * c;
+      for (int k = 0; k < 3; k++) { /* G P */
+        double a = G[k * 3 + p], b = G[k * 3 + q];
+        G[k * 3 + p] = c * a - sn * b;
+        G[k * 3 + q] = sn * a + c * b;
+      }
+      for (int k = 0; k < 3; k++) { /* P^T G */
+        double a = G[p * 3 + k], b = G[q * 3 + k];
+        G[p * 3 + k] = c * a - sn * b;
+        G[q * 3 + k] = sn * a + c * b;
+      }
+      for (int k = 0; k < 3; k++) {
+        double a = Vm[k * 3 + p], b = Vm[k * 3 + q];
+        Vm[k * 3 + p] = c * a - sn * b;
+        Vm[k * 3 + q] = sn * a + c * b;
+      }
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    double sj = sqrt(G[j * 3 + j] > 0 ? G[j * 3 + j] : 0);
+    s[j] = sj;
+    for (int i = 0; i < 3; i++) {
+      double x = A[i * 3 + 0] * Vm[0 * 3 + j] + A[i * 3 + 1] * Vm[1 * 3 + j] + A[i * 3 + 2] * Vm[2 * 3 + j];
+      U[i * 3 + j] = sj > 0 ? x / sj : (i == j ? 1.0 : 0.0);
+    }
+  }
+  for (int i = 0; i < 2; i++) {
+    int pos = i;
+    for (int j = i + 1; j < 3; j++)
+      if (s[j] > s[pos])
+        pos = j;
+    if (pos != i) {
+      double tmp = s[i];
+      s[i] = s[pos];
+      s[pos] = tmp;
+      for (int k = 0; k < 3; k++) {
+        tmp = U[k * 3 + i];
+        U[k * 3 + i] = U[k * 3 + pos];
+        U[k * 3 + pos] = tmp;
+        tmp = Vm[k * 3 + i];
+        Vm[k * 3 + i] = Vm[k * 3 + pos];
+        Vm[k * 3 + pos] = tmp;
+      }
+    }
+  }
+}
+
+static double posv(double x) { return x > 0. ? x : 0.; } /* std::max(0., x) */
+static const double ORC_PI = 3.14159265358979323846;
+
+/* analytic.cpp:17-70 */
+static double nondim_time(double r, double detA3, double m0, double u0, double dt,
+                          const gpr_par *p) {
+  if (p->POWER_LAW) {
+    double n = p->n;
+    double cs2 = c_s2(r, p);
+    double a = 9. * m0 - u0 - 9.;
+    double b = 6. * m0 - u0 - 6.;
+    double c, lam, t1, t2;
+    if (p->SOLID) {
+      c = (108. * a - 324. * b + 108. * a * a - 396. * a * b + 297. * b * b -
+           24. * (a * a * b - 2. * a * b * b + b * b * b) - 4. * pow(a - b, 4.));
+      if (c <= 0.)
+        return 0.;
+      lam = c / (18. * a - 36. * b + 9. * a * a - 132. / 5. * a * b + 33. / 2. * b * b -
+                 8. / 7. * a * a * b + 2. * a * b * b - 8. / 9. * b * b * b - pow(a, 4.) / 6. +
+                 16. / 27. * a * a * a * b - 4. / 5. * a * a * b * b + 16. / 33. * a * b * b * b -
+                 pow(b, 4.) / 9.);
+      t1 = n * lam / p->tau0 * pow(detA3, 4. * n + 7.);
+      t2 = pow(sqrt(c) * r * cs2 / (6. * p->sigmaY), n);
+      return 2. / (n * lam) * log(t1 * t2 * dt + 1.);
+    }
+    double k = (1 - n) / n;
+    c = (108. * a - 324. * b + 180. * a * a - 612. * a * b + 459. * b * b -
+         24. * (a * a * b - 2. * a * b * b + b * b * b) - 4. * pow(a - b, 4.));
+    if (c <= 0.)
+      return 0.;
+    lam = c / (18. * a - 36. * b + 15. * a * a - 204. / 5. * a * b + 51. / 2. * b * b -
+               8. / 7. * a * a * b + 2. * a * b * b - 8. / 9. * b * b * b - pow(a, 4.) / 6. +
+               16. / 27. * a * a * a * b - 4. / 5. * a * a * b * b + 16. / 33. * a * b * b * b -
+               pow(b, 4.) / 9.);
+    double tau1 = 6 * pow(p->mu, 1. / n) / (p->rho0 * p->b02);
+    t1 = k * lam / tau1 * pow(detA3, 4. * k + 7.);
+    t2 = pow(sqrt(c) * r * cs2 / (6. * sqrt(3.)), k);
+    return 2. / (k * lam) * log(t1 * t2 * dt + 1.);
+  }
+  return p->rho0 * p->b02 * pow(detA3, 7.) / (3. * p->mu) * dt;
+}
+
+/* analytic.cpp:72-129 */
+static void analytic_distortion(double *Q, double dt, const gpr_par *p) {
+  double *A = Q + 5;
+  double U[9], s[3], Vm[9];
+  orc_svd3(A, U, s, Vm);
+  double detA = s[0] * s[1] * s[2];
+  double r = detA * p->rho0;
+  double d13 = cbrt(detA);
+  double d23 = d13 * d13;
+  double s0 = s[0] * s[0] / d23, s1 = s[1] * s[1] / d23, s2 = s[2] * s[2] / d23;
+  double m0 = (s0 + s1 + s2) / 3.;
+  double u0 = ((s0 - s1) * (s0 - s1) + (s1 - s2) * (s1 - s2) + (s2 - s0) * (s2 - s0)) / 3.;
+  if (!(u0 >= 1e-12))
+    return;
+  double tau = nondim_time(r, d13, m0, u0, dt, p);
+  double e6 = exp(-6. * tau), e9 = exp(-9. * tau);
+  double a = 3. * m0 - u0 / 3. - 3.;
+  double b = 2. * m0 - u0 / 3. - 2.;
+  double m = 1. + a * e6 - b * e9;
+  double u = posv(6. * a * e6 - 9. * b * e9);
+  double D = -2. * m * m * m + m * u + 2.;
+  double arg1 = posv(6. * u * u * u - 81. * D * D);
+  double x1;
+  if (fabs(D) < 1e-12)
+    x1 = sqrt(6. * u) / 3. * cos(ORC_PI / 6.) + m;
+  else if (arg1 == 0.) {
+    double t = cbrt(54. * D);
+    x1 = t / 6. + u / t + m;
+  } else {
+    double th = atan(sqrt(arg1) / (9. * D));
+    x1 = sqrt(6. * u) / 3. * cos(th / 3.) + m;
+  }
+  double t2 = 3. * m - x1;
+  double arg2 = posv(x1 * t2 * t2 - 4.);
+  double x2 = 0.5 * (sqrt(arg2 / x1) + t2);
+  double x3 = 1. / (x1 * x2);
+  double x[3] = {x1, x2, x3}, tmp; /* ascending */
+  if (x[1] < x[0]) { tmp = x[0]; x[0] = x[1]; x[1] = tmp; }
+  if (x[2] < x[1]) { tmp = x[1]; x[1] = x[2]; x[2] = tmp; }
+  if (x[1] < x[0]) { tmp = x[0]; x[0] = x[1]; x[1] = tmp; }
+  double g[3];
+  for (int i = 0; i < 3; i++)
+    g[i] = d13 * sqrt(x[2 - i]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      A[i * 3 + j] = U[i * 3 + 0] * (g[0] * Vm[j * 3 + 0]) + U[i * 3 + 1] * (g[1] * Vm[j * 3 + 1]) +
+                     U[i * 3 + 2] * (g[2] * Vm[j * 3 + 2]);
+}
+
+/* analytic.cpp:131-152 */
+static void analytic_thermal(double *Q, double dt, const gpr_par *p) {
+  double r = Q[0];
+  double E = Q[1] / r;
+  double v0 = Q[2] / r, v1 = Q[3] / r, v2 = Q[4] / r;
+  double c1 = E - E_2A(r, Q + 5, p) - (v0 * v0 + v1 * v1 + v2 * v2) / 2;
+  double c2 = p->calpha2 / 2.;
+  double k = 2 * p->calpha2 / (p->kappa * r * p->cv);
+  c1 *= k;
+  c2 *= k;
+  double ea = exp(-c1 * dt / 2);
+  double J2 = Q[14] * Q[14] + Q[15] * Q[15] + Q[16] * Q[16];
+  double den = 1 - c2 / c1 * (1 - ea * ea) * J2 / (r * r);
+  for (int i = 14; i < 17; i++)
+    Q[i] *= ea / sqrt(den);
+}
+
+void orc_analytic_solver(double *Q, long count, double dt, const gpr_par *p, int part,
+                         const orc_cfg *c) {
+  int V = c->thermal ? 17 : 14;
+  for (long i = 0; i < count; i++) {
+    if (part == 0)
+      analytic_distortion(Q + i * V, dt, p);
+    else if (c->thermal)
+      analytic_thermal(Q + i * V, dt, p);
+  }
+}
+
+/* ode.cpp:5-11 with ode_stepper_analytic (analytic.cpp:154-162); the
+ * numerical (reaction) part does nothing for REACTION = -1 */
+void orc_ode_launcher(double *u, long ncell, double dt, const gpr_par *p, const orc_cfg *c) {
+  int V = c->thermal ? 17 : 14;
+  for (long i = 0; i < ncell; i++) {
+    double *Q = u + i * V;
+    for (int k = 0; k < 3; k++)
+      Q[2 + k] += p->dp[k] * dt;
+    analytic_distortion(Q, dt, p);
+    if (c->thermal)
+      analytic_thermal(Q, dt, p);
+  }
+}
+
+/* homogeneous.cpp:5-67 */
+void orc_midstepper(double *wh, long ncell, double dt, const double *dX, const gpr_par *p,
+                    const uint8_t *mask, const orc_cfg *c) {
+  const consts_t *K = get_consts(c->N);
+  int N = c->N, V = c->thermal ? 17 : 14, th = c->thermal;
+  int ns = c->ndim == 1 ? N : N * N;
+  double F[MAXN * MAXN][MAXV], G[MAXN * MAXN][MAXV];
+  double dwx[MAXN * MAXN][MAXV], dwy[MAXN * MAXN][MAXV], dFx[MAXN * MAXN][MAXV], dGy[MAXN * MAXN][MAXV];
+  for (long ind = 0; ind < ncell; ind++) {
+    if (!mask[ind])
+      continue;
+    double *w = wh + ind * ns * V;
+    for (int sN = 0; sN < ns; sN++) {
+      memset(F[sN], 0, sizeof F[sN]);
+      memset(G[sN], 0, sizeof G[sN]);
+      orc_flux(F[sN], w + sN * V, 0, p, th);
+      if (c->ndim == 2)
+        orc_flux(G[sN], w + sN * V, 1, p, th);
+    }
+    for (int a = 0; a < (c->ndim == 1 ? N : N); a++)
+      for (int b = 0; b < (c->ndim == 1 ? 1 : N); b++) {
+        int sN = c->ndim == 1 ? a : a * N + b;
+        for (int v = 0; v < V; v++) {
+          double sx = 0, sfx = 0, sy = 0, sgy = 0;
+          for (int k = 0; k < N; k++) {
+            int nx_ = c->ndim == 1 ? k : k * N + b;
+            sx += K->der[a][k] * w[nx_ * V + v];
+            sfx += K->der[a][k] * F[nx_][v];
+            if (c->ndim == 2) {
+              int ny_ = a * N + k;
+              sy += K->der[b][k] * w[ny_ * V + v];
+              sgy += K->der[b][k] * G[ny_][v];
+            }
+          }
+          dwx[sN][v] = sx;
+          dFx[sN][v] = sfx;
+          dwy[sN][v] = sy;
+          dGy[sN][v] = sgy;
+        }
+      }
+    for (int sN = 0; sN < ns; sN++) {
+      double bx[MAXV], by[MAXV];
+      orc_bdot(bx, w + sN * V, dwx[sN], 0, th);
+      if (c->ndim == 2)
+        orc_bdot(by, w + sN * V, dwy[sN], 1, th);
+      for (int v = 0; v < V; v++) {
+        double inc = (dFx[sN][v] + bx[v]) / dX[0];
+        if (c->ndim == 2)
+          inc += (dGy[sN][v] + by[v]) / dX[1];
+        w[sN * V + v] -= dt / 2 * inc;
+      }
+    }
+  }
+}
+
+/* One split step inside the time loop (iterator.cpp:110-165 with SPLIT,
+ * steppers.cpp:50-66) */
+int orc_split_step(double *u, const int32_t *nX, const double *dX, double CFL,
+                   const int32_t *bcs, double *t, long *count, double tf,
+                   double contorted_tol, int half_step, const gpr_par *p, const orc_cfg *c,
+                   double *dt_out) {
+  int N = c->N, V = c->thermal ? 17 : 14;
+  long ncell = c->ndim == 1 ? nX[0] : (long)nX[0] * nX[1];
+  long next = c->ndim == 1 ? nX[0] + 2 : (long)(nX[0] + 2) * (nX[1] + 2);
+  long nb = c->ndim == 1 ? nX[0] + 2 * N : (long)(nX[0] + 2 * N) * (nX[1] + 2 * N);
+  int ns = c->ndim == 1 ? N : N * N;
+  double dt = orc_timestep(u, nX, dX, CFL, *t, tf, *count, p, c);
+  double *ub = malloc(sizeof(double) * nb * V);
+  double *wh = malloc(sizeof(double) * next * ns * V);
+  uint8_t *m = malloc(ncell), *mb = malloc(next);
+  memset(m, 1, ncell);
+  orc_extend_mask(m, mb, nX, c);
+  orc_boundaries(u, ub, nX, bcs, c);
+  orc_ode_launcher(u, ncell, dt / 2, p, c);
+  orc_ode_launcher(ub, nb, dt / 2, p, c);
+  orc_weno_launcher(wh, ub, nX, c);
+  if (half_step)
+    orc_midstepper(wh, next, dt, dX, p, mb, c);
+  orc_fv_launcher_ex(u, wh, nX, dt, dX, p, mb, 0, 0, c);
+  orc_ode_launcher(u, ncell, dt / 2, p, c);
+  *t += dt;
+  *count += 1;
+  orc_renorm_distortion(u, ncell, p, V);
+  if (orc_contorted(u, ncell, contorted_tol, V))
+    orc_reset_distortion(u, ncell, p, V);
+  free(ub);
+  free(wh);
+  free(m);
+  free(mb);
+  if (dt_out)
+    *dt_out = dt;
+  for (long i = 0; i < ncell * V; i++)
+    if (u[i] != u[i])
+      return -1;
+  return 0;
 }

@@ -106,6 +106,24 @@ int orc_step(double *u, const int32_t *nX, const double *dX, double CFL,
              double contorted_tol, const gpr_par *p, const orc_cfg *c,
              double *dt_out, orc_stats *st);
 
+/* fv_launcher with its SOURCES and TIME flags (TIME = 0: one time level per cell) */
+void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double dt,
+                        const double *dX, const gpr_par *p, const uint8_t *mask,
+                        int sources, int time, const orc_cfg *c);
+
+/* split-WENO step (steppers.cpp:50-94, solvers/split, relaxation/analytic.cpp) */
+void orc_ode_launcher(double *u, long ncell, double dt, const gpr_par *p, const orc_cfg *c);
+void orc_analytic_solver(double *Q, long count, double dt, const gpr_par *p, int part,
+                         const orc_cfg *c);
+void orc_midstepper(double *wh, long ncell, double dt, const double *dX, const gpr_par *p,
+                    const uint8_t *mask, const orc_cfg *c);
+/* One split step of the time loop; half_step runs the midstepper.  Returns 0,
+ * or -1 on a non-finite u. */
+int orc_split_step(double *u, const int32_t *nX, const double *dX, double CFL,
+                   const int32_t *bcs, double *t, long *count, double tf,
+                   double contorted_tol, int half_step, const gpr_par *p, const orc_cfg *c,
+                   double *dt_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -236,6 +236,53 @@ class Oracle:
                                     ctypes.byref(MP), self.V)
         return u
 
+    def fv_launcher_ex(self, u, rec, nX, dt, dX, MP, mask, sources, time):
+        nX = np.asarray(nX, dtype=np.int32)
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        dX = np.asarray(dX, dtype=np.float64)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.L.orc_fv_launcher_ex(_dp(u), _dp(rec), _ip(nX), ctypes.c_double(dt), _dp(dX),
+                                  ctypes.byref(MP), _u8(m), ctypes.c_int(int(sources)),
+                                  ctypes.c_int(int(time)), ctypes.byref(self.cfg))
+        return u
+
+    def ode_launcher(self, u, dt, MP):
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        self.L.orc_ode_launcher(_dp(u), ctypes.c_long(u.size // self.V), ctypes.c_double(dt),
+                                ctypes.byref(MP), ctypes.byref(self.cfg))
+        return u
+
+    def analytic_solver(self, Q, dt, MP, part):
+        Q = np.array(Q, dtype=np.float64, copy=True).ravel()
+        self.L.orc_analytic_solver(_dp(Q), ctypes.c_long(Q.size // self.V), ctypes.c_double(dt),
+                                   ctypes.byref(MP), ctypes.c_int(part), ctypes.byref(self.cfg))
+        return Q
+
+    def midstepper(self, wh, dt, dX, MP, mask):
+        wh = np.array(wh, dtype=np.float64, copy=True).ravel()
+        dX = np.asarray(dX, dtype=np.float64)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.L.orc_midstepper(_dp(wh), ctypes.c_long(m.size), ctypes.c_double(dt), _dp(dX),
+                              ctypes.byref(MP), _u8(m), ctypes.byref(self.cfg))
+        return wh
+
+    def split_step(self, u, nX, dX, CFL, bcs, t, count, tf, MP, contorted_tol=1.0,
+                   half_step=False):
+        """One split-WENO step on a copy of u; returns (u, t, count, dt, rc)."""
+        nX = np.asarray(nX, dtype=np.int32)
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        dX = np.asarray(dX, dtype=np.float64)
+        b = np.asarray(bcs, dtype=np.int32)
+        tt = ctypes.c_double(t)
+        cc = ctypes.c_long(count)
+        dt = ctypes.c_double(0)
+        rc = self.L.orc_split_step(_dp(u), _ip(nX), _dp(dX), ctypes.c_double(CFL), _ip(b),
+                                   ctypes.byref(tt), ctypes.byref(cc), ctypes.c_double(tf),
+                                   ctypes.c_double(contorted_tol), ctypes.c_int(int(half_step)),
+                                   ctypes.byref(MP), ctypes.byref(self.cfg), ctypes.byref(dt))
+        return u, tt.value, cc.value, dt.value, rc
+
     def step(self, u, nX, dX, CFL, bcs, t, count, tf, MP, contorted_tol=1.0):
         """One full step on a copy of u; returns (u, t, count, dt, rc, stats)."""
         nX = np.asarray(nX, dtype=np.int32)

@@ -21,12 +21,15 @@
 #include "solvers/fv/fluxes.h"
 #include "solvers/fv/fv.h"
 #include "solvers/iterator.h"
+#include "solvers/split/homogeneous.h"
+#include "solvers/split/ode.h"
 #include "solvers/steppers.h"
 #include "solvers/utils.h"
 #include "solvers/weno/weno.h"
 #include "system/eig.h"
 #include "system/equations.h"
 #include "system/objects.h"
+#include "system/relaxation/analytic.h"
 #include "system/variables/state.h"
 
 #include "../include/gpr_par.h"
@@ -294,6 +297,78 @@ void ref_fv_launcher(double *u, const double *qh, const int32_t *nX, int ndim,
   fv_launcher(um, qv, n, dt, d, true, true, 0, P, m);
 }
 
+// fv_launcher with the SOURCES and TIME flags as given (TIME = 0: rec holds
+// one time level per cell, as the split step's wh)
+void ref_fv_launcher_ex(double *u, const double *rec, const int32_t *nX, int ndim,
+                        double dt, const double *dX, const gpr_par *p,
+                        const uint8_t *mask, int sources, int time) {
+  Par P = to_par(p);
+  long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
+  long nq = ne * ipow(N, ndim + (time ? 1 : 0));
+  VMap um(u, nc * V);
+  Vec qv = Eigen::Map<const Vec>(rec, nq * V);
+  Vec d = make_dX(dX, ndim);
+  iVec n = make_nX(nX, ndim);
+  bVec m(ne);
+  for (long i = 0; i < ne; i++)
+    m(i) = mask[i] != 0;
+  fv_launcher(um, qv, n, dt, d, sources != 0, time != 0, 0, P, m);
+}
+
+// solvers/split/ode.cpp:5-11 on ncell states
+void ref_ode_launcher(double *u, long ncell, double dt, const gpr_par *p) {
+  Par P = to_par(p);
+  VMap um(u, ncell * V);
+  ode_launcher(um, dt, P);
+}
+
+// analyticSolver_distortion (part 0) or analyticSolver_thermal (part 1) on
+// count states, one at a time
+void ref_analytic_solver(double *Q, long count, double dt, const gpr_par *p,
+                         int part) {
+  Par P = to_par(p);
+  for (long i = 0; i < count; i++) {
+    VMap q(Q + i * V, V);
+    if (part == 0)
+      analyticSolver_distortion(q, dt, P);
+    else
+      analyticSolver_thermal(q, dt, P);
+  }
+}
+
+// solvers/split/homogeneous.cpp:5-67; ncell = the mask's size
+void ref_midstepper(double *wh, long ncell, double dt, const double *dX,
+                    int ndim, const gpr_par *p, const uint8_t *mask) {
+  Par P = to_par(p);
+  VMap w(wh, ncell * ipow(N, ndim) * V);
+  Vec d = make_dX(dX, ndim);
+  bVec m(ncell);
+  for (long i = 0; i < ncell; i++)
+    m(i) = mask[i] != 0;
+  midstepper(w, dt, d, P, m);
+}
+
+// Whole split step on the x-slab OpenMP split (steppers.cpp:68-94), from u
+// with its boundaries and extended mask built as the iterator builds them.
+void ref_split_stepper_para(double *u, const int32_t *nX, int ndim, double dt,
+                            const double *dX, const int32_t *bcs,
+                            int half_step, const gpr_par *p) {
+  Par P = to_par(p);
+  long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
+  long nb = prod_ext(nX, ndim, N);
+  VMap um(u, nc * V);
+  Vec ub(nb * V);
+  iVec n = make_nX(nX, ndim);
+  iVec b(2 * ndim);
+  for (int i = 0; i < 2 * ndim; i++)
+    b(i) = bcs[i];
+  boundaries(um, ub, n, b);
+  bVec m = bVec::Ones(nc), mb(ne);
+  extend_mask(m, mb, n);
+  Vec d = make_dX(dX, ndim);
+  split_stepper_para(um, ub, n, dt, d, half_step != 0, 0, P, mb);
+}
+
 double ref_timestep(const double *u, const int32_t *nX, int ndim,
                     const double *dX, double CFL, double t, double tf,
                     int count, const gpr_par *p) {
@@ -345,7 +420,7 @@ void ref_ader_stepper_para(double *u, const int32_t *nX, int ndim, double dt,
 }
 
 // Runs up to `nsteps` steps of the reference time loop (iterator.cpp:104-166,
-// LSET = 0, SPLIT = false, FLUX = 0, STIFF as given) from (t, count), writing
+// LSET = 0, FLUX = 0; STIFF, SPLIT and HALF_STEP as given) from (t, count), writing
 // each step's dt into dts[] and its wall time into secs[] (may be null).  The
 // extended mask is built once: it is constant for a single all-true material
 // mask, and its 2D builder is O(nx ny (nx + ny)) (grid.cpp:213-226).
@@ -353,7 +428,8 @@ void ref_ader_stepper_para(double *u, const int32_t *nX, int ndim, double dt,
 long ref_run(double *u, const int32_t *nX, int ndim, const double *dX,
              double CFL, const int32_t *bcs, double *t_io, long *count_io,
              double tf, long nsteps, double contorted_tol, int stiff,
-             const gpr_par *p, double *dts, double *secs) {
+             int split, int half_step, const gpr_par *p, double *dts,
+             double *secs) {
   Par P = to_par(p);
   std::vector<Par> MPs(1, P);
   long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
@@ -381,7 +457,10 @@ long ref_run(double *u, const int32_t *nX, int ndim, const double *dX,
     grids[0] = um;
     double dt = timestep(grids, masks, da, CFL, t, tf, count, MPs, 1);
     boundaries(grids[0], ub, n, b);
-    ader_stepper_para(grids[0], ub, n, dt, dv, stiff != 0, 0, MPs[0], mb);
+    if (split)
+      split_stepper_para(grids[0], ub, n, dt, dv, half_step != 0, 0, MPs[0], mb);
+    else
+      ader_stepper_para(grids[0], ub, n, dt, dv, stiff != 0, 0, MPs[0], mb);
     um = grids[0];
     t += dt;
     count += 1;

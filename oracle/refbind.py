@@ -182,6 +182,47 @@ class RefSolver:
                                  ctypes.c_double(dt), _dp(dX), ctypes.byref(MP), _u8(m))
         return u
 
+    def fv_launcher_ex(self, u, rec, nX, dt, dX, MP, mask, sources, time):
+        nX = np.asarray(nX, dtype=np.int32)
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        dX = np.asarray(dX, dtype=np.float64)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.lib.ref_fv_launcher_ex(_dp(u), _dp(rec), _ip(nX), ctypes.c_int(len(nX)),
+                                    ctypes.c_double(dt), _dp(dX), ctypes.byref(MP), _u8(m),
+                                    ctypes.c_int(int(sources)), ctypes.c_int(int(time)))
+        return u
+
+    def ode_launcher(self, u, dt, MP):
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        self.lib.ref_ode_launcher(_dp(u), ctypes.c_long(u.size // self.V), ctypes.c_double(dt),
+                                  ctypes.byref(MP))
+        return u
+
+    def analytic_solver(self, Q, dt, MP, part):
+        Q = np.array(Q, dtype=np.float64, copy=True).ravel()
+        self.lib.ref_analytic_solver(_dp(Q), ctypes.c_long(Q.size // self.V),
+                                     ctypes.c_double(dt), ctypes.byref(MP), ctypes.c_int(part))
+        return Q
+
+    def midstepper(self, wh, dt, dX, MP, mask):
+        wh = np.array(wh, dtype=np.float64, copy=True).ravel()
+        dX = np.asarray(dX, dtype=np.float64)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.lib.ref_midstepper(_dp(wh), ctypes.c_long(m.size), ctypes.c_double(dt), _dp(dX),
+                                ctypes.c_int(len(dX)), ctypes.byref(MP), _u8(m))
+        return wh
+
+    def split_stepper_para(self, u, nX, dt, dX, bcs, MP, half_step=False):
+        nX = np.asarray(nX, dtype=np.int32)
+        u = np.array(u, dtype=np.float64, copy=True)
+        dX = np.asarray(dX, dtype=np.float64)
+        b = np.asarray(bcs, dtype=np.int32)
+        self.lib.ref_split_stepper_para(_dp(u), _ip(nX), ctypes.c_int(len(nX)),
+                                        ctypes.c_double(dt), _dp(dX), _ip(b),
+                                        ctypes.c_int(int(half_step)), ctypes.byref(MP))
+        return u
+
     def timestep(self, u, nX, dX, CFL, t, tf, count, MP):
         nX = np.asarray(nX, dtype=np.int32)
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -219,7 +260,7 @@ class RefSolver:
         return u
 
     def run(self, u, nX, dX, CFL, bcs, tf, nsteps, MP, t=0.0, count=0,
-            contorted_tol=1.0, stiff=False):
+            contorted_tol=1.0, stiff=False, split=False, half_step=False):
         """Advance a copy of u by up to nsteps steps of the reference loop.
 
         Returns (u, t, count, dts, secs, status) with status -1 on NaN."""
@@ -235,14 +276,16 @@ class RefSolver:
                              ctypes.c_double(CFL), _ip(b), ctypes.byref(tt),
                              ctypes.byref(cc), ctypes.c_double(tf),
                              ctypes.c_long(nsteps), ctypes.c_double(contorted_tol),
-                             ctypes.c_int(int(stiff)), ctypes.byref(MP), _dp(dts), _dp(secs))
+                             ctypes.c_int(int(stiff)), ctypes.c_int(int(split)),
+                             ctypes.c_int(int(half_step)), ctypes.byref(MP), _dp(dts), _dp(secs))
         n = k if k >= 0 else int(np.count_nonzero(secs))
         return u, tt.value, cc.value, dts[:n], secs[:n], (0 if k >= 0 else -1)
 
 
 def _time_run_main(argv):
     """python oracle/refbind.py STATE.npz: time the reference's step loop on
-    the state in STATE.npz (u, nX, dX, bcs, t, count, nsteps, par as JSON) and
+    the state in STATE.npz (u, nX, dX, bcs, t, count, nsteps, par as JSON;
+    split and half_step select the split step) and
     print one JSON line {"secs": [...], "status": s}.  Run as a child process
     so that an abort inside the reference (its Eigen assertions are live)
     cannot take the caller down."""
@@ -256,9 +299,12 @@ def _time_run_main(argv):
         setattr(MP, k, v)
     nX = [int(x) for x in d["nX"]]
     R = RefSolver(3, int(d["V"]) == 17)
+    split = int(d["split"]) if "split" in d.files else 0
+    half = int(d["half_step"]) if "half_step" in d.files else 0
     _, _, _, _, secs, status = R.run(d["u"], nX, d["dX"], float(d["cfl"]), d["bcs"], 1e300,
                                      int(d["nsteps"]), MP, t=float(d["t"]),
-                                     count=int(d["count"]))
+                                     count=int(d["count"]), split=bool(split),
+                                     half_step=bool(half))
     print(json.dumps({"secs": [float(s) for s in secs], "status": int(status)}))
 
 

@@ -33,6 +33,7 @@ SYMBOLS = [
     "gpr_batch_get_state", "gpr_batch_set_block", "gpr_batch_steps", "gpr_batch_run",
     "gpr_batch_status", "gpr_dev_steps_dt", "gpr_adj_forward", "gpr_adj_backward",
     "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats", "gpr_adj_grad_kappa",
+    "gpr_split_stepper", "gpr_ode_launcher", "gpr_analytic_solver", "gpr_midstepper",
 ]
 
 
@@ -46,7 +47,8 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("dg_it", ctypes.c_int32), ("dg_tol", ctypes.c_double),
                 ("stiff_it", ctypes.c_int32), ("stiff_tol", ctypes.c_double),
                 ("bailout", ctypes.c_int32), ("mode", ctypes.c_int32),
-                ("contorted_tol", ctypes.c_double), ("precision", ctypes.c_int32)]
+                ("contorted_tol", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("split", ctypes.c_int32), ("half_step", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p
@@ -82,6 +84,10 @@ ARGTYPES = {
     "gpr_fv_launcher": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _i, _M, _Up],
     "gpr_timestep": [_P, _Dp, _Dp, _Ip, _Dp, _d, _d, _d, _l, _M],
     "gpr_ader_stepper": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
+    "gpr_split_stepper": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
+    "gpr_ode_launcher": [_P, _Dp, _l, _d, _M],
+    "gpr_analytic_solver": [_P, _Dp, _l, _d, _M, _i],
+    "gpr_midstepper": [_P, _Dp, _d, _Dp, _l, _M, _Up],
     "gpr_renorm_distortion": [_P, _Dp, _l, _M],
     "gpr_contorted": [_P, _Ip, _Dp, _l, _d],
     "gpr_reset_distortion": [_P, _Dp, _l, _M],

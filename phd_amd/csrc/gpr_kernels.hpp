@@ -16,7 +16,7 @@
 
 #include <type_traits>
 
-#include "gpr_predictor.hpp"
+#include "gpr_relax.hpp"
 
 namespace gpr {
 
@@ -327,30 +327,36 @@ struct FvArgs {
   double dt, dx, dy;
   int nx, ny;
   int post;              // fuse renormalisation, contortion flag and CFL maximum
+  int time;              // 1: qh holds N time nodes (ADER); 0: one, weight dt (split step's wh)
+  int sources;           // 1: the cell integral includes S(q)
+  int ode;               // with post: the split step's closing half-step ODE first
   Mat m;
 };
 
 // Faces: interfs1/interfs2 (fv.cpp:111-160, 162-268).  One lane per (face,
 // time node t, face node s): it extrapolates both cells' states to the face
 // node, evaluates one Rusanov flux and one path integral, and weights them;
-// the NT = N * NSF lanes of a face are then summed through LDS.  Face d = 0
+// the NT = NTT * NSF lanes of a face are then summed through LDS.  Face d = 0
 // joins extended cells (i, j+1) and (i+1, j+1) for i in [0, nx], j in
 // [0, ny); face d = 1 joins (i+1, j) and (i+1, j+1) for i in [0, nx),
-// j in [0, ny].
-template <int N, bool TH, int ND> struct FaceShape {
+// j in [0, ny].  TIME = false is fv_launcher's TIME = false: the input holds
+// one time level per cell (the split step's WENO reconstruction) and the
+// time weight is dt (fv.cpp:156-159, 273-279).
+template <int N, bool TH, int ND, bool TIME = true> struct FaceShape {
+  static constexpr int NTT = TIME ? N : 1;     // time nodes
   static constexpr int NSF = ND == 2 ? N : 1;  // face nodes
-  static constexpr int NT = N * NSF;           // lanes per face
+  static constexpr int NT = NTT * NSF;         // lanes per face
   static constexpr int FPW = WAVE / NT;        // faces per wavefront
 };
 
-template <int N, bool TH, int ND>
+template <int N, bool TH, int ND, bool TIME = true>
 __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
-  constexpr int NN = NS * N;
-  constexpr int NSF = FaceShape<N, TH, ND>::NSF;
-  constexpr int NT = FaceShape<N, TH, ND>::NT;
-  constexpr int FPW = FaceShape<N, TH, ND>::FPW;
+  constexpr int NN = NS * FaceShape<N, TH, ND, TIME>::NTT;
+  constexpr int NSF = FaceShape<N, TH, ND, TIME>::NSF;
+  constexpr int NT = FaceShape<N, TH, ND, TIME>::NT;
+  constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
   __shared__ double sR[FPW * NT * 2 * V];
   const Basis &K = basis_dev<N>();
   const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
@@ -385,7 +391,7 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   }
   if (ok && a.mask[e0] && a.mask[e1]) {
     const double h = d == 0 ? a.dx : a.dy;
-    const double kt = dt * K.w[t] / (2.0 * h);
+    const double kt = (TIME ? dt * K.w[t] : dt) / (2.0 * h);
     const double kk = ND == 2 ? kt * K.w[sf] : kt;
     double qL[V], qR[V];
     const double *pl = a.qh + (e0 * NN + t * NS) * V;
@@ -434,13 +440,20 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
 // Cell centres (fv.cpp:11-109) with the face terms: one lane per space-time
 // node of a real cell, as in the predictor.  Each lane forms
 // dt W_t W_a W_b (S(q) - B_x dq/dx - B_y dq/dy) at its node; the cell's
-// lanes are summed through LDS and the face jumps subtracted.
-template <int N, bool TH, int ND>
+// lanes are summed through LDS and the face jumps subtracted.  TIME = false:
+// one time level, weight dt; SRC = false: no S(q) (fv_launcher's SOURCES).
+template <int N, int ND, bool TIME> struct CellShape {
+  static constexpr int NS = ND == 1 ? N : N * N;  // spatial nodes per cell
+  static constexpr int NN = NS * (TIME ? N : 1);  // nodes per cell
+  static constexpr int CPW = WAVE / NN;           // cells per wavefront
+};
+
+template <int N, bool TH, int ND, bool TIME = true, bool SRC = true>
 __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   constexpr int V = Dims<TH>::V;
-  constexpr int NS = PredShape<N, ND>::NS;
-  constexpr int NN = PredShape<N, ND>::NN;
-  constexpr int CPW = PredShape<N, ND>::CPW;
+  constexpr int NS = CellShape<N, ND, TIME>::NS;
+  constexpr int NN = CellShape<N, ND, TIME>::NN;
+  constexpr int CPW = CellShape<N, ND, TIME>::CPW;
   __shared__ double sQ[CPW * NN * V];
   const Basis &K = basis_dev<N>();
   const int lane = threadIdx.x;
@@ -473,7 +486,12 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   {
     NodeState<double, TH> st;
     node_state<double, TH>(q, a.m, st);
-    source<double, TH>(q, st, a.m, S);
+    if (SRC) {
+      source<double, TH>(q, st, a.m, S);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; v++) S[v] = 0.0;
+    }
     double dq[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
@@ -493,7 +511,8 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
       }
       bdot_sub(st.v, dq, 1, 1.0, S);
     }
-    const double wgt = ND == 2 ? dt * K.w[t] * K.w[ia] * K.w[ib] : dt * K.w[t] * K.w[ia];
+    const double wt = TIME ? dt * K.w[t] : dt;
+    const double wgt = ND == 2 ? wt * K.w[ia] * K.w[ib] : wt * K.w[ia];
 #pragma unroll
     for (int v = 0; v < V; v++) S[v] = active ? wgt * S[v] : 0.0;
   }
@@ -521,15 +540,19 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
 // Post-step work on the updated u, one thread per cell: renorm_distortion
 // (utils.cpp:6-23), the contortion flag (utils.cpp:65-85), a non-finite flag
 // (iterator.cpp:161) and the CFL maximum for the next dt (iterator.cpp:58-72).
-template <bool TH, int ND>
+// ODE: the split step's closing half-step source ODE first
+// (steppers.cpp:65), which the iterator's post-step then follows.
+template <bool TH, int ND, bool ODE = false>
 __global__ __launch_bounds__(256) void k_post(double *u, long ncell, double dx, double dy, Mat m, StepState *st) {
   constexpr int V = Dims<TH>::V;
   double local_max = 0.0;
   int contorted = 0, nonfinite = 0;
+  const double hdt = ODE ? st->dt / 2 : 0.0;
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
     double q[V];
 #pragma unroll
     for (int v = 0; v < V; v++) q[v] = u[c * V + v];
+    if (ODE) ode_analytic<TH>(q, hdt, m.p);
     if (q[0] > 0) {
       double cf = cbrt(q[0] / (m.p.rho0 * det3(q + 5)));
 #pragma unroll
@@ -547,8 +570,13 @@ __global__ __launch_bounds__(256) void k_post(double *u, long ncell, double dx, 
 #pragma unroll
     for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
     if (sp > local_max) local_max = sp;
+    if (ODE) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[c * V + 5 + k] = q[5 + k];
+      for (int v = 0; v < V; v++) u[c * V + v] = q[v];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; k++) u[c * V + 5 + k] = q[5 + k];
+    }
   }
   double mx = wave_max(local_max);
   int cf = __any(contorted), nf = __any(nonfinite);

@@ -341,17 +341,57 @@ template <int N, bool TH, int ND> struct PredictorSplitOp {
   static int run(gpr_ctx *c, PredArgs a) { return predictor_split<N, TH, ND, double, double>(c, a); }
 };
 
+// fv_launcher's faces and cells for one (TIME, SOURCES) pair
+template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_launch(gpr_ctx *c, const FvArgs &a) {
+  long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
+  constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
+  hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
+  long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
+  constexpr int CPW = CellShape<N, ND, TIME>::CPW;
+  hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a);
+}
+
 template <int N, bool TH, int ND> struct FvOp {
   static int run(gpr_ctx *c, FvArgs a) {
-    long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
-    constexpr int FPW = FaceShape<N, TH, ND>::FPW;
-    hipLaunchKernelGGL((k_fv_faces<N, TH, ND>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
+    if (a.time && a.sources)
+      fv_launch<N, TH, ND, true, true>(c, a);
+    else if (!a.time && !a.sources)
+      fv_launch<N, TH, ND, false, false>(c, a);
+    else if (a.time)
+      fv_launch<N, TH, ND, true, false>(c, a);
+    else
+      fv_launch<N, TH, ND, false, true>(c, a);
     long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
-    constexpr int CPW = PredShape<N, ND>::CPW;
-    hipLaunchKernelGGL((k_fv_cells<N, TH, ND>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a);
-    if (a.post)
+    if (a.post && a.ode)
+      hipLaunchKernelGGL((k_post<TH, ND, true>), dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a.u, nc, a.dx,
+                         a.dy, a.m, a.st);
+    else if (a.post)
       hipLaunchKernelGGL((k_post<TH, ND>), dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a.u, nc, a.dx, a.dy,
                          a.m, a.st);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+// ode_launcher (solvers/split/ode.cpp:5-11) over n cells of x; with u, x is
+// the extended grid ub and interior results also go to u (k_ode).
+template <int N, bool TH, int ND> struct OdeOp {
+  static int run(gpr_ctx *c, double *x, long n, double *u, int nx, int ny, const double *dt_dev, double dt,
+                 double scale, const gpr_par &p) {
+    hipLaunchKernelGGL((k_ode<TH, ND>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, n, u, nx, ny, N, dt_dev,
+                       dt, scale, p);
+    HIP_TRY(hipGetLastError());
+    return GPR_OK;
+  }
+};
+
+// midstepper (solvers/split/homogeneous.cpp:5-67) over the ncell masked cells of wh
+template <int N, bool TH, int ND> struct MidstepOp {
+  static int run(gpr_ctx *c, double *wh, long ncell, const uint8_t *mask, const double *dt_dev, double dt,
+                 double dx, double dy, const Mat &m) {
+    constexpr int CPW = MidShape<N, ND>::CPW;
+    hipLaunchKernelGGL((k_midstep<N, TH, ND>), dim3((ncell + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, wh, ncell,
+                       mask, dt_dev, dt, dx, dy, m);
     HIP_TRY(hipGetLastError());
     return GPR_OK;
   }
@@ -377,7 +417,10 @@ template <int N, bool TH, int ND> struct ResetOp {
 };
 
 // pointwise operations over a batch of states
-enum PointOp { OP_FLUX, OP_SOURCE, OP_BDOT, OP_EIGS, OP_TAT, OP_DRUS, OP_BINT, OP_RENORM, OP_RESET, OP_CONTORTED };
+enum PointOp {
+  OP_FLUX, OP_SOURCE, OP_BDOT, OP_EIGS, OP_TAT, OP_DRUS, OP_BINT, OP_RENORM, OP_RESET, OP_CONTORTED,
+  OP_ODE_DISTORTION, OP_ODE_THERMAL  // dt in tol
+};
 
 template <int N, bool TH>
 __global__ void k_pointwise(int op, double *ret, const double *A, const double *B, long count, int d, Mat m,
@@ -446,6 +489,14 @@ __global__ void k_pointwise(int op, double *ret, const double *A, const double *
       for (int k = 0; k < 9; k++) f += q[5 + k] * q[5 + k];
       double d13 = cbrt(fabs(det3(q + 5)));
       if (f / (d13 * d13) / 3.0 - 1.0 > tol) atomicOr(flag, 1);
+    } break;
+    case OP_ODE_DISTORTION:
+    case OP_ODE_THERMAL: {
+      if (op == OP_ODE_DISTORTION)
+        ode_distortion(q + 5, tol, m.p);
+      else if (TH)
+        ode_thermal(q, tol, m.p);
+      for (int v = 0; v < V; v++) ret[i * V + v] = q[v];
     } break;
     }
   }
@@ -563,6 +614,8 @@ FvArgs fv_args(gpr_ctx *c, const Mat &m, double dt, const double *dX, const int3
   a.dy = c->ndim == 2 ? dX[1] : 1.0;
   a.nx = nX[0];
   a.ny = c->ndim == 2 ? nX[1] : 1;
+  a.time = 1;
+  a.sources = 1;
   a.m = m;
   return a;
 }
@@ -614,6 +667,8 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
   c->opts.mode = 0;
   c->opts.precision = 0;
   c->opts.contorted_tol = 1.0;
+  c->opts.split = 0;
+  c->opts.half_step = 0;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_st, sizeof(StepState));
@@ -653,7 +708,8 @@ int gpr_ctx_destroy(gpr_ctx *c) {
 int gpr_ctx_set_opts(gpr_ctx *c, const gpr_solver_opts *o) {
   if (!c || !o) return fail(GPR_E_ARG, "null argument");
   if (o->dg_it < 1 || o->stiff_it < 1 || !(o->dg_tol > 0) || !(o->stiff_tol > 0) || o->bailout < 0 ||
-      o->mode < 0 || o->mode > 2 || o->precision < 0 || o->precision > 1)
+      o->mode < 0 || o->mode > 2 || o->precision < 0 || o->precision > 1 || o->split < 0 || o->split > 1 ||
+      o->half_step < 0 || o->half_step > 1)
     return fail(GPR_E_ARG, "invalid solver options");
   c->opts = *o;
   return GPR_OK;
@@ -841,7 +897,6 @@ int gpr_predictor(gpr_ctx *c, double *qh, const double *wh, double dt, const dou
 int gpr_fv_launcher(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, double dt, const double *dX,
                     int sources, int time, int flux, const gpr_par *MP, const uint8_t *mask) {
   if (!c || !u || !qh || !dX || !mask) return fail(GPR_E_ARG, "null argument");
-  if (!sources || !time) return fail(GPR_E_UNSUPPORTED, "only SOURCES = TIME = true (the ADER step) is supported");
   if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
   int rc = check_nX(c, nX);
   if (rc) return rc;
@@ -850,7 +905,7 @@ int gpr_fv_launcher(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, 
   if (rc) return rc;
   long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1);
   int NS = c->ndim == 2 ? c->N * c->N : c->N;
-  long nq = ne * NS * c->N * c->V;
+  long nq = ne * NS * (time ? c->N : 1) * c->V;
   double *du = (double *)buf(c, "u", nc * c->V * 8, rc);
   if (!du) return rc;
   double *dqh = (double *)buf(c, "qh", nq * 8, rc);
@@ -864,6 +919,8 @@ int gpr_fv_launcher(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, 
   a.u = du;
   a.qh = dqh;
   a.mask = dm;
+  a.time = time != 0;
+  a.sources = sources != 0;
   rc = fv_buffers(c, a);
   if (rc) return rc;
   rc = dispatch<FvOp>(c, c, a);
@@ -956,6 +1013,106 @@ int gpr_ader_stepper(gpr_ctx *c, double *u, const double *ub, const int32_t *nX,
   return GPR_OK;
 }
 
+int gpr_split_stepper(gpr_ctx *c, double *u, double *ub, const int32_t *nX, double dt, const double *dX,
+                      int half_step, int flux, const gpr_par *MP, const uint8_t *mask) {
+  if (!c || !u || !ub || !dX || !mask) return fail(GPR_E_ARG, "null argument");
+  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  Mat m;
+  rc = make_mat(MP, m);
+  if (rc) return rc;
+  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1), nb = ext_cells(c, nX, c->N);
+  int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  long nw = ne * NS * c->V;
+  double *du = (double *)buf(c, "u", nc * c->V * 8, rc);
+  if (!du) return rc;
+  double *dub = (double *)buf(c, "ub", nb * c->V * 8, rc);
+  if (!dub) return rc;
+  double *dwh = (double *)buf(c, "wh", nw * 8, rc);
+  if (!dwh) return rc;
+  uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
+  if (!dm) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, nc * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dub, ub, nb * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dm, mask, ne, hipMemcpyHostToDevice, c->stream));
+  int ny = c->ndim == 2 ? nX[1] : 1;
+  // u and ub are the caller's and need not agree: two ODE launches
+  rc = dispatch<OdeOp>(c, c, du, nc, (double *)nullptr, nX[0], ny, (const double *)nullptr, dt, 0.5, m.p);
+  if (!rc) rc = dispatch<OdeOp>(c, c, dub, nb, (double *)nullptr, nX[0], ny, (const double *)nullptr, dt, 0.5, m.p);
+  if (!rc) rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, nX[0], ny);
+  if (!rc && half_step)
+    rc = dispatch<MidstepOp>(c, c, dwh, ne, (const uint8_t *)dm, (const double *)nullptr, dt, dX[0],
+                             c->ndim == 2 ? dX[1] : 1.0, m);
+  if (rc) return rc;
+  FvArgs fa = fv_args(c, m, dt, dX, nX);
+  fa.u = du;
+  fa.qh = dwh;
+  fa.mask = dm;
+  fa.time = 0;
+  fa.sources = 0;
+  rc = fv_buffers(c, fa);
+  if (!rc) rc = dispatch<FvOp>(c, c, fa);
+  if (!rc) rc = dispatch<OdeOp>(c, c, du, nc, (double *)nullptr, nX[0], ny, (const double *)nullptr, dt, 0.5, m.p);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(u, du, nc * c->V * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(ub, dub, nb * c->V * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+int gpr_ode_launcher(gpr_ctx *c, double *u, int64_t ncell, double dt, const gpr_par *MP) {
+  if (!c || !u) return fail(GPR_E_ARG, "null argument");
+  if (ncell < 0) return fail(GPR_E_ARG, "negative cell count");
+  Mat m;
+  int rc = make_mat(MP, m);
+  if (rc) return rc;
+  if (ncell == 0) return GPR_OK;
+  double *du = (double *)buf(c, "u", ncell * c->V * 8, rc);
+  if (!du) return rc;
+  HIP_TRY(hipMemcpyAsync(du, u, ncell * c->V * 8, hipMemcpyHostToDevice, c->stream));
+  rc = dispatch<OdeOp>(c, c, du, (long)ncell, (double *)nullptr, 1, 1, (const double *)nullptr, dt, 1.0, m.p);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(u, du, ncell * c->V * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
+int gpr_analytic_solver(gpr_ctx *c, double *Q, int64_t count, double dt, const gpr_par *MP, int part) {
+  if (!c || !Q) return fail(GPR_E_ARG, "null argument");
+  if (count < 0) return fail(GPR_E_ARG, "negative count");
+  if (part != 0 && part != 1) return fail(GPR_E_ARG, "part must be 0 (distortion) or 1 (thermal)");
+  if (part == 1 && !c->thermal) return fail(GPR_E_UNSUPPORTED, "the thermal solver needs THERMAL (V = 17)");
+  Mat m;
+  int rc = make_mat(MP, m);
+  if (rc) return rc;
+  return pointwise(c, part == 0 ? OP_ODE_DISTORTION : OP_ODE_THERMAL, Q, Q, nullptr, count, 0, MP, c->V, dt);
+}
+
+int gpr_midstepper(gpr_ctx *c, double *wh, double dt, const double *dX, int64_t ncell, const gpr_par *MP,
+                   const uint8_t *mask) {
+  if (!c || !wh || !dX || !mask) return fail(GPR_E_ARG, "null argument");
+  if (ncell < 0) return fail(GPR_E_ARG, "negative cell count");
+  Mat m;
+  int rc = make_mat(MP, m);
+  if (rc) return rc;
+  if (ncell == 0) return GPR_OK;
+  int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  long nw = ncell * NS * c->V;
+  double *dwh = (double *)buf(c, "wh", nw * 8, rc);
+  if (!dwh) return rc;
+  uint8_t *dm = (uint8_t *)buf(c, "mask", ncell, rc);
+  if (!dm) return rc;
+  HIP_TRY(hipMemcpyAsync(dwh, wh, nw * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dm, mask, ncell, hipMemcpyHostToDevice, c->stream));
+  rc = dispatch<MidstepOp>(c, c, dwh, (long)ncell, (const uint8_t *)dm, (const double *)nullptr, dt, dX[0],
+                           c->ndim == 2 ? dX[1] : 1.0, m);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(wh, dwh, nw * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GPR_OK;
+}
+
 // ------------------------------------------------------------ device memory
 int gpr_dev_malloc(gpr_ctx *c, void **d, int64_t bytes) {
   if (!c || !d || bytes <= 0) return fail(GPR_E_ARG, "gpr_dev_malloc: bad argument");
@@ -993,6 +1150,8 @@ int gpr_dev_begin(gpr_ctx *c, double *d_u, const int32_t *nX, const double *dX, 
   Mat m;
   rc = make_mat(MP, m);
   if (rc) return rc;
+  if (c->opts.split && c->opts.precision != 0)
+    return fail(GPR_E_UNSUPPORTED, "the split-WENO step runs in FP64 only");
   c->run_ready = false;
   c->run_u = d_u;
   c->run_nx = nX[0];
@@ -1005,7 +1164,7 @@ int gpr_dev_begin(gpr_ctx *c, double *d_u, const int32_t *nX, const double *dX, 
   int NS = c->ndim == 2 ? c->N * c->N : c->N;
   if (!buf(c, "ub", nb * c->V * 8, rc)) return rc;
   if (!buf(c, "wh", ne * NS * c->V * 8, rc)) return rc;
-  if (!buf(c, "qh", ne * NS * c->N * c->V * 8, rc)) return rc;
+  if (!c->opts.split && !buf(c, "qh", ne * NS * c->N * c->V * 8, rc)) return rc;
   uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
   if (!dm) return rc;
   if (c->ndim == 2 && !buf(c, "ux", (long)(nX[0] + 2) * (nX[1] + 2 * c->N) * c->N * c->V * 8, rc)) return rc;
@@ -1048,11 +1207,14 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
   int rc;
   int nX[2] = {c->run_nx, c->run_ny};
   double dX[2] = {c->run_dx, c->run_dy};
-  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1);
+  long nc = ext_cells(c, nX, 0), ne = ext_cells(c, nX, 1), nb = ext_cells(c, nX, c->N);
+  const bool split = c->opts.split != 0;
+  if (split && c->opts.precision != 0) return fail(GPR_E_UNSUPPORTED, "the split-WENO step runs in FP64 only");
   double *dub = (double *)c->bufs["ub"].first;
   double *dwh = (double *)c->bufs["wh"].first;
-  double *dqh = (double *)c->bufs["qh"].first;
+  double *dqh = split ? nullptr : (double *)c->bufs["qh"].first;
   uint8_t *dm = (uint8_t *)c->bufs["mask"].first;
+  if (!split && !dqh) return fail(GPR_E_ARG, "gpr_dev_begin ran for the split step; call it again");
   for (int64_t k = 0; k < nsteps; k++) {
     hipEvent_t *ev = nullptr;
     if (c->timing) {
@@ -1068,6 +1230,10 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
     }
     if (dts) hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->stream, c->d_st, dts[k]);
     rc = dispatch<BoundariesOp>(c, c, (const double *)c->run_u, dub, c->run_nx, c->run_ny, c->run_bc);
+    if (rc) return rc;
+    // split step (steppers.cpp:50-66): the half-step ODE on u and ub, one launch
+    if (split) rc = dispatch<OdeOp>(c, c, dub, nb, c->run_u, c->run_nx, c->run_ny, (const double *)&c->d_st->dt, 0.0,
+                                    0.5, c->run_mat.p);
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
     const bool f32 = c->opts.precision == 1;
@@ -1086,6 +1252,34 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
     }
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
+    if (split) {
+      if (c->opts.half_step)
+        rc = dispatch<MidstepOp>(c, c, dwh, ne, (const uint8_t *)dm, (const double *)&c->d_st->dt, 0.0, dX[0], dX[1],
+                                 c->run_mat);
+      if (rc) return rc;
+      if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
+      FvArgs fs = fv_args(c, c->run_mat, 0, dX, nX);
+      fs.st = c->d_st;
+      fs.u = c->run_u;
+      fs.qh = dwh;
+      fs.mask = dm;
+      fs.post = 1;
+      fs.time = 0;
+      fs.sources = 0;
+      fs.ode = 1;
+      rc = fv_buffers(c, fs);
+      if (rc) return rc;
+      rc = dispatch<FvOp>(c, c, fs);
+      if (rc) return rc;
+      if (ev) HIP_TRY(hipEventRecord(ev[4], c->stream));
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_st);
+      rc = dispatch<ResetOp>(c, c, c->run_u, nc, dX[0], dX[1], c->run_mat, c->d_st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(1), 0, c->stream, c->d_st);
+      HIP_TRY(hipGetLastError());
+      if (ev) HIP_TRY(hipEventRecord(ev[5], c->stream));
+      continue;
+    }
     PredArgs pa = pred_args(c, c->run_mat, 0, dX);
     pa.dt_dev = &c->d_st->dt;
     pa.qh = dqh;
@@ -1180,9 +1374,7 @@ int gpr_dev_status(gpr_ctx *c, double *t, int64_t *count, double *dt, int32_t *p
 int gpr_iterator(gpr_ctx *c, double *u, double tf, const int32_t *nX, const double *dX, double CFL,
                  const int32_t *bcs, int split, int half_step, int stiff, int flux, const gpr_par *MPs, int nmat,
                  double contorted_tol, double *snapshots, int32_t *nsnap, int64_t *steps) {
-  (void)half_step;
   if (!c || !u || !dX || !MPs) return fail(GPR_E_ARG, "null argument");
-  if (split) return fail(GPR_E_UNSUPPORTED, "SPLIT = true (the split-WENO stepper) is not on this path");
   if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
   if (nmat != 1) return fail(GPR_E_UNSUPPORTED, "one material (LSET = 0) is supported");
   int rc = check_nX(c, nX);
@@ -1197,10 +1389,14 @@ int gpr_iterator(gpr_ctx *c, double *u, double tf, const int32_t *nX, const doub
   gpr_solver_opts saved = c->opts;
   c->opts.contorted_tol = contorted_tol;
   if (stiff) c->opts.mode = 1;
+  c->opts.split = split != 0;
+  c->opts.half_step = half_step != 0;
+  if (split) c->opts.precision = 0;
   rc = gpr_dev_begin(c, du, nX, dX, CFL, bcs, 0.0, tf, 0, MPs);
-  c->opts = saved;
-  if (rc) return rc;
-  if (stiff) c->opts.mode = 1;
+  if (rc) {
+    c->opts = saved;
+    return rc;
+  }
   int push = 0;
   int64_t k = 0;
   int status = GPR_OK;

@@ -67,10 +67,15 @@ class _Module:
             oscillation_indicator=lambda *a: self._consts()["sig"])
         dg = types.SimpleNamespace(predictor=self.predictor)
         fv = types.SimpleNamespace(Bint=self.Bint, D_RUS=self.D_RUS, fv_launcher=self.fv_launcher)
+        split = types.SimpleNamespace(
+            midstepper=self.midstepper, ode_launcher=self.ode_launcher,
+            analyticSolver_distortion=self.analyticSolver_distortion,
+            analyticSolver_thermal=self.analyticSolver_thermal)
         self.solvers = types.SimpleNamespace(
-            common=common, weno=weno, dg=dg, fv=fv,
+            common=common, weno=weno, dg=dg, fv=fv, split=split,
             extended_dimensions=self.extended_dimensions,
             ader_stepper=self.ader_stepper, ader_stepper_para=self.ader_stepper,
+            split_stepper=self.split_stepper, split_stepper_para=self.split_stepper_para,
             iterator=self.iterator, timestep=self.timestep, contorted=self.contorted,
             renorm_distortion=self.renorm_distortion,
             reset_distortion=self.reset_distortion)
@@ -227,6 +232,50 @@ class _Module:
                                            iptr(nX), float(dt), dptr(_in(dX, np.float64)),
                                            int(bool(STIFF)), int(FLUX), par_ref(MP),
                                            u8ptr(_mask_in(mask))))
+
+    def split_stepper(self, u, ub, nX, dt, dX, HALF_STEP, FLUX, MP, mask):
+        """steppers.cpp:50-66; u and ub are both advanced in place (ub by the
+        half-step source ODE, as the reference's ode_launcher(ub, ...))."""
+        _ref(u, np.float64, "u")
+        _ref(ub, np.float64, "ub")
+        nX = _in(nX, np.int32)
+        check(_lib.load().gpr_split_stepper(self.ctx(len(nX)).h, dptr(u), dptr(ub), iptr(nX),
+                                            float(dt), dptr(_in(dX, np.float64)),
+                                            int(bool(HALF_STEP)), int(FLUX), par_ref(MP),
+                                            u8ptr(_mask_in(mask))))
+
+    def split_stepper_para(self, u, ub, nX, dt, dX, HALF_STEP, FLUX, MP, mask):
+        """steppers.cpp:68-94: the same step on a private copy of ub (the
+        reference hands each thread a copied slab of ub)."""
+        self.split_stepper(u, np.array(ub, dtype=np.float64), nX, dt, dX, HALF_STEP, FLUX, MP,
+                           mask)
+
+    def ode_launcher(self, u, dt, MP):
+        """solvers/split/ode.cpp:5-11, u in place."""
+        _ref(u, np.float64, "u")
+        check(_lib.load().gpr_ode_launcher(self.ctx().h, dptr(u), u.size // self._V, float(dt),
+                                           par_ref(MP)))
+
+    def analyticSolver_distortion(self, Q, dt, MP):
+        """relaxation/analytic.cpp:72-129 on one state (or a block of states), in place."""
+        _ref(Q, np.float64, "Q")
+        check(_lib.load().gpr_analytic_solver(self.ctx().h, dptr(Q), Q.size // self._V,
+                                              float(dt), par_ref(MP), 0))
+
+    def analyticSolver_thermal(self, Q, dt, MP):
+        """relaxation/analytic.cpp:131-152 on one state (or a block of states), in place."""
+        _ref(Q, np.float64, "Q")
+        check(_lib.load().gpr_analytic_solver(self.ctx().h, dptr(Q), Q.size // self._V,
+                                              float(dt), par_ref(MP), 1))
+
+    def midstepper(self, wh, dt, dX, MP, mask):
+        """solvers/split/homogeneous.cpp:5-67, wh in place; the cell count is
+        the mask's size and the number of dimensions dX's."""
+        _ref(wh, np.float64, "wh")
+        dX = _in(dX, np.float64)
+        m = _mask_in(mask)
+        check(_lib.load().gpr_midstepper(self.ctx(len(dX)).h, dptr(wh), float(dt), dptr(dX),
+                                         m.size, par_ref(MP), u8ptr(m)))
 
     def renorm_distortion(self, u, MPs):
         _ref(u, np.float64, "u")

@@ -46,6 +46,30 @@ def mats(thermal):
     return out
 
 
+def relerr(a, b):
+    """max |a - b| / (1 + |b|) (bindings_tests/test_functions.py:20-29)"""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / (1.0 + np.abs(b)))) if a.size else 0.0
+
+
+def ulp_moves(fn, X, rows=False, seeds=4):
+    """How far fn moves when each nonzero entry of its input moves by one ulp
+    in a random direction (zeros stay zero): max |fn(X') - fn(X)| / (1 + |fn(X)|)
+    over a few draws, overall or per row."""
+    X = np.asarray(X, np.float64)
+    r0 = np.asarray(fn(X)).reshape(X.shape)
+    out = np.zeros(len(X)) if rows else 0.0
+    for sd in range(seeds):
+        up = np.random.default_rng(1000 + sd).random(X.shape) < 0.5
+        Xp = np.where(up, np.nextafter(X, np.inf), np.nextafter(X, -np.inf))
+        Xp = np.where(X == 0, 0.0, Xp)
+        r = np.asarray(fn(Xp)).reshape(X.shape)
+        e = np.abs(r - r0) / (1.0 + np.abs(r0))
+        e = np.where(np.isnan(e), 0.0, e)
+        out = np.maximum(out, e.max(axis=1)) if rows else max(out, float(e.max()))
+    return out
+
+
 def save(name, **arrs):
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **arrs)
@@ -234,6 +258,131 @@ def post():
     save("post.npz", **out)
 
 
+def split_ode():
+    """The split step's closed-form source ODE (relaxation/analytic.cpp) on
+    random states with perturbed distortion, per material, at three dt:
+    analyticSolver_distortion, analyticSolver_thermal and ode_launcher, from
+    the reference built with and without fused multiply-add.  Also states
+    with A = diag(a, b, b) (equal singular values: the ordering rule decides
+    which relaxed value each axis gets), A = c I (no change) and a fully
+    relaxed dt.  States whose reference result is non-finite are dropped."""
+    out = {"dts": np.array([1e-4, 1e-2, 1.0, 1e6])}
+    for thermal in (False, True):
+        R = RefSolver(3, thermal)
+        R2 = RefSolver(3, thermal, nofma=True)
+        ms = mats(thermal)
+        ms["pl_fluid"] = material_params("sg", 1.0, cv=1.0, γ=1.4, b0=1.0, μ=1e-2, n=0.5,
+                                         **({"cα": 1.0, "Pr": 0.75} if thermal else {}))
+        for name, MP in ms.items():
+            rng = np.random.default_rng(61)
+            Q = problems.random_states(24, MP, thermal=thermal, seed=62)
+            A = Q[:, 5:14].reshape(-1, 3, 3)
+            Q[:, 5:14] = (A + 0.05 * rng.standard_normal(A.shape)).reshape(-1, 9)
+            if thermal:
+                Q[:, 14:17] *= 0.2
+            sp = Q[:4].copy()
+            for k, (a, b) in enumerate([(1.1, 0.95), (0.9, 1.05), (1.0, 1.0), (1.2, 1.2)]):
+                sp[k, 5:14] = np.diag([a, b, b]).ravel()
+            Q = np.concatenate([Q, sp])
+            keep = np.ones(len(Q), bool)
+            res, floors = {}, {}
+            for k, dt in enumerate(out["dts"]):
+                fns = [("dist", lambda S, R, dt=dt: R.analytic_solver(S, dt, MP, 0))]
+                if thermal:
+                    fns.append(("therm", lambda S, R, dt=dt: R.analytic_solver(S, dt, MP, 1)))
+                fns.append(("ode", lambda S, R, dt=dt: R.ode_launcher(S, dt, MP)))
+                for key, fn in fns:
+                    r = fn(Q, R).reshape(Q.shape)
+                    r2 = fn(Q, R2).reshape(Q.shape)
+                    gap = np.where(np.isnan(r - r2), 0.0,
+                                   np.abs(r - r2) / (1 + np.abs(r))).max(axis=1)
+                    res[f"{key}{k}"] = r
+                    floors[f"{key}{k}_floor"] = np.maximum(gap, ulp_moves(lambda S: fn(S, R), Q, rows=True))
+            for v in res.values():
+                keep &= np.all(np.isfinite(v), axis=1)
+            tag = f"v{R.V}_{name}"
+            out[f"{tag}_par"] = np.array(par_json(MP))
+            out[f"{tag}_Q"] = Q[keep]
+            for k, v in list(res.items()) + list(floors.items()):
+                out[f"{tag}_{k}"] = v[keep]
+    save("split_ode.npz", **out)
+
+
+def split_steps(tag, thermal, ndim, n, mu, nsteps):
+    """Split-WENO steps (steppers.cpp:50-94 inside iterator.cpp:110-165), with
+    and without HALF_STEP, and the step's stages on the first state:
+    midstepper (homogeneous.cpp:5-67) and fv_launcher with TIME = SOURCES =
+    false on the WENO reconstruction."""
+    R = RefSolver(3, thermal)
+    R2 = RefSolver(3, thermal, nofma=True)
+    if ndim == 2 and thermal:
+        u, MP, tf, dX, bcs = problems.grf_density_2d(n, seed=3, μ=mu, p_amp=0.05, thermal=True)
+        nX, tf = [n, n], 1e9
+    elif ndim == 2 and mu is None:
+        u, MP, tf, dX, bcs = problems.plastic_pulse_2d(n)
+        nX, tf = [n, n], 1e9
+    elif ndim == 2:
+        u, MP, tf, dX, bcs = problems.double_shear_2d(n, μ=mu)
+        nX, tf = [n, n], 1e9
+    else:
+        u, MP, tf, dX, bcs = problems.sod_1d(n, thermal=thermal, μ=mu)
+        nX = [n]
+    u = u.ravel()
+    out = dict(par=np.array(par_json(MP)), u=u, nX=np.array(nX), dX=np.array(dX),
+               bcs=np.array(bcs), tf=np.array(tf))
+    # stages at a state one ramp step in (count 10: full dt)
+    u1 = R.run(u, nX, dX, 0.8, bcs, tf, 1, MP, split=True)[0]
+    ub = R.boundaries(u1, nX, bcs)
+    mask = R.extend_mask(np.ones(int(np.prod(nX)), np.uint8), nX)
+    wh = R.weno_launcher(ub, nX, MP)
+    dt = R.timestep(u1, nX, dX, 0.8, 0.0, tf, 10, MP)
+    out.update(stage_u=u1, stage_wh=wh, stage_mask=mask, stage_dt=np.array(dt),
+               stage_mid=R.midstepper(wh, dt, dX, MP, mask),
+               stage_mid_nofma=R2.midstepper(wh, dt, dX, MP, mask),
+               stage_fv=R.fv_launcher_ex(u1, wh, nX, dt, dX, MP, mask, 0, 0),
+               stage_fv_nofma=R2.fv_launcher_ex(u1, wh, nX, dt, dX, MP, mask, 0, 0))
+    for hs in (0, 1):
+        steps, steps2, dts, f_open, f_close = [], [], [], [], []
+        ua, t, c = u.copy(), 0.0, 0
+        for _ in range(nsteps):
+            u2, t2, c2, d1, _, st = R.run(ua, nX, dX, 0.8, bcs, tf, 1, MP, t=t, count=c,
+                                          split=True, half_step=bool(hs))
+            u3, _, _, _, _, _ = R2.run(ua, nX, dX, 0.8, bcs, tf, 1, MP, t=t, count=c,
+                                       split=True, half_step=bool(hs))
+            assert st == 0
+            # the step's two ODE calls on the reference's own intermediates,
+            # with and without FMA (steppers.cpp:54-65)
+            dt = d1[0]
+
+            def ode(x, S=R):
+                return S.ode_launcher(x, dt / 2, MP)
+
+            def ode_floor(x):
+                x2 = np.asarray(x).reshape(-1, R.V)
+                return max(relerr(ode(x, R2), ode(x)), ulp_moves(ode, x2))
+
+            ub = R.boundaries(ua, nX, bcs)
+            uo = ode(ua)
+            f_open.append(max(ode_floor(ua), ode_floor(ub)))
+            wh = R.weno_launcher(ode(ub), nX, MP)
+            if hs:
+                wh = R.midstepper(wh, dt, dX, MP, mask)
+            ufv = R.fv_launcher_ex(uo, wh, nX, dt, dX, MP, mask, 0, 0)
+            uc = ode(ufv)
+            f_close.append(ode_floor(ufv))
+            assert relerr(R.renorm_distortion(uc, MP), u2) < 1e-15  # the same chain
+            steps.append(u2)
+            steps2.append(u3)
+            dts.append(dt)
+            ua, t, c = u2, t2, c2
+        out[f"h{hs}_steps"] = np.array(steps)
+        out[f"h{hs}_steps_nofma"] = np.array(steps2)
+        out[f"h{hs}_dts"] = np.array(dts)
+        out[f"h{hs}_floor_open"] = np.array(f_open)
+        out[f"h{hs}_floor_close"] = np.array(f_close)
+    save(f"split_{tag}.npz", **out)
+
+
 if __name__ == "__main__":
     constants()
     pointwise()
@@ -243,3 +392,9 @@ if __name__ == "__main__":
     stiff()
     post()
     boundaries()
+    split_ode()
+    split_steps("2d_dsl12", False, 2, 12, 0.1, 4)
+    split_steps("2d_grf12_th", True, 2, 12, 1e-2, 4)
+    split_steps("2d_plastic16", False, 2, 16, None, 3)
+    split_steps("1d_sod48_th", True, 1, 48, 1e-2, 4)
+    split_steps("1d_sod48", False, 1, 48, 1e-2, 4)
