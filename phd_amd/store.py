@@ -59,8 +59,9 @@ def save_results(uList, T, N, cfl, SPLIT, FLUX, directory="results"):
 def run(T, N=3, cfl=0.8, SPLIT=False, FLUX=0, contorted_tol=1.0, HALF_STEP=False,
         directory="results"):
     """main.py's driver: build the problem, run iterator on the GPU, keep the
-    snapshots and save them.  SPLIT, HALF_STEP and FLUX != 0 are outside this
-    build and raise GprError (GPR_E_UNSUPPORTED), as the library returns."""
+    snapshots and save them.  SPLIT selects the split-WENO step (HALF_STEP
+    its midstepper); FLUX != 0 is outside this build and raises GprError
+    (GPR_E_UNSUPPORTED), as the library returns."""
     from . import gprpy
     u0, MPs, tf, dX, bcs = T()
     u0 = np.asarray(u0, dtype=np.float64)
