@@ -125,6 +125,14 @@ int gpr_predictor(gpr_ctx *ctx, double *qh, const double *wh, double dt, const d
 int gpr_fv_launcher(gpr_ctx *ctx, double *u, const double *qh, const int32_t *nX, double dt,
                     const double *dX, int sources, int time, int flux, const gpr_par *MP,
                     const uint8_t *mask);
+/* solvers.fv.centers1 / centers2 (fv.cpp:46-54, 99-109): the cell integrals
+ * of fv_launcher alone, u += dt W (S - B_x dq/dx - B_y dq/dy) */
+int gpr_fv_centers(gpr_ctx *ctx, double *u, const double *rec, const int32_t *nX, double dt,
+                   const double *dX, int sources, int time, const gpr_par *MP, const uint8_t *mask);
+/* solvers.fv.interfs1 / interfs2 (fv.cpp:151-160, 270-279): the face terms of
+ * fv_launcher alone, FLUX = 0 */
+int gpr_fv_interfs(gpr_ctx *ctx, double *u, const double *rec, const int32_t *nX, double dt,
+                   const double *dX, int time, int flux, const gpr_par *MP, const uint8_t *mask);
 /* solvers.timestep (iterator.cpp:54-83) for one material: *dt_out */
 int gpr_timestep(gpr_ctx *ctx, double *dt_out, const double *u, const int32_t *nX, const double *dX,
                  double CFL, double t, double tf, int64_t count, const gpr_par *MP);

@@ -1568,6 +1568,13 @@ void orc_fv_launcher(double *u, const double *qh, const int32_t *nX, double dt,
 void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double dt,
                         const double *dX, const gpr_par *p, const uint8_t *mask,
                         int sources, int time, const orc_cfg *c) {
+  orc_fv_parts(u, qh, nX, dt, dX, p, mask, sources, time, 3, c);
+}
+
+/* centers1/2 (parts & 1) and interfs1/2 (parts & 2), fv.cpp:11-279 */
+void orc_fv_parts(double *u, const double *qh, const int32_t *nX, double dt,
+                  const double *dX, const gpr_par *p, const uint8_t *mask,
+                  int sources, int time, int parts, const orc_cfg *c) {
   const consts_t *K = get_consts(c->N);
   int N = c->N, V = c->thermal ? 17 : 14, th = c->thermal;
   int nt = time ? N : 1;
@@ -1576,7 +1583,7 @@ void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double d
     int nx = nX[0];
     double dx = dX[0];
     /* centers1 (fv.cpp:11-54) */
-    for (int t = 0; t < nt; t++) {
+    for (int t = 0; t < ((parts & 1) ? nt : 0); t++) {
       double wt = time ? dt * K->w[t] : dt;
       for (int i = 0; i < nx; i++) {
         if (!mask[i + 1])
@@ -1601,7 +1608,7 @@ void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double d
       }
     }
     /* interfs1 (fv.cpp:111-160) */
-    for (int t = 0; t < nt; t++) {
+    for (int t = 0; t < ((parts & 2) ? nt : 0); t++) {
       double k = (time ? dt * K->w[t] : dt) / (2. * dx);
       for (int i = 0; i < nx + 1; i++) {
         if (!(mask[i] && mask[i + 1]))
@@ -1634,7 +1641,7 @@ void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double d
   double dx = dX[0], dy = dX[1];
   int NNV = N * N * V;
   /* centers2 (fv.cpp:56-109) */
-  for (int t = 0; t < nt; t++) {
+  for (int t = 0; t < ((parts & 1) ? nt : 0); t++) {
     double wt = time ? dt * K->w[t] : dt;
     for (int i = 0; i < nx; i++)
       for (int j = 0; j < ny; j++) {
@@ -1670,7 +1677,7 @@ void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double d
       }
   }
   /* interfs2 (fv.cpp:162-279) */
-  for (int t = 0; t < nt; t++) {
+  for (int t = 0; t < ((parts & 2) ? nt : 0); t++) {
     double wts = time ? dt * K->w[t] : dt;
     for (int i = 0; i < nx + 1; i++)
       for (int j = 0; j < ny + 1; j++) {

@@ -106,6 +106,10 @@ int orc_step(double *u, const int32_t *nX, const double *dX, double CFL,
              double contorted_tol, const gpr_par *p, const orc_cfg *c,
              double *dt_out, orc_stats *st);
 
+/* centers (parts & 1) and interfaces (parts & 2) of fv_launcher */
+void orc_fv_parts(double *u, const double *qh, const int32_t *nX, double dt,
+                  const double *dX, const gpr_par *p, const uint8_t *mask,
+                  int sources, int time, int parts, const orc_cfg *c);
 /* fv_launcher with its SOURCES and TIME flags (TIME = 0: one time level per cell) */
 void orc_fv_launcher_ex(double *u, const double *qh, const int32_t *nX, double dt,
                         const double *dX, const gpr_par *p, const uint8_t *mask,

@@ -247,6 +247,19 @@ class Oracle:
                                   ctypes.c_int(int(time)), ctypes.byref(self.cfg))
         return u
 
+    def fv_parts(self, u, rec, nX, dt, dX, MP, mask, sources, time, parts):
+        """centers (parts 1), interfaces (parts 2) or both (3)"""
+        nX = np.asarray(nX, dtype=np.int32)
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        dX = np.asarray(dX, dtype=np.float64)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.L.orc_fv_parts(_dp(u), _dp(rec), _ip(nX), ctypes.c_double(dt), _dp(dX),
+                            ctypes.byref(MP), _u8(m), ctypes.c_int(int(sources)),
+                            ctypes.c_int(int(time)), ctypes.c_int(int(parts)),
+                            ctypes.byref(self.cfg))
+        return u
+
     def ode_launcher(self, u, dt, MP):
         u = np.array(u, dtype=np.float64, copy=True).ravel()
         self.L.orc_ode_launcher(_dp(u), ctypes.c_long(u.size // self.V), ctypes.c_double(dt),

@@ -34,6 +34,7 @@ SYMBOLS = [
     "gpr_batch_status", "gpr_dev_steps_dt", "gpr_adj_forward", "gpr_adj_backward",
     "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats", "gpr_adj_grad_kappa",
     "gpr_split_stepper", "gpr_ode_launcher", "gpr_analytic_solver", "gpr_midstepper",
+    "gpr_fv_centers", "gpr_fv_interfs",
 ]
 
 
@@ -88,6 +89,8 @@ ARGTYPES = {
     "gpr_ode_launcher": [_P, _Dp, _l, _d, _M],
     "gpr_analytic_solver": [_P, _Dp, _l, _d, _M, _i],
     "gpr_midstepper": [_P, _Dp, _d, _Dp, _l, _M, _Up],
+    "gpr_fv_centers": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
+    "gpr_fv_interfs": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
     "gpr_renorm_distortion": [_P, _Dp, _l, _M],
     "gpr_contorted": [_P, _Ip, _Dp, _l, _d],
     "gpr_reset_distortion": [_P, _Dp, _l, _M],

@@ -330,6 +330,7 @@ struct FvArgs {
   int time;              // 1: qh holds N time nodes (ADER); 0: one, weight dt (split step's wh)
   int sources;           // 1: the cell integral includes S(q)
   int ode;               // with post: the split step's closing half-step ODE first
+  int parts;             // 1: cell integrals (centers1/2), 2: face terms (interfs1/2), 3: both
   Mat m;
 };
 
@@ -492,6 +493,7 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
 #pragma unroll
       for (int v = 0; v < V; v++) S[v] = 0.0;
     }
+    const bool centers = a.parts & 1;
     double dq[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
@@ -514,7 +516,7 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
     const double wt = TIME ? dt * K.w[t] : dt;
     const double wgt = ND == 2 ? wt * K.w[ia] * K.w[ib] : wt * K.w[ia];
 #pragma unroll
-    for (int v = 0; v < V; v++) S[v] = active ? wgt * S[v] : 0.0;
+    for (int v = 0; v < V; v++) S[v] = active && centers ? wgt * S[v] : 0.0;
   }
   __syncthreads();
   if (lane_ok)
@@ -525,6 +527,10 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   for (int v = node; valid && v < V; v += NN) {
     double du = 0;
     for (int k = 0; k < NN; k++) du += myQ[k * V + v];
+    if (!(a.parts & 2)) {
+      a.u[c * V + v] += du;
+      continue;
+    }
     const double *fl = a.fx + ((long)ci * ny + cj) * 2 * V;        // left face: hi side
     const double *fr = a.fx + ((long)(ci + 1) * ny + cj) * 2 * V;  // right face: lo side
     du -= fl[V + v] + fr[v];

@@ -345,7 +345,8 @@ template <int N, bool TH, int ND> struct PredictorSplitOp {
 template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_launch(gpr_ctx *c, const FvArgs &a) {
   long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
   constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
-  hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
+  if (a.parts & 2)
+    hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
   long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
   constexpr int CPW = CellShape<N, ND, TIME>::CPW;
   hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a);
@@ -616,6 +617,7 @@ FvArgs fv_args(gpr_ctx *c, const Mat &m, double dt, const double *dX, const int3
   a.ny = c->ndim == 2 ? nX[1] : 1;
   a.time = 1;
   a.sources = 1;
+  a.parts = 3;
   a.m = m;
   return a;
 }
@@ -894,8 +896,26 @@ int gpr_predictor(gpr_ctx *c, double *qh, const double *wh, double dt, const dou
   return GPR_OK;
 }
 
+static int fv_host(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, double dt, const double *dX,
+                   int sources, int time, int flux, const gpr_par *MP, const uint8_t *mask, int parts);
+
 int gpr_fv_launcher(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, double dt, const double *dX,
                     int sources, int time, int flux, const gpr_par *MP, const uint8_t *mask) {
+  return fv_host(c, u, qh, nX, dt, dX, sources, time, flux, MP, mask, 3);
+}
+
+int gpr_fv_centers(gpr_ctx *c, double *u, const double *rec, const int32_t *nX, double dt, const double *dX,
+                   int sources, int time, const gpr_par *MP, const uint8_t *mask) {
+  return fv_host(c, u, rec, nX, dt, dX, sources, time, 0, MP, mask, 1);
+}
+
+int gpr_fv_interfs(gpr_ctx *c, double *u, const double *rec, const int32_t *nX, double dt, const double *dX,
+                   int time, int flux, const gpr_par *MP, const uint8_t *mask) {
+  return fv_host(c, u, rec, nX, dt, dX, 1, time, flux, MP, mask, 2);
+}
+
+static int fv_host(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, double dt, const double *dX,
+                   int sources, int time, int flux, const gpr_par *MP, const uint8_t *mask, int parts) {
   if (!c || !u || !qh || !dX || !mask) return fail(GPR_E_ARG, "null argument");
   if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
   int rc = check_nX(c, nX);
@@ -921,6 +941,7 @@ int gpr_fv_launcher(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, 
   a.mask = dm;
   a.time = time != 0;
   a.sources = sources != 0;
+  a.parts = parts;
   rc = fv_buffers(c, a);
   if (rc) return rc;
   rc = dispatch<FvOp>(c, c, a);

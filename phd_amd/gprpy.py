@@ -41,6 +41,36 @@ def _mask_in(m):
     return np.ascontiguousarray(m.astype(np.uint8) if m.dtype != np.uint8 else m)
 
 
+class poly:
+    """classes.poly (scipy/poly.h, bindings.cpp:173-178): a polynomial by its
+    coefficients in increasing degree."""
+
+    def __init__(self, coef=()):
+        self.coef = np.asarray(coef, dtype=np.float64)
+
+    def intt(self):
+        """The integral with zero constant term (poly.cpp:6-14)."""
+        c = self.coef
+        return poly(np.concatenate([[0.0], c / np.arange(1, c.size + 1)]))
+
+    def diff(self, n=1):
+        """The n-th derivative (poly.cpp:16-24, 55-60)."""
+        c = self.coef
+        for _ in range(n):
+            c = c[1:] * np.arange(1, c.size)
+        return poly(c)
+
+    def eval(self, x):
+        """sum_i coef_i x^i (poly.cpp:26-33)."""
+        return float(sum(ci * x ** i for i, ci in enumerate(self.coef)))
+
+    def __truediv__(self, c):
+        return poly(self.coef / c)
+
+    def __mul__(self, other):
+        return poly(np.convolve(self.coef, other.coef))
+
+
 class _Module:
     def __init__(self, N=3, THERMAL=False):
         self._N = int(N)
@@ -50,7 +80,7 @@ class _Module:
         self.options = types.SimpleNamespace(
             N=lambda: self._N, NV=lambda: self._V, VISCOUS=lambda: True,
             THERMAL=lambda: self._TH, MULTI=lambda: False, LSET=lambda: 0)
-        self.classes = types.SimpleNamespace(Par=Par, Params=Params)
+        self.classes = types.SimpleNamespace(Par=Par, Params=Params, poly=poly)
         self.system = types.SimpleNamespace(
             flux=self.flux, source=self.source, Bdot=self.Bdot,
             max_abs_eigs=self.max_abs_eigs,
@@ -58,15 +88,17 @@ class _Module:
         common = types.SimpleNamespace(
             scaled_nodes=lambda: self._consts()["nodes"],
             scaled_weights=lambda: self._consts()["wghts"],
-            end_values=lambda: self._consts()["endvals"],
-            derivative_values=lambda: self._consts()["dervals"])
+            basis_polys=self.basis_polys, end_values=self.end_values,
+            derivative_values=self.derivative_values)
         weno = types.SimpleNamespace(
             weno_launcher=self.weno_launcher,
             coefficient_matrices=lambda *a: [np.linalg.inv(self._consts()[k])
                                              for k in ("mLinv", "mRinv", "mCLinv", "mCLinv")],
             oscillation_indicator=lambda *a: self._consts()["sig"])
         dg = types.SimpleNamespace(predictor=self.predictor)
-        fv = types.SimpleNamespace(Bint=self.Bint, D_RUS=self.D_RUS, fv_launcher=self.fv_launcher)
+        fv = types.SimpleNamespace(Bint=self.Bint, D_RUS=self.D_RUS, fv_launcher=self.fv_launcher,
+                                   centers1=self.centers1, centers2=self.centers2,
+                                   interfs1=self.interfs1, interfs2=self.interfs2)
         split = types.SimpleNamespace(
             midstepper=self.midstepper, ode_launcher=self.ode_launcher,
             analyticSolver_distortion=self.analyticSolver_distortion,
@@ -99,6 +131,35 @@ class _Module:
         for k in names[3:]:
             out[k] = out[k].reshape(N, N)
         return out
+
+    # -------------------------------------------------- solvers.common.*
+    def basis_polys(self):
+        """The Lagrange polynomials at the scaled Gauss nodes (basis.cpp:22-53),
+        built as the reference builds them."""
+        x = self._consts()["nodes"]
+        out = []
+        for i in range(self._N):
+            p = poly([1.0])
+            for j in range(self._N):
+                if j != i:
+                    p = p * poly([-x[j], 1.0]) / (x[i] - x[j])
+            out.append(p)
+        return out
+
+    def end_values(self, basis=None):
+        """ret[e, j] = psi_j(e) for e = 0, 1 (basis.cpp:55-63); without an
+        argument, the library's table."""
+        if basis is None:
+            return self._consts()["endvals"]
+        return np.array([[b.eval(0.0) for b in basis], [b.eval(1.0) for b in basis]])
+
+    def derivative_values(self, basis=None, nodes=None):
+        """ret[i, j] = psi_j'(x_i) (basis.cpp:65-72); without arguments, the
+        library's table."""
+        if basis is None:
+            return self._consts()["dervals"]
+        nodes = self._consts()["nodes"] if nodes is None else nodes
+        return np.array([[b.diff(1).eval(xi) for b in basis] for xi in nodes])
 
     def _states(self, Q):
         Q = _in(Q, np.float64)
@@ -208,6 +269,39 @@ class _Module:
                                           iptr(nX), float(dt), dptr(_in(dX, np.float64)),
                                           int(bool(SOURCES)), int(bool(TIME)), int(FLUX),
                                           par_ref(MP), u8ptr(_mask_in(mask))))
+
+    def _fv_part(self, fn, u, rec, nX, dt, dX, flag, TIME, MP, mask):
+        _ref(u, np.float64, "u")
+        nX = np.asarray(nX, dtype=np.int32)
+        check(fn(self.ctx(len(nX)).h, dptr(u), dptr(_in(rec, np.float64)), iptr(nX), float(dt),
+                 dptr(_in(dX, np.float64)), int(flag), int(bool(TIME)), par_ref(MP),
+                 u8ptr(_mask_in(mask))))
+
+    def centers1(self, u, rec, nx, dt, dx, SOURCES, TIME, MP, mask):
+        """fv.cpp:46-54, u in place."""
+        self._fv_part(_lib.load().gpr_fv_centers, u, rec, [nx], dt, [dx], bool(SOURCES), TIME,
+                      MP, mask)
+
+    def centers2(self, u, rec, nx, ny, dt, dx, dy, SOURCES, TIME, MP, mask):
+        """fv.cpp:99-109, u in place."""
+        self._fv_part(_lib.load().gpr_fv_centers, u, rec, [nx, ny], dt, [dx, dy],
+                      bool(SOURCES), TIME, MP, mask)
+
+    def interfs1(self, u, rec, nx, dt, dx, TIME, FLUX, MP, mask):
+        """fv.cpp:151-160, u in place."""
+        _ref(u, np.float64, "u")
+        check(_lib.load().gpr_fv_interfs(self.ctx(1).h, dptr(u), dptr(_in(rec, np.float64)),
+                                         iptr(np.array([nx], np.int32)), float(dt),
+                                         dptr(np.array([dx], np.float64)), int(bool(TIME)),
+                                         int(FLUX), par_ref(MP), u8ptr(_mask_in(mask))))
+
+    def interfs2(self, u, rec, nx, ny, dt, dx, dy, TIME, FLUX, MP, mask):
+        """fv.cpp:270-279, u in place."""
+        _ref(u, np.float64, "u")
+        check(_lib.load().gpr_fv_interfs(self.ctx(2).h, dptr(u), dptr(_in(rec, np.float64)),
+                                         iptr(np.array([nx, ny], np.int32)), float(dt),
+                                         dptr(np.array([dx, dy], np.float64)), int(bool(TIME)),
+                                         int(FLUX), par_ref(MP), u8ptr(_mask_in(mask))))
 
     def timestep(self, grids, masks, dX, CFL, t, tf, count, MPs, nmat, nX=None):
         """iterator.cpp:54-83.  The reference recovers the cell count from the

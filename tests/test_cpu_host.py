@@ -185,3 +185,31 @@ def test_1d_equals_2d_for_y_uniform_data():
     b = stage(O2, u2, [nx, ny], [dX[0], dX[0]], [0, 0, 1, 1]).reshape(nx, ny, 14)
     for j in range(2, ny - 2):
         assert relerr(b[3:nx - 3, j, :], a[3:nx - 3]) < 1e-12
+
+
+def test_basis_polys_mirror():
+    """solvers.common.basis_polys / end_values / derivative_values and
+    classes.poly (basis.cpp:22-72, poly.cpp): Lagrange property at the
+    scaled Gauss nodes; end and derivative values equal the reference's own
+    tables (tests/golden/constants.npz) and the library's; poly arithmetic in
+    closed form."""
+    import os
+    from phd_amd.gprpy import load, poly
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "constants.npz"))
+    for N in (2, 3):
+        G = load(N, False)
+        B = G.solvers.common.basis_polys()
+        x = G.solvers.common.scaled_nodes()
+        L = np.array([[b.eval(xi) for b in B] for xi in x])
+        assert np.allclose(L, np.eye(N), atol=1e-14)
+        ev = G.solvers.common.end_values(B)
+        dv = G.solvers.common.derivative_values(B, x)
+        assert np.allclose(ev, g[f"n{N}_endvals"], rtol=0, atol=1e-13)
+        assert np.allclose(dv, g[f"n{N}_dervals"], rtol=0, atol=1e-12)
+        assert np.allclose(ev, G.solvers.common.end_values(), rtol=0, atol=1e-13)
+    p = poly([1.0, 2.0, 3.0])
+    assert np.allclose(p.intt().coef, [0, 1, 1, 1])
+    assert np.allclose(p.diff(1).coef, [2, 6]) and np.allclose(p.diff(2).coef, [6])
+    assert p.eval(2.0) == 17.0
+    assert np.allclose((p * poly([0.0, 1.0])).coef, [0, 1, 2, 3])

@@ -177,6 +177,33 @@ def test_fv(gp, case):
     assert relerr(ug, O.fv_launcher(u, qh, nX, dt, dX, MP, mask)) < TOL
 
 
+@pytest.mark.parametrize("case", ["2d", "1d_thermal"])
+@pytest.mark.parametrize("time,sources", [(1, 1), (0, 0), (1, 0), (0, 1)])
+def test_fv_centers_and_interfaces(gp, case, time, sources):
+    """solvers.fv.centers1/2 and interfs1/2 (fv.cpp:11-279) on their own,
+    each against the oracle's part, for every SOURCES/TIME pair; TIME = 0
+    reads the WENO reconstruction."""
+    if case == "2d":
+        O = oracle(3, False, 2)
+        G = gp(3, False)
+        u, MP, dX, bcs, nX, ub, wh, mask, dt = _stage_inputs(O, 16, 0.1)
+    else:
+        O = oracle(3, True, 1)
+        G = gp(3, True)
+        u, MP, dX, bcs, nX, ub, wh, mask, dt = _stage_inputs(O, 48, 1e-2, ndim=1, thermal=True)
+    rec = O.predictor(wh, dt, dX, nX, MP, mask) if time else wh
+    uc, ui = u.copy(), u.copy()
+    if case == "2d":
+        G.solvers.fv.centers2(uc, rec, 16, 16, dt, dX[0], dX[1], sources, time, MP, mask)
+        G.solvers.fv.interfs2(ui, rec, 16, 16, dt, dX[0], dX[1], time, 0, MP, mask)
+    else:
+        G.solvers.fv.centers1(uc, rec, 48, dt, dX[0], sources, time, MP, mask)
+        G.solvers.fv.interfs1(ui, rec, 48, dt, dX[0], time, 0, MP, mask)
+    assert relerr(uc, O.fv_parts(u, rec, nX, dt, dX, MP, mask, sources, time, 1)) < TOL
+    assert relerr(ui, O.fv_parts(u, rec, nX, dt, dX, MP, mask, sources, time, 2)) < TOL
+    assert relerr(ui, u) > 1e-8  # (the 2D start has A = I: its cell integrals vanish)
+
+
 def test_timestep(gp):
     O = oracle(3, False, 2)
     G = gp(3, False)

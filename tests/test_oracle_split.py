@@ -173,3 +173,21 @@ def test_split_step_conserves_mass_momentum_energy():
         s1 = u.reshape(-1, V)[:, :5].sum(axis=0)
         scale = np.abs(u.reshape(-1, V)[:, :5]).sum(axis=0) + 1
         assert np.all(np.abs(s1 - s0) / scale < 1e-14), (thermal, (s1 - s0) / scale)
+
+
+@pytest.mark.parametrize("time", [1, 0])
+def test_fv_parts_compose_to_fv_launcher(time):
+    """centers then interfaces equal fv_launcher bit for bit (fv.cpp:281-297
+    calls them in that order on the same u)."""
+    from phd_amd import problems
+    u, MP, _, dX, bcs = problems.double_shear_2d(12, μ=0.1)
+    O = oracle(False, 2)
+    u, nX = u.ravel(), [12, 12]
+    ub = O.boundaries(u, nX, bcs)
+    m = O.extend_mask(np.ones(144, np.uint8), nX)
+    wh = O.weno_launcher(ub, nX)
+    rec = O.predictor(wh, 1e-3, dX, nX, MP, m) if time else wh
+    for src in (1, 0):
+        a = O.fv_parts(u, rec, nX, 1e-3, dX, MP, m, src, time, 1)
+        b = O.fv_parts(a, rec, nX, 1e-3, dX, MP, m, src, time, 2)
+        assert np.array_equal(b, O.fv_launcher_ex(u, rec, nX, 1e-3, dX, MP, m, src, time))
