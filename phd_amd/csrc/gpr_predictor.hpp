@@ -240,7 +240,8 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   constexpr bool FB = KIND != KIND_PICARD;  // this instantiation has the fallback
   static_assert(!FB || sizeof(R) == sizeof(double), "the fallback computes in double");
   __shared__ R sQ[CPW * NN * V];   // iterate; in the fallback update also P g
-  __shared__ R sF[CPW * NN * V];   // fluxes, then rhs or g
+  __shared__ R sF[CPW * NN * V];   // x fluxes, then rhs or g
+  __shared__ R sG[ND == 2 ? CPW * NN * V : 1];  // y fluxes
   __shared__ double sPre[FB ? CPW * NS * PRE : 1];
   __shared__ R sRat[WAVE];
 
@@ -291,6 +292,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   const int sl = lane_ok ? slot : 0;
   R *myQ = sQ + (sl * NN) * V;
   R *myF = sF + (sl * NN) * V;
+  R *myG = sG + (ND == 2 ? (sl * NN) * V : 0);
   double *myPre = sPre + (FB ? (sl * NS + s) * PRE : 0);
   const WIn *wsrc = (const WIn *)a.wh + (cell * NS + s) * V;
 
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     wave_sync();
 
     // (2) spatial derivatives, source, non-conservative products, fluxes
-    R rhs[V], G[V];
+    R rhs[V];
     {
       NodeState<R, TH> st;
       node_state<R, TH>(q0, M, st);
@@ -372,10 +374,15 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       if (lane_ok)
 #pragma unroll
         for (int v = 0; v < V; v++) myF[node * V + v] = F[v] * wb;
-      if (ND == 2) flux<R, TH>(q0, st, M, 1, G);
+      if (ND == 2) {
+        flux<R, TH>(q0, st, M, 1, F);
+        if (lane_ok)
+#pragma unroll
+          for (int v = 0; v < V; v++) myG[node * V + v] = F[v];
+      }
     }
     wave_sync();
-    // (3) flux contractions (rhs2, dg2.cpp:173-201) through one buffer
+    // (3) flux contractions (rhs2, dg2.cpp:173-201): x from sF, y from sG
     {
       const R cx = Wt * inv_dx;
 #pragma unroll
@@ -387,17 +394,12 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       }
     }
     if (ND == 2) {
-      wave_sync();
-      if (lane_ok)
-#pragma unroll
-        for (int v = 0; v < V; v++) myF[node * V + v] = G[v];
-      wave_sync();
       const R cy = Wt * R(K.w[ia]) * inv_dy;
 #pragma unroll
       for (int v = 0; v < V; v++) {
         R sy = 0;
 #pragma unroll
-        for (int j = 0; j < N; j++) sy += cgy[j] * myF[(t * NS + ia * N + j) * V + v];
+        for (int j = 0; j < N; j++) sy += cgy[j] * myG[(t * NS + ia * N + j) * V + v];
         rhs[v] -= cy * sy;
       }
     }
