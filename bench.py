@@ -344,6 +344,44 @@ def split_leg(n, steps, warmup, threads, cpu):
     return out
 
 
+def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode):
+    """Per-cell predictor iterations of one step from state u (the timed
+    leg's starting state), through the host entry point gpr_predictor with
+    the same solver options: a histogram of Picard iterations over the cells
+    Picard solves, and of fallback iterations over the cells it leaves."""
+    from phd_amd.gprpy import load
+    G = load(3, False)
+    ctx = G.ctx(2)
+    saved = {"bailout": ctx.opts.bailout, "mode": ctx.opts.mode}
+    ctx.set_opts(bailout=bailout, mode=mode)
+    try:
+        nXa = np.array(nX, np.int32)
+        ne = int(np.prod(nXa + 2))
+        nb = int(np.prod(nXa + 6))
+        ub = np.zeros(nb * 14)
+        G.boundaries(u, ub, nXa, np.array(bcs, np.int32))
+        mask = np.zeros(ne, bool)
+        G.extend_mask(np.ones(int(np.prod(nXa)), bool), mask, nXa)
+        wh = np.zeros(ne * 9 * 14)
+        G.solvers.weno.weno_launcher(wh, ub, nXa, MP)
+        dt = G.solvers.timestep([u], [None], dX, 0.8, t, 1e300, count, [MP], 1)
+        qh = np.zeros(ne * 27 * 14)
+        iters = np.zeros(ne, np.int32)
+        G.solvers.dg.predictor(qh, wh, dt, dX, False, False, MP, mask, iters=iters)
+    finally:
+        ctx.set_opts(**saved)
+    it = iters[mask]
+    pic = it[(it >= 0) & (it < 1000)]
+    fb = it[it >= 1000] - 1000
+    return {"cells": int(it.size), "picard_cells": int(pic.size), "fallback_cells": int(fb.size),
+            "failed_cells": int(np.sum(it < 0)),
+            "picard_mean": float(pic.mean()) if pic.size else 0.0,
+            "picard_histogram": {int(k): int(v) for k, v in zip(*np.unique(pic, return_counts=True))},
+            "fallback_histogram": {int(k): int(v) for k, v in zip(*np.unique(fb, return_counts=True))},
+            "note": "Picard iterations of the cells Picard solves; cells it leaves (after "
+                    "growing `bailout` times in a row) are counted by their fallback iterations"}
+
+
 def timed_run(run, steps):
     """K steps bracketed by a synchronisation of the library's stream (all
     device work of this process runs on it); HIP-event phase timing on the
@@ -434,6 +472,8 @@ def main():
         "fallback_iters_per_cell": tm["fallback_iters"] / max(tm["cells"], 1),
         "fallback_cell_fraction": tm["fallback_cells"] / max(tm["cells"], 1),
         "phase_ms_per_step": {k: v / K for k, v in tm["ms"].items()},
+        "iterations_first_timed_step": iteration_histogram(u_w, MP, nX, dX, bcs, st0["t"],
+                                                           st0["count"], args.bailout, args.mode),
     }
 
     out = {
