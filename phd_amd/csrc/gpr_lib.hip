@@ -1186,7 +1186,9 @@ int gpr_dev_begin(gpr_ctx *c, double *d_u, const int32_t *nX, const double *dX, 
   if (!buf(c, "ub", nb * c->V * 8, rc)) return rc;
   if (!buf(c, "wh", ne * NS * c->V * 8, rc)) return rc;
   if (!c->opts.split && !buf(c, "qh", ne * NS * c->N * c->V * 8, rc)) return rc;
-  uint8_t *dm = (uint8_t *)buf(c, "mask", ne, rc);
+  // the run's mask is set here once and read by every step: a buffer of its
+  // own, so that host-pointer stage calls on this context cannot change it
+  uint8_t *dm = (uint8_t *)buf(c, "run_mask", ne, rc);
   if (!dm) return rc;
   if (c->ndim == 2 && !buf(c, "ux", (long)(nX[0] + 2) * (nX[1] + 2 * c->N) * c->N * c->V * 8, rc)) return rc;
   FvArgs fa = fv_args(c, m, 0, dX, nX);
@@ -1234,7 +1236,7 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
   double *dub = (double *)c->bufs["ub"].first;
   double *dwh = (double *)c->bufs["wh"].first;
   double *dqh = split ? nullptr : (double *)c->bufs["qh"].first;
-  uint8_t *dm = (uint8_t *)c->bufs["mask"].first;
+  uint8_t *dm = (uint8_t *)c->bufs["run_mask"].first;
   if (!split && !dqh) return fail(GPR_E_ARG, "gpr_dev_begin ran for the split step; call it again");
   for (int64_t k = 0; k < nsteps; k++) {
     hipEvent_t *ev = nullptr;
@@ -1923,7 +1925,7 @@ int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt, const dou
   double *dub = (double *)c->bufs["ub"].first;
   double *dwh = (double *)c->bufs["wh"].first;
   double *dqh = (double *)c->bufs["qh"].first;
-  uint8_t *dm = (uint8_t *)c->bufs["mask"].first;
+  uint8_t *dm = (uint8_t *)c->bufs["run_mask"].first;
   rc = dispatch<BoundariesOp>(c, c, u_n, dub, c->run_nx, c->run_ny, c->run_bc);
   if (rc) return rc;
   rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
@@ -1976,7 +1978,7 @@ int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *
   a.qh = qh_saved ? qh_saved : (const double *)c->bufs["qh"].first;
   a.ux = (const double *)c->bufs["ux"].first;
   a.ub = (const double *)c->bufs["ub"].first;
-  a.mask = (const uint8_t *)c->bufs["mask"].first;
+  a.mask = (const uint8_t *)c->bufs["run_mask"].first;
   a.gmu = d_gmu;
   a.gkap = d_gkap;
   a.kappa = c->run_mat.p.kappa;

@@ -204,6 +204,28 @@ def test_fv_centers_and_interfaces(gp, case, time, sources):
     assert relerr(ui, u) > 1e-8  # (the 2D start has A = I: its cell integrals vanish)
 
 
+def test_host_calls_do_not_disturb_a_device_run(gp):
+    """A host-pointer stage call on the context of a device-resident run (on
+    another grid, with an all-false mask) between its steps leaves the run's
+    result bit-identical to an uninterrupted run."""
+    from phd_amd.runner import DeviceRun
+    u0, MP, _, dX, bcs = dsl(16, 0.1)
+    ref = DeviceRun(u0.ravel(), MP, [16, 16], dX, bcs, CFL=0.8)
+    ref.steps(3)
+    uref = ref.host()
+    ref.close()
+    G = gp(3, False)
+    run = DeviceRun(u0.ravel(), MP, [16, 16], dX, bcs, CFL=0.8, ctx=G.ctx(2))
+    run.steps(1)
+    O = oracle(3, False, 2)
+    u8, MP8, dX8, bcs8, nX8, ub8, wh8, mask8, dt8 = _stage_inputs(O, 8, 0.1)
+    qh8 = np.zeros(wh8.size * 3)
+    G.solvers.dg.predictor(qh8, wh8, dt8, dX8, False, False, MP8, np.zeros(mask8.size, bool))
+    run.steps(2)
+    assert np.array_equal(run.host(), uref)
+    run.close()
+
+
 def test_timestep(gp):
     O = oracle(3, False, 2)
     G = gp(3, False)
