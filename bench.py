@@ -153,6 +153,8 @@ def fp32_bulk_leg(n, steps):
                      "fallback_iters": tm["fallback_iters"],
                      "fallback_cells": tm["fallback_cells"]}
         states[name] = run.host().reshape(-1, 14)
+        if name == "fp64":
+            end = run.status()
         run.close()
     a, b = states["fp32_bulk"], states["fp64"]
     scale = np.max(np.abs(b), axis=0)
@@ -165,6 +167,12 @@ def fp32_bulk_leg(n, steps):
         "L1": {k: float(v) for k, v in zip(names, np.mean(np.abs(a - b), axis=0) / scale)},
         "Linf": {k: float(v) for k, v in zip(names, np.max(np.abs(a - b), axis=0) / scale)}}
     out["speedup"] = out["fp32_bulk"]["value"] / out["fp64"]["value"]
+    # per-cell Picard iterations of the two predictors on one state (the FP64
+    # run's last): FP32 rounding of the stiff solid's state sits near DG_TOL
+    out["iterations_last_state"] = {
+        name: iteration_histogram(states["fp64"].ravel(), MP, [n, n], dX, bcs, end["t"], end["count"],
+                                  5, 0, precision=prec)
+        for name, prec in (("fp64", 0), ("fp32_bulk", 1))}
     return out
 
 
@@ -344,7 +352,7 @@ def split_leg(n, steps, warmup, threads, cpu):
     return out
 
 
-def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode):
+def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode, precision=0):
     """Per-cell predictor iterations of one step from state u (the timed
     leg's starting state), through the host entry point gpr_predictor with
     the same solver options: a histogram of Picard iterations over the cells
@@ -352,8 +360,8 @@ def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode):
     from phd_amd.gprpy import load
     G = load(3, False)
     ctx = G.ctx(2)
-    saved = {"bailout": ctx.opts.bailout, "mode": ctx.opts.mode}
-    ctx.set_opts(bailout=bailout, mode=mode)
+    saved = {"bailout": ctx.opts.bailout, "mode": ctx.opts.mode, "precision": ctx.opts.precision}
+    ctx.set_opts(bailout=bailout, mode=mode, precision=precision)
     try:
         nXa = np.array(nX, np.int32)
         ne = int(np.prod(nXa + 2))

@@ -56,9 +56,11 @@ typedef struct gpr_solver_opts {
                             the fallback as a second launch over a list of the
                             cells Picard leaves */
   double contorted_tol; /* reset threshold of iterator(), its contorted_tol argument */
-  int32_t precision;  /* device loop only (gpr_dev_steps): 0 FP64 throughout;
+  int32_t precision;  /* device loop (gpr_dev_steps): 0 FP64 throughout;
                          1 FP32 bulk: WENO and Picard in FP32, the stiff
-                         fallback, FV accumulation, post-step and CFL in FP64 */
+                         fallback, FV accumulation, post-step and CFL in FP64.
+                         gpr_predictor without STIFF: 1 rounds wh to float and
+                         runs the same two predictor launches */
   int32_t split;      /* device loop: 0 the ADER step; 1 the split-WENO step
                          (steppers.cpp:50-66, iterator's SPLIT), FP64 only */
   int32_t half_step;  /* with split: the midstepper between WENO and FV (HALF_STEP) */
@@ -114,7 +116,8 @@ int gpr_weno_launcher(gpr_ctx *ctx, double *wh, const double *ub, const int32_t 
 /* solvers.dg.predictor(qh, wh, dt, dX, STIFF, STIFF_IG, MP, mask) (dg.h:7-8);
  * ncell = mask length (ghost-extended cells), as the reference takes it from
  * the mask.  iters (optional, [ncell]) receives the Picard count, 1000 + the
- * fallback count, or -1 for a failed cell.  STIFF selects mode 1. */
+ * fallback count, or -1 for a failed cell.  STIFF selects mode 1.  With the
+ * solver option precision = 1 (and STIFF = 0), the FP32-bulk predictor. */
 int gpr_predictor(gpr_ctx *ctx, double *qh, const double *wh, double dt, const double *dX,
                   int64_t ncell, int stiff, int stiff_ig, const gpr_par *MP, const uint8_t *mask,
                   int32_t *iters);

@@ -885,7 +885,26 @@ int gpr_predictor(gpr_ctx *c, double *qh, const double *wh, double dt, const dou
   a.mask = dm;
   a.iters = dit;
   a.ncell = ne;
-  rc = dispatch<PredictorOp>(c, c, a);
+  std::vector<float> whf;
+  if (c->opts.precision == 1 && !stiff) {
+    // the FP32-bulk form: wh rounded to float, Picard in FP32, the cells it
+    // leaves solved again in FP64 by the fallback launch over a list
+    whf.resize(nw);
+    for (long i = 0; i < nw; i++) whf[i] = (float)wh[i];
+    float *dwf = (float *)buf(c, "whf", nw * sizeof(float), rc);
+    if (!dwf) return rc;
+    int *dlist = (int *)buf(c, "plist", ne * sizeof(int), rc);
+    if (!dlist) return rc;
+    int *dlcnt = (int *)buf(c, "plcnt", sizeof(int), rc);
+    if (!dlcnt) return rc;
+    HIP_TRY(hipMemcpyAsync(dwf, whf.data(), nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    a.wh = dwf;
+    a.list = dlist;
+    a.list_count = dlcnt;
+    rc = dispatch<PredictorF32Op>(c, c, a);
+  } else {
+    rc = dispatch<PredictorOp>(c, c, a);
+  }
   if (rc) return rc;
   int nfail = 0;
   HIP_TRY(hipMemcpyAsync(qh, dqh, nq * 8, hipMemcpyDeviceToHost, c->stream));
