@@ -390,6 +390,41 @@ def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode, precision=0
                     "growing `bailout` times in a row) are counted by their fallback iterations"}
 
 
+def full_run_leg(u0, MP, nX, dX, bcs, total, seg, bailout, mode):
+    """The whole job of the configured workload: `total` steps from t = 0 in
+    segments of `seg` steps, each timed as the main line is (stream
+    synchronisation around it, HIP-event phase timing inside), so that the
+    segments show how the cost moves as the shear layers roll up."""
+    from phd_amd.runner import DeviceRun
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=bailout, mode=mode)
+    cells = int(np.prod(nX))
+    segs, done, wall = [], 0, 0.0
+    while done < total:
+        k = min(seg, total - done)
+        try:
+            sec, tm = timed_run(run, k)
+        except SystemExit as e:
+            # reported, not fatal: the main line above is the measurement
+            run.close()
+            return {"config": f"the same workload from t = 0, {total} steps", "failed": str(e),
+                    "segments": segs}
+        K = max(tm["steps"], 1)
+        segs.append({"steps": f"{done}-{done + k}", "ms_per_step": 1e3 * sec / k,
+                     "predictor_ms": tm["ms"]["predictor"] / K,
+                     "picard_iters_per_cell": tm["picard_iters"] / max(tm["cells"], 1),
+                     "fallback_cell_fraction": tm["fallback_cells"] / max(tm["cells"], 1)})
+        wall += sec
+        done += k
+    st = run.status()
+    u = run.host().reshape(-1, 14)
+    run.close()
+    u0 = u0.reshape(-1, 14)
+    return {"config": f"the same workload from t = 0, all {total} steps", "steps": total,
+            "t_end": st["t"], "seconds": wall, "value": cells * total / wall,
+            "ms_per_step": 1e3 * wall / total, "segments": segs,
+            "mass_change": float(abs(u[:, 0].sum() - u0[:, 0].sum()) / u0[:, 0].sum())}
+
+
 def timed_run(run, steps):
     """K steps bracketed by a synchronisation of the library's stream (all
     device work of this process runs on it); HIP-event phase timing on the
@@ -412,6 +447,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--full-steps", type=int, default=2000,
+                    help="steps of the whole-job run from t = 0 (0 skips it)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -495,6 +532,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
     }
+    if args.full_steps > 0:
+        out["full_run"] = full_run_leg(u0, MP, nX, dX, bcs, args.full_steps, 500, args.bailout,
+                                       args.mode)
+        progress(f"full run: {args.full_steps} steps in {out['full_run']['seconds']:.2f} s")
     if args.bailout != 0 and not args.no_faithful:
         # the same steps with Picard run to DG_IT = 50 for every cell, as the
         # reference does, from the same starting state
