@@ -2,10 +2,13 @@
 and Picard in FP32, the stiff fallback, FV accumulation, post-step and CFL in
 FP64.  Checked against the plain FP64 device loop on the same inputs.
 
-The difference is set less by FP32 rounding (~6e-8) than by where Picard
-stops: both forms stop once no entry moved by more than DG_TOL (1 + |q|)
-with DG_TOL = 6e-6 (dg2.cpp:149), and rounding can move that stop by an
-iteration, so the two predictor solutions differ at that tolerance.  The
+The difference is set by where Picard stops: both forms stop once no entry
+moved by more than DG_TOL (1 + |q|) with DG_TOL = 6e-6 (dg2.cpp:149), and
+rounding can move that stop by an iteration.  On a stiff state, FP32
+rounding itself reaches that level: one rounding of A and rho (6e-8)
+becomes ~1e-5 in the stress and pressure of the elastoplastic solid
+(rho c_s^2 ~ 39, rho0 c0^2 ~ 136).  Either way the two predictor
+solutions differ at about the tolerance.  The
 bound is therefore stated in the reference's norm max|a-b| / (1 + |b|)
 (bindings_tests/test_functions.py:20-29) as a multiple of DG_TOL per step."""
 
