@@ -108,6 +108,10 @@ long ipow(long b, int e) {
   return r;
 }
 
+// Riemann solver of the step entry points below (fluxes.h: 0 Rusanov, 1 Roe,
+// 2 Osher); set by ref_set_flux.
+int g_flux = 0;
+
 } // namespace
 
 extern "C" {
@@ -315,6 +319,66 @@ void ref_fv_launcher_ex(double *u, const double *rec, const int32_t *nX, int ndi
   fv_launcher(um, qv, n, dt, d, sources != 0, time != 0, 0, P, m);
 }
 
+// One half of fv_launcher (fv.cpp:11-279) as the bindings expose it
+// (bindings.cpp:248-251): part 1 the cell centres (centers1/centers2), part 2
+// the interfaces (interfs1/interfs2) with the given Riemann solver.
+void ref_fv_part(double *u, const double *rec, const int32_t *nX, int ndim,
+                 double dt, const double *dX, const gpr_par *p,
+                 const uint8_t *mask, int sources, int time, int part,
+                 int flux) {
+  Par P = to_par(p);
+  long nc = prod_ext(nX, ndim, 0), ne = prod_ext(nX, ndim, 1);
+  long nq = ne * ipow(N, ndim + (time ? 1 : 0));
+  VMap um(u, nc * V);
+  Vec qv = Eigen::Map<const Vec>(rec, nq * V);
+  bVec m(ne);
+  for (long i = 0; i < ne; i++)
+    m(i) = mask[i] != 0;
+  if (ndim == 1) {
+    if (part == 1)
+      centers1(um, qv, nX[0], dt, dX[0], sources != 0, time != 0, P, m);
+    else
+      interfs1(um, qv, nX[0], dt, dX[0], time != 0, flux, P, m);
+  } else {
+    if (part == 1)
+      centers2(um, qv, nX[0], nX[1], dt, dX[0], dX[1], sources != 0,
+               time != 0, P, m);
+    else
+      interfs2(um, qv, nX[0], nX[1], dt, dX[0], dX[1], time != 0, flux, P,
+               m);
+  }
+}
+
+void ref_set_flux(int flux) { g_flux = flux; }
+
+// D_ROE / D_OSH (fluxes.cpp:22-64) and system_matrix (equations.cpp:169-176)
+void ref_d_roe(double *ret, const double *qL, const double *qR, int d,
+               const gpr_par *p) {
+  Par P = to_par(p);
+  VecV a = Eigen::Map<const VecV>(qL), b = Eigen::Map<const VecV>(qR);
+  VecV r = D_ROE(a, b, d, P);
+  for (int i = 0; i < V; i++)
+    ret[i] = r(i);
+}
+
+void ref_d_osh(double *ret, const double *qL, const double *qR, int d,
+               const gpr_par *p) {
+  Par P = to_par(p);
+  VecV a = Eigen::Map<const VecV>(qL), b = Eigen::Map<const VecV>(qR);
+  VecV r = D_OSH(a, b, d, P);
+  for (int i = 0; i < V; i++)
+    ret[i] = r(i);
+}
+
+void ref_system_matrix(double *ret, const double *Q, int d, const gpr_par *p) {
+  Par P = to_par(p);
+  VecV q = Eigen::Map<const VecV>(Q);
+  MatV_V M = system_matrix(q, d, P);
+  for (int i = 0; i < V; i++)
+    for (int j = 0; j < V; j++)
+      ret[i * V + j] = M(i, j);
+}
+
 // solvers/split/ode.cpp:5-11 on ncell states
 void ref_ode_launcher(double *u, long ncell, double dt, const gpr_par *p) {
   Par P = to_par(p);
@@ -416,7 +480,7 @@ void ref_ader_stepper_para(double *u, const int32_t *nX, int ndim, double dt,
   bVec m = bVec::Ones(nc), mb(ne);
   extend_mask(m, mb, n);
   Vec d = make_dX(dX, ndim);
-  ader_stepper_para(um, ub, n, dt, d, false, 0, P, mb);
+  ader_stepper_para(um, ub, n, dt, d, false, g_flux, P, mb);
 }
 
 // Runs up to `nsteps` steps of the reference time loop (iterator.cpp:104-166,
@@ -458,9 +522,9 @@ long ref_run(double *u, const int32_t *nX, int ndim, const double *dX,
     double dt = timestep(grids, masks, da, CFL, t, tf, count, MPs, 1);
     boundaries(grids[0], ub, n, b);
     if (split)
-      split_stepper_para(grids[0], ub, n, dt, dv, half_step != 0, 0, MPs[0], mb);
+      split_stepper_para(grids[0], ub, n, dt, dv, half_step != 0, g_flux, MPs[0], mb);
     else
-      ader_stepper_para(grids[0], ub, n, dt, dv, stiff != 0, 0, MPs[0], mb);
+      ader_stepper_para(grids[0], ub, n, dt, dv, stiff != 0, g_flux, MPs[0], mb);
     um = grids[0];
     t += dt;
     count += 1;

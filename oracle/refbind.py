@@ -123,6 +123,31 @@ class RefSolver:
         self.lib.ref_bint(_dp(r), _dp(qL), _dp(qR), ctypes.c_int(d), ctypes.byref(MP))
         return r
 
+    def D_ROE(self, qL, qR, d, MP):
+        qL = np.ascontiguousarray(qL, dtype=np.float64)
+        qR = np.ascontiguousarray(qR, dtype=np.float64)
+        r = np.zeros(self.V)
+        self.lib.ref_d_roe(_dp(r), _dp(qL), _dp(qR), ctypes.c_int(d), ctypes.byref(MP))
+        return r
+
+    def D_OSH(self, qL, qR, d, MP):
+        qL = np.ascontiguousarray(qL, dtype=np.float64)
+        qR = np.ascontiguousarray(qR, dtype=np.float64)
+        r = np.zeros(self.V)
+        self.lib.ref_d_osh(_dp(r), _dp(qL), _dp(qR), ctypes.c_int(d), ctypes.byref(MP))
+        return r
+
+    def system_matrix(self, Q, d, MP):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        r = np.zeros((self.V, self.V))
+        self.lib.ref_system_matrix(_dp(r), _dp(Q), ctypes.c_int(d), ctypes.byref(MP))
+        return r
+
+    def set_flux(self, flux):
+        """The Riemann solver (0 Rusanov, 1 Roe, 2 Osher) of ader_stepper_para
+        and run."""
+        self.lib.ref_set_flux(ctypes.c_int(int(flux)))
+
     # -- grid
     def boundaries(self, u, nX, bcs):
         nX = np.asarray(nX, dtype=np.int32)
@@ -191,6 +216,20 @@ class RefSolver:
         self.lib.ref_fv_launcher_ex(_dp(u), _dp(rec), _ip(nX), ctypes.c_int(len(nX)),
                                     ctypes.c_double(dt), _dp(dX), ctypes.byref(MP), _u8(m),
                                     ctypes.c_int(int(sources)), ctypes.c_int(int(time)))
+        return u
+
+    def fv_part(self, u, rec, nX, dt, dX, MP, mask, sources, time, part, flux=0):
+        """centers1/2 (part 1) or interfs1/2 (part 2, Riemann solver `flux`),
+        fv.cpp:11-279, on a copy of u."""
+        nX = np.asarray(nX, dtype=np.int32)
+        u = np.array(u, dtype=np.float64, copy=True).ravel()
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        dX = np.asarray(dX, dtype=np.float64)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.lib.ref_fv_part(_dp(u), _dp(rec), _ip(nX), ctypes.c_int(len(nX)), ctypes.c_double(dt),
+                             _dp(dX), ctypes.byref(MP), _u8(m), ctypes.c_int(int(sources)),
+                             ctypes.c_int(int(time)), ctypes.c_int(int(part)),
+                             ctypes.c_int(int(flux)))
         return u
 
     def ode_launcher(self, u, dt, MP):

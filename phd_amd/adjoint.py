@@ -160,14 +160,16 @@ try:
             uT = run.forward(spec["nsteps"], spec["K"])
             ctx.run = run
             ctx.has_kappa = kappa is not None
-            return torch.from_numpy(uT.copy())
+            ctx.shape = u0.shape
+            return torch.from_numpy(uT.copy()).reshape(u0.shape)
 
         @staticmethod
         def backward(ctx, grad_uT):
-            lam0, gmu = ctx.run.backward(grad_uT.detach().cpu().numpy())
+            lam0, gmu = ctx.run.backward(grad_uT.detach().cpu().contiguous().numpy().ravel())
             gkap = (torch.tensor(ctx.run.grad_kappa(), dtype=torch.float64)
                     if ctx.has_kappa else None)
             ctx.run.close()
-            return (torch.from_numpy(lam0), torch.tensor(gmu, dtype=torch.float64), None, gkap)
+            return (torch.from_numpy(lam0).reshape(ctx.shape), torch.tensor(gmu, dtype=torch.float64),
+                    None, gkap)
 except ImportError:  # the adjoint itself needs no torch
     StepsFunction = None

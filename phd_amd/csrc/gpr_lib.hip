@@ -1945,6 +1945,8 @@ int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt, const dou
   double *dwh = (double *)c->bufs["wh"].first;
   double *dqh = (double *)c->bufs["qh"].first;
   uint8_t *dm = (uint8_t *)c->bufs["run_mask"].first;
+  if (!dub || !dwh || !dm || (!qh_in && !dqh))
+    return fail(GPR_E_ARG, "the device run has no ADER stage buffers (gpr_dev_begin in the ADER FP64 form is needed)");
   rc = dispatch<BoundariesOp>(c, c, u_n, dub, c->run_nx, c->run_ny, c->run_bc);
   if (rc) return rc;
   rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
@@ -2067,6 +2069,10 @@ int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
   if (!c || !c->run_ready) return fail(GPR_E_ARG, "gpr_dev_begin has not been called");
   if (c->ndim != 2 || c->N != 3) return fail(GPR_E_UNSUPPORTED, "the adjoint is built for 2D, N = 3");
   if (nsteps < 1 || K < 1) return fail(GPR_E_ARG, "nsteps and K must be >= 1");
+  // the reverse sweep transposes the plain FP64 ADER step; a forward sweep
+  // in the split or FP32-bulk form would be a different map
+  if (c->opts.split != 0) return fail(GPR_E_UNSUPPORTED, "the adjoint covers the ADER step, not the split step");
+  if (c->opts.precision != 0) return fail(GPR_E_UNSUPPORTED, "the adjoint covers the FP64 step, not the FP32-bulk form");
   if (c->run_mat.p.BINGHAM || (c->run_mat.p.POWER_LAW && !c->run_mat.p.SOLID))
     return fail(GPR_E_UNSUPPORTED,
                 "the adjoint covers Newtonian fluids and elastic or power-law solids, not Bingham or power-law fluids");

@@ -36,6 +36,14 @@ def _in(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+def _size(a, n, name):
+    """The library reads and writes exactly n elements of a (sizes follow
+    from nX, N and V); any other size would overrun the caller's buffer."""
+    if np.size(a) != n:
+        raise ValueError(f"{name}: expected {n} elements for this grid, got {np.size(a)}")
+    return a
+
+
 def _mask_in(m):
     m = np.asarray(m)
     return np.ascontiguousarray(m.astype(np.uint8) if m.dtype != np.uint8 else m)
@@ -92,8 +100,7 @@ class _Module:
             derivative_values=self.derivative_values)
         weno = types.SimpleNamespace(
             weno_launcher=self.weno_launcher,
-            coefficient_matrices=lambda *a: [np.linalg.inv(self._consts()[k])
-                                             for k in ("mLinv", "mRinv", "mCLinv", "mCLinv")],
+            coefficient_matrices=self.coefficient_matrices,
             oscillation_indicator=lambda *a: self._consts()["sig"])
         dg = types.SimpleNamespace(predictor=self.predictor)
         fv = types.SimpleNamespace(Bint=self.Bint, D_RUS=self.D_RUS, fv_launcher=self.fv_launcher,
@@ -161,6 +168,31 @@ class _Module:
         nodes = self._consts()["nodes"] if nodes is None else nodes
         return np.array([[b.diff(1).eval(xi) for b in basis] for xi in nodes])
 
+    def coefficient_matrices(self, basis=None):
+        """[mL, mR, mCL, mCR] (weno_matrices.cpp:9-29): entry (i, j) is the
+        integral of basis polynomial j over cell i of the left, right,
+        centre-left (offset ceil((N-1)/2)) and centre-right (offset
+        floor((N-1)/2)) stencils.  The two centre stencils coincide for N = 3;
+        for N = 2, mCR is mR."""
+        basis = self.basis_polys() if basis is None else basis
+        N = self._N
+        cn2, fn2 = N // 2, (N - 1) // 2
+        offs = (N - 1, 0, cn2, fn2)  # stencil cell i covers [i - off, i - off + 1]
+        out = [np.zeros((N, N)) for _ in offs]
+        for j in range(N):
+            P = basis[j].intt()
+            for i in range(N):
+                for m, off in zip(out, offs):
+                    m[i, j] = P.eval(i - off + 1) - P.eval(i - off)
+        return out
+
+    # element counts of the stage buffers for a grid nX (C-order, V innermost)
+    def _cells(self, nX, ext=0):
+        return int(np.prod(np.asarray(nX, dtype=np.int64) + 2 * ext))
+
+    def _nodes(self, nX, time):
+        return self._N ** (len(nX) + (1 if time else 0))
+
     def _states(self, Q):
         Q = _in(Q, np.float64)
         if Q.size % self._V:
@@ -187,7 +219,7 @@ class _Module:
     def Bdot(self, ret, Q, x, d, MP):
         _ref(ret, np.float64, "ret")
         Q, n = self._states(Q)
-        x = _in(x, np.float64)
+        x = _size(_in(x, np.float64), Q.size, "x")
         out = np.zeros_like(Q)
         check(_lib.load().gpr_bdot(self.ctx().h, dptr(out), dptr(Q), dptr(x), n, int(d),
                                    par_ref(MP)))
@@ -210,7 +242,7 @@ class _Module:
 
     def D_RUS(self, qL, qR, d, MP):
         qL, n = self._states(qL)
-        qR = _in(qR, np.float64)
+        qR = _size(_in(qR, np.float64), qL.size, "qR")
         out = np.zeros_like(qL)
         check(_lib.load().gpr_d_rus(self.ctx().h, dptr(out), dptr(qL), dptr(qR), n, int(d),
                                     par_ref(MP)))
@@ -218,7 +250,7 @@ class _Module:
 
     def Bint(self, qL, qR, d, MP):
         qL, n = self._states(qL)
-        qR = _in(qR, np.float64)
+        qR = _size(_in(qR, np.float64), qL.size, "qR")
         out = np.zeros_like(qL)
         check(_lib.load().gpr_bint(self.ctx().h, dptr(out), dptr(qL), dptr(qR), n, int(d),
                                    par_ref(MP)))
@@ -231,51 +263,75 @@ class _Module:
     def boundaries(self, u, ub, nX, bcs):
         _ref(ub, np.float64, "ub")
         nX = _in(nX, np.int32)
-        check(_lib.load().gpr_boundaries(self.ctx(len(nX)).h, dptr(_in(u, np.float64)),
-                                         dptr(ub), iptr(nX), iptr(_in(bcs, np.int32))))
+        V = self._V
+        u = _size(_in(u, np.float64), self._cells(nX) * V, "u")
+        _size(ub, self._cells(nX, self._N) * V, "ub")
+        bcs = _size(_in(bcs, np.int32), 2 * len(nX), "bcs")
+        check(_lib.load().gpr_boundaries(self.ctx(len(nX)).h, dptr(u), dptr(ub), iptr(nX),
+                                         iptr(bcs)))
 
     def extend_mask(self, mask, maskb, nX):
         if not isinstance(maskb, np.ndarray) or maskb.dtype not in (np.bool_, np.uint8):
             raise TypeError("maskb: expected a bool array")
         nX = _in(nX, np.int32)
+        _size(maskb, self._cells(nX, 1), "maskb")
+        mask = _size(_mask_in(mask), self._cells(nX), "mask")
         out = np.zeros(maskb.size, dtype=np.uint8)
-        check(_lib.load().gpr_extend_mask(self.ctx(len(nX)).h, u8ptr(_mask_in(mask)),
-                                          u8ptr(out), iptr(nX)))
+        check(_lib.load().gpr_extend_mask(self.ctx(len(nX)).h, u8ptr(mask), u8ptr(out),
+                                          iptr(nX)))
         maskb[...] = out.reshape(maskb.shape).astype(maskb.dtype)
 
     # -------------------------------------------------------------- stages
     def weno_launcher(self, wh, ub, nX, MP):
         _ref(wh, np.float64, "wh")
         nX = _in(nX, np.int32)
-        check(_lib.load().gpr_weno_launcher(self.ctx(len(nX)).h, dptr(wh),
-                                            dptr(_in(ub, np.float64)), iptr(nX), par_ref(MP)))
+        V = self._V
+        _size(wh, self._cells(nX, 1) * self._nodes(nX, False) * V, "wh")
+        ub = _size(_in(ub, np.float64), self._cells(nX, self._N) * V, "ub")
+        check(_lib.load().gpr_weno_launcher(self.ctx(len(nX)).h, dptr(wh), dptr(ub), iptr(nX),
+                                            par_ref(MP)))
 
     def predictor(self, qh, wh, dt, dX, STIFF, STIFF_IG, MP, mask, iters=None):
         _ref(qh, np.float64, "qh")
         dX = _in(dX, np.float64)
         m = _mask_in(mask)
+        nd, V = len(dX), self._V
+        _size(qh, m.size * self._N ** (nd + 1) * V, "qh")
+        wh = _size(_in(wh, np.float64), m.size * self._N ** nd * V, "wh")
         it_ptr = None
         if iters is not None:
             _ref(iters, np.int32, "iters")
+            _size(iters, m.size, "iters")
             it_ptr = iptr(iters)
-        check(_lib.load().gpr_predictor(self.ctx(len(dX)).h, dptr(qh), dptr(_in(wh, np.float64)),
+        check(_lib.load().gpr_predictor(self.ctx(len(dX)).h, dptr(qh), dptr(wh),
                                         float(dt), dptr(dX), m.size, int(bool(STIFF)),
                                         int(bool(STIFF_IG)), par_ref(MP), u8ptr(m), it_ptr))
 
     def fv_launcher(self, u, rec, nX, dt, dX, SOURCES, TIME, FLUX, MP, mask):
         _ref(u, np.float64, "u")
         nX = _in(nX, np.int32)
-        check(_lib.load().gpr_fv_launcher(self.ctx(len(nX)).h, dptr(u), dptr(_in(rec, np.float64)),
+        rec, mask = self._fv_sizes(u, rec, nX, TIME, mask)
+        check(_lib.load().gpr_fv_launcher(self.ctx(len(nX)).h, dptr(u), dptr(rec),
                                           iptr(nX), float(dt), dptr(_in(dX, np.float64)),
                                           int(bool(SOURCES)), int(bool(TIME)), int(FLUX),
-                                          par_ref(MP), u8ptr(_mask_in(mask))))
+                                          par_ref(MP), u8ptr(mask)))
+
+    def _fv_sizes(self, u, rec, nX, TIME, mask):
+        """u: the cells; rec: the predictor's qh (TIME) or the WENO wh, and
+        mask, over the cells plus one ghost layer (fv.cpp:281-297)."""
+        V = self._V
+        _size(u, self._cells(nX) * V, "u")
+        rec = _size(_in(rec, np.float64), self._cells(nX, 1) * self._nodes(nX, TIME) * V, "rec")
+        mask = _size(_mask_in(mask), self._cells(nX, 1), "mask")
+        return rec, mask
 
     def _fv_part(self, fn, u, rec, nX, dt, dX, flag, TIME, MP, mask):
         _ref(u, np.float64, "u")
         nX = np.asarray(nX, dtype=np.int32)
-        check(fn(self.ctx(len(nX)).h, dptr(u), dptr(_in(rec, np.float64)), iptr(nX), float(dt),
+        rec, mask = self._fv_sizes(u, rec, nX, TIME, mask)
+        check(fn(self.ctx(len(nX)).h, dptr(u), dptr(rec), iptr(nX), float(dt),
                  dptr(_in(dX, np.float64)), int(flag), int(bool(TIME)), par_ref(MP),
-                 u8ptr(_mask_in(mask))))
+                 u8ptr(mask)))
 
     def centers1(self, u, rec, nx, dt, dx, SOURCES, TIME, MP, mask):
         """fv.cpp:46-54, u in place."""
@@ -290,18 +346,20 @@ class _Module:
     def interfs1(self, u, rec, nx, dt, dx, TIME, FLUX, MP, mask):
         """fv.cpp:151-160, u in place."""
         _ref(u, np.float64, "u")
-        check(_lib.load().gpr_fv_interfs(self.ctx(1).h, dptr(u), dptr(_in(rec, np.float64)),
+        rec, mask = self._fv_sizes(u, rec, [nx], TIME, mask)
+        check(_lib.load().gpr_fv_interfs(self.ctx(1).h, dptr(u), dptr(rec),
                                          iptr(np.array([nx], np.int32)), float(dt),
                                          dptr(np.array([dx], np.float64)), int(bool(TIME)),
-                                         int(FLUX), par_ref(MP), u8ptr(_mask_in(mask))))
+                                         int(FLUX), par_ref(MP), u8ptr(mask)))
 
     def interfs2(self, u, rec, nx, ny, dt, dx, dy, TIME, FLUX, MP, mask):
         """fv.cpp:270-279, u in place."""
         _ref(u, np.float64, "u")
-        check(_lib.load().gpr_fv_interfs(self.ctx(2).h, dptr(u), dptr(_in(rec, np.float64)),
+        rec, mask = self._fv_sizes(u, rec, [nx, ny], TIME, mask)
+        check(_lib.load().gpr_fv_interfs(self.ctx(2).h, dptr(u), dptr(rec),
                                          iptr(np.array([nx, ny], np.int32)), float(dt),
                                          dptr(np.array([dx, dy], np.float64)), int(bool(TIME)),
-                                         int(FLUX), par_ref(MP), u8ptr(_mask_in(mask))))
+                                         int(FLUX), par_ref(MP), u8ptr(mask)))
 
     def timestep(self, grids, masks, dX, CFL, t, tf, count, MPs, nmat, nX=None):
         """iterator.cpp:54-83.  The reference recovers the cell count from the
@@ -322,10 +380,14 @@ class _Module:
     def ader_stepper(self, u, ub, nX, dt, dX, STIFF, FLUX, MP, mask):
         _ref(u, np.float64, "u")
         nX = _in(nX, np.int32)
-        check(_lib.load().gpr_ader_stepper(self.ctx(len(nX)).h, dptr(u), dptr(_in(ub, np.float64)),
+        V = self._V
+        _size(u, self._cells(nX) * V, "u")
+        ub = _size(_in(ub, np.float64), self._cells(nX, self._N) * V, "ub")
+        mask = _size(_mask_in(mask), self._cells(nX, 1), "mask")
+        check(_lib.load().gpr_ader_stepper(self.ctx(len(nX)).h, dptr(u), dptr(ub),
                                            iptr(nX), float(dt), dptr(_in(dX, np.float64)),
                                            int(bool(STIFF)), int(FLUX), par_ref(MP),
-                                           u8ptr(_mask_in(mask))))
+                                           u8ptr(mask)))
 
     def split_stepper(self, u, ub, nX, dt, dX, HALF_STEP, FLUX, MP, mask):
         """steppers.cpp:50-66; u and ub are both advanced in place (ub by the
@@ -333,10 +395,14 @@ class _Module:
         _ref(u, np.float64, "u")
         _ref(ub, np.float64, "ub")
         nX = _in(nX, np.int32)
+        V = self._V
+        _size(u, self._cells(nX) * V, "u")
+        _size(ub, self._cells(nX, self._N) * V, "ub")
+        mask = _size(_mask_in(mask), self._cells(nX, 1), "mask")
         check(_lib.load().gpr_split_stepper(self.ctx(len(nX)).h, dptr(u), dptr(ub), iptr(nX),
                                             float(dt), dptr(_in(dX, np.float64)),
                                             int(bool(HALF_STEP)), int(FLUX), par_ref(MP),
-                                            u8ptr(_mask_in(mask))))
+                                            u8ptr(mask)))
 
     def split_stepper_para(self, u, ub, nX, dt, dX, HALF_STEP, FLUX, MP, mask):
         """steppers.cpp:68-94: the same step on a private copy of ub (the
@@ -368,6 +434,7 @@ class _Module:
         _ref(wh, np.float64, "wh")
         dX = _in(dX, np.float64)
         m = _mask_in(mask)
+        _size(wh, m.size * self._N ** len(dX) * self._V, "wh")
         check(_lib.load().gpr_midstepper(self.ctx(len(dX)).h, dptr(wh), float(dt), dptr(dX),
                                          m.size, par_ref(MP), u8ptr(m)))
 
@@ -395,6 +462,8 @@ class _Module:
         reference's default-constructed Vecs)."""
         _ref(u, np.float64, "u")
         nX = _in(nX, np.int32)
+        _size(u, self._cells(nX) * self._V, "u")
+        boundaryTypes = _size(_in(boundaryTypes, np.int32), 2 * len(nX), "boundaryTypes")
         snaps = np.zeros((100, u.size))
         nsnap = np.zeros(1, dtype=np.int32)
         steps = np.zeros(1, dtype=np.int64)

@@ -66,6 +66,14 @@ class Par(ctypes.Structure):
         ctypes.memmove(ctypes.byref(c), ctypes.byref(self), ctypes.sizeof(Par))
         return c
 
+    @classmethod
+    def from_dict(cls, d):
+        """The inverse of as_dict."""
+        MP = cls()
+        for k, v in d.items():
+            setattr(MP, k, v)
+        return MP
+
     def as_dict(self):
         d = {f: getattr(self, f) for f in _DOUBLE_FIELDS}
         d["dp"] = list(self.dp)

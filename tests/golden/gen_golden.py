@@ -383,6 +383,41 @@ def split_steps(tag, thermal, ndim, n, mu, nsteps):
     save(f"split_{tag}.npz", **out)
 
 
+def fv_parts():
+    """centers2/interfs2 and centers1/interfs1 (fv.cpp:11-279), each half on its
+    own, on the stage fixtures' own inputs: the space-time predictor result
+    (TIME = 1, with and without sources) and the WENO reconstruction at one
+    time level (TIME = 0), plus the no-FMA build's values (the rounding
+    spread of the reference's own evaluation order)."""
+    from phd_amd.params import Par
+    out = {}
+    for tag, thermal in (("2d_dsl12", False), ("1d_sod48_th", True)):
+        g = np.load(os.path.join(HERE, f"stages_{tag}.npz"))
+        MP = Par.from_dict(json.loads(str(g["par"])))
+        R = RefSolver(3, thermal)
+        R2 = RefSolver(3, thermal, nofma=True)
+        nX, dX, dt, mask = g["nX"], g["dX"], float(g["dt"]), g["mask"]
+        # 1% noise on every component: the stage data itself is relaxed
+        # (A = rho^(1/3) I), where the sources and B.grad q vanish
+        rng = np.random.default_rng(47)
+        qh = g["qh"] * (1 + 0.01 * rng.standard_normal(g["qh"].shape))
+        wh = g["wh"] * (1 + 0.01 * rng.standard_normal(g["wh"].shape))
+        out[f"{tag}_qh"], out[f"{tag}_wh"] = qh, wh
+        for name, rec, src, time, part in (
+                ("centers_src", qh, 1, 1, 1), ("centers_nosrc", qh, 0, 1, 1),
+                ("interfs", qh, 0, 1, 2), ("centers_t0", wh, 0, 0, 1),
+                ("interfs_t0", wh, 0, 0, 2)):
+            a = R.fv_part(g["u"], rec, nX, dt, dX, MP, mask, src, time, part)
+            b = R2.fv_part(g["u"], rec, nX, dt, dX, MP, mask, src, time, part)
+            out[f"{tag}_{name}"] = a
+            out[f"{tag}_{name}_nofma"] = b
+        # the two halves compose to fv_launcher (fv.cpp:281-297)
+        both = R.fv_part(R.fv_part(g["u"], g["qh"], nX, dt, dX, MP, mask, 1, 1, 1), g["qh"], nX,
+                         dt, dX, MP, mask, 1, 1, 2)
+        assert relerr(both, g["u_fv"]) < 1e-14, tag
+    save("fv_parts.npz", **out)
+
+
 if __name__ == "__main__":
     constants()
     pointwise()

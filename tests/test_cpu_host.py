@@ -213,3 +213,68 @@ def test_basis_polys_mirror():
     assert np.allclose(p.diff(1).coef, [2, 6]) and np.allclose(p.diff(2).coef, [6])
     assert p.eval(2.0) == 17.0
     assert np.allclose((p * poly([0.0, 1.0])).coef, [0, 1, 2, 3])
+
+
+def test_coefficient_matrices_mirror():
+    """solvers.weno.coefficient_matrices (weno_matrices.cpp:9-29) returns
+    [mL, mR, mCL, mCR].  The first three are the inverses of the reference's
+    own tables (tests/golden/constants.npz).  mCR integrates over the stencil
+    offset floor((N-1)/2): it equals mCL at N = 3 and mR at N = 2, where the
+    two centre stencils differ.  Each matrix maps the basis's coefficients
+    of the constant 1 (all ones) to cell averages of 1."""
+    import os
+    from phd_amd.gprpy import load
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "constants.npz"))
+    for N in (2, 3):
+        mL, mR, mCL, mCR = load(N, False).solvers.weno.coefficient_matrices()
+        for m, k in ((mL, "mLinv"), (mR, "mRinv"), (mCL, "mCLinv")):
+            assert np.allclose(m @ g[f"n{N}_{k}"], np.eye(N), rtol=0, atol=1e-12), (N, k)
+        for m in (mL, mR, mCL, mCR):
+            assert np.allclose(m @ np.ones(N), np.ones(N), rtol=0, atol=1e-13)
+        assert np.allclose(mCR, mCL if N == 3 else mR, rtol=0, atol=1e-14)
+        if N == 2:
+            assert not np.allclose(mCR, mCL)
+
+
+def test_gprpy_rejects_mismatched_sizes():
+    """Arrays whose size does not match nX (and N, V) raise ValueError before
+    the library is called, so the library never copies past a caller's
+    buffer.  Runs without a GPU: the checks come before any device call."""
+    from phd_amd.gprpy import load
+    from phd_amd import problems
+    G = load(3, False)
+    nx, ny, V = 6, 5, 14
+    nX = np.array([nx, ny], np.int32)
+    dX = np.array([1.0 / nx, 1.0 / ny])
+    _, MP, _, _, _ = problems.double_shear_2d(4, μ=0.1)
+    ub = np.zeros((nx + 6) * (ny + 6) * V)
+    wh = np.zeros((nx + 2) * (ny + 2) * 9 * V)
+    qh = np.zeros((nx + 2) * (ny + 2) * 27 * V)
+    u = np.zeros(nx * ny * V)
+    mask = np.ones((nx + 2) * (ny + 2), bool)
+    with pytest.raises(ValueError):
+        G.solvers.weno.weno_launcher(wh[:-1].copy(), ub, nX, MP)
+    with pytest.raises(ValueError):
+        G.solvers.weno.weno_launcher(wh, ub[:-V], nX, MP)
+    with pytest.raises(ValueError):
+        G.solvers.dg.predictor(qh[:-1].copy(), wh, 1e-3, dX, False, False, MP, mask)
+    with pytest.raises(ValueError):
+        G.solvers.dg.predictor(qh, wh[:-1], 1e-3, dX, False, False, MP, mask)
+    with pytest.raises(ValueError):
+        G.solvers.fv.fv_launcher(u, qh[:-1], nX, 1e-3, dX, True, True, 0, MP, mask)
+    with pytest.raises(ValueError):
+        G.solvers.fv.fv_launcher(u, wh, nX, 1e-3, dX, True, True, 0, MP, mask)  # TIME needs qh
+    with pytest.raises(ValueError):
+        G.solvers.fv.centers2(u[:-V].copy(), qh, nx, ny, 1e-3, dX[0], dX[1], True, True, MP, mask)
+    with pytest.raises(ValueError):
+        G.solvers.fv.interfs2(u, qh, nx, ny, 1e-3, dX[0], dX[1], True, 0, MP, mask[:-1])
+    with pytest.raises(ValueError):
+        G.solvers.ader_stepper(u, ub[:-1], nX, 1e-3, dX, False, 0, MP, mask)
+    with pytest.raises(ValueError):
+        G.boundaries(u[:-1], ub, nX, np.array([1, 1, 1, 1], np.int32))
+    with pytest.raises(ValueError):
+        G.solvers.iterator(u[:-V].copy(), 0.1, nX, dX, 0.8, np.array([1, 1, 1, 1], np.int32),
+                           False, False, False, 0, [MP], 1.0)
+    with pytest.raises(ValueError):
+        G.solvers.fv.D_RUS(np.zeros(2 * V), np.zeros(V), 0, MP)

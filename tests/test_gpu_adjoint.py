@@ -162,6 +162,39 @@ def test_reset_in_forward_sweep_is_refused():
     run.close()
 
 
+@pytest.mark.parametrize("opt", ["split", "precision"])
+def test_split_and_fp32_forward_sweeps_are_refused(opt):
+    """The reverse sweep transposes the plain FP64 ADER step, so a forward
+    sweep in the split-WENO or FP32-bulk form is refused with
+    GPR_E_UNSUPPORTED before any step runs."""
+    from phd_amd._lib import GprError
+    from phd_amd.adjoint import AdjointRun
+    u0, MP, nX, dX, bcs = case()
+    run = AdjointRun(u0, MP, nX, dX, bcs, **{opt: 1})
+    with pytest.raises(GprError) as e:
+        run.forward(3, 2)
+    assert e.value.code == -2
+    assert np.array_equal(run.run.host(), u0)  # no step was taken
+    run.close()
+
+
+def test_torch_function_keeps_the_input_shape(adj):
+    """A shaped u_0 tensor (nx, ny, V) gives u_T and d L / d u_0 of that shape."""
+    import torch
+    from phd_amd.adjoint import StepsFunction
+    u0, MP = adj["u0"], adj["MP"]
+    spec = {"MP": MP, "nX": [16, 16], "dX": [1 / 16, 1 / 16],
+            "bcs": [1, 1, 1, 1], "nsteps": NSTEPS, "K": K}
+    u0_t = torch.tensor(u0.reshape(16, 16, 14), requires_grad=True)
+    mu_t = torch.tensor(MP.mu, dtype=torch.float64, requires_grad=True)
+    uT = StepsFunction.apply(u0_t, mu_t, spec)
+    assert uT.shape == (16, 16, 14)
+    L = ((uT[..., 0].reshape(-1) - torch.from_numpy(adj["target"])) ** 2).sum()
+    L.backward()
+    assert u0_t.grad.shape == (16, 16, 14)
+    assert np.allclose(u0_t.grad.numpy().ravel(), adj["lam0"], rtol=1e-12, atol=1e-14)
+
+
 def test_transposed_solver_forms_agree():
     """The transposed predictor solve started in the transposed Picard form
     and started in the preconditioned form reach the same fixed point: the

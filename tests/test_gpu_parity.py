@@ -204,6 +204,44 @@ def test_fv_centers_and_interfaces(gp, case, time, sources):
     assert relerr(ui, u) > 1e-8  # (the 2D start has A = I: its cell integrals vanish)
 
 
+@pytest.mark.parametrize("tag", ["2d_dsl12", "1d_sod48_th"])
+def test_fv_halves_vs_reference(gp, tag):
+    """solvers.fv.centers1/2 and interfs1/2 on the device, each half against
+    the reference's own half (tests/golden/fv_parts.npz: the stage data with
+    1% noise, TIME = 1 with and without sources, and TIME = 0).  Each
+    increment to u is compared on its own scale, within 1e-12 or 4x the
+    reference's FMA/no-FMA spread where that is larger."""
+    g = _golden(f"stages_{tag}.npz")
+    f = _golden("fv_parts.npz")
+    thermal = tag.endswith("_th")
+    G = gp(3, thermal)
+    MP = _par(g["par"])
+    nX, dX, dt, m, u = g["nX"], g["dX"], float(g["dt"]), g["mask"].astype(bool), g["u"]
+    qh, wh = f[f"{tag}_qh"], f[f"{tag}_wh"]
+    for name, rec, src, time, part in (("centers_src", qh, 1, 1, 1), ("centers_nosrc", qh, 0, 1, 1),
+                                       ("interfs", qh, 0, 1, 2), ("centers_t0", wh, 0, 0, 1),
+                                       ("interfs_t0", wh, 0, 0, 2)):
+        got = u.copy()
+        if len(nX) == 2:
+            if part == 1:
+                G.solvers.fv.centers2(got, rec, nX[0], nX[1], dt, dX[0], dX[1], src, time, MP, m)
+            else:
+                G.solvers.fv.interfs2(got, rec, nX[0], nX[1], dt, dX[0], dX[1], time, 0, MP, m)
+        else:
+            if part == 1:
+                G.solvers.fv.centers1(got, rec, nX[0], dt, dX[0], src, time, MP, m)
+            else:
+                G.solvers.fv.interfs1(got, rec, nX[0], dt, dX[0], time, 0, MP, m)
+        ref = f[f"{tag}_{name}"]
+        scale = np.max(np.abs(ref - u))
+        if scale == 0:
+            assert np.array_equal(got, ref), name
+            continue
+        floor = np.max(np.abs(f[f"{tag}_{name}_nofma"] - ref)) / scale
+        err = np.max(np.abs(got - ref)) / scale
+        assert err <= max(1e-12, 4 * floor), (name, err, floor)
+
+
 def test_host_calls_do_not_disturb_a_device_run(gp):
     """A host-pointer stage call on the context of a device-resident run (on
     another grid, with an all-false mask) between its steps leaves the run's
@@ -335,21 +373,24 @@ def test_iterator_global_reset(gp):
     assert relerr(ug, uo) < 1e-11
 
 
-def test_full_size_512(gp):
+@pytest.mark.parametrize("bailout", [5, 0])
+def test_full_size_512(gp, bailout):
     """The benchmark workload at its full size (2D double shear layer 512x512,
-    periodic, mu = 1e-3, c_s = 8) through the device-resident loop: the first
-    step against the oracle; steps 7 and 8 (the first full-dt steps, where
-    Picard diverges in 10-15% of cells and the stiff fallback runs) against the
-    oracle from the device's own state; a repeated run is bit-identical."""
-    import os
+    periodic, mu = 1e-3, c_s = 8) through the device-resident loop, with the
+    benchmark's Picard bailout (5) and with the reference's rule (0: Picard
+    runs to DG_IT = 50): the first step against the oracle; steps 7 and 8
+    (the first full-dt steps, where Picard diverges in 10-15% of cells and
+    the stiff fallback runs) against the oracle from the device's own state;
+    with bailout 5 also step 21, inside the benchmark's timed window; a
+    repeated run is bit-identical."""
     from phd_amd import problems
     from phd_amd.runner import DeviceRun
-    print("full-size 512x512 case", flush=True)
+    print(f"full-size 512x512 case, bailout {bailout}", flush=True)
     n = 512
     u0, MP, _, dX, bcs = problems.double_shear_2d(n)
     nX = [n, n]
-    O = oracle(3, False, 2, bailout=3)
-    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=3)
+    O = oracle(3, False, 2, bailout=bailout)
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=bailout)
     run.steps(1)
     u1 = run.host()
     uo, t, c, dt, rc, st = O.step(u0.ravel(), nX, dX, 0.8, bcs, 0.0, 0, 1e300, MP)
@@ -363,9 +404,19 @@ def test_full_size_512(gp):
     uo, t, c, dt, rc, st = O.step(u7, nX, dX, 0.8, bcs, s["t"], s["count"], 1e300, MP)
     assert rc == 0 and st["fallback_cells"] > 0.05 * n * n
     assert relerr(u8, uo) < TOL
-    run2 = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=3)
+    if bailout == 5:
+        run.steps(12)
+        s = run.status()
+        u20 = run.host()
+        run.steps(1)
+        uo, t, c, dt, rc, st = O.step(u20, nX, dX, 0.8, bcs, s["t"], s["count"], 1e300, MP)
+        assert rc == 0 and st["fallback_cells"] > 0.05 * n * n
+        assert relerr(run.host(), uo) < TOL
+    run.close()
+    run2 = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=bailout)
     run2.steps(8)
     assert np.array_equal(run2.host(), u8)
+    run2.close()
 
 
 @pytest.mark.parametrize("V", [14, 17])

@@ -232,3 +232,34 @@ def test_boundaries_vs_reference(V):
                 assert relerr(ub, ref) < 1e-15, (k, key)
             else:
                 assert np.array_equal(ub, ref), (k, key)
+
+
+@pytest.mark.parametrize("tag,thermal,ndim", [("2d_dsl12", False, 2), ("1d_sod48_th", True, 1)])
+def test_fv_halves_vs_reference(tag, thermal, ndim):
+    """centers1/2 and interfs1/2 (fv.cpp:11-279) one at a time, against the
+    reference's own halves (tests/golden/fv_parts.npz): with the predictor's
+    space-time result (TIME = 1, with and without sources) and with one time
+    level (TIME = 0), both with 1% noise so that sources and B.grad q are
+    not zero.  Each half's increment to u is compared on its own scale, so a
+    term moved from one half to the other fails."""
+    g = load(f"stages_{tag}.npz")
+    f = load("fv_parts.npz")
+    O = oracle(3, thermal, ndim)
+    MP = par(g["par"])
+    nX, dX, dt, m, u = g["nX"], g["dX"], float(g["dt"]), g["mask"], g["u"]
+    qh, wh = f[f"{tag}_qh"], f[f"{tag}_wh"]  # the stage data with 1% noise
+    cases = (("centers_src", qh, 1, 1, 1), ("centers_nosrc", qh, 0, 1, 1),
+             ("interfs", qh, 0, 1, 2), ("centers_t0", wh, 0, 0, 1), ("interfs_t0", wh, 0, 0, 2))
+    for name, rec, src, time, part in cases:
+        ref = f[f"{tag}_{name}"]
+        got = O.fv_parts(u, rec, nX, dt, dX, MP, m, src, time, part)
+        inc_ref, inc = ref - u, got - u
+        scale = np.max(np.abs(inc_ref))
+        if scale == 0:  # e.g. B.grad q in 1D flow along x: the half adds exactly 0
+            assert np.array_equal(got, ref), name
+            continue
+        floor = np.max(np.abs(f[f"{tag}_{name}_nofma"] - ref)) / scale
+        err = np.max(np.abs(inc - inc_ref)) / scale
+        assert err <= max(1e-12, 4 * floor), (name, err, floor)
+    # the centre half with and without sources differ by the source term alone
+    assert not np.allclose(f[f"{tag}_centers_src"], f[f"{tag}_centers_nosrc"], rtol=0, atol=1e-15)
