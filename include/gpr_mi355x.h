@@ -142,6 +142,25 @@ int gpr_timestep(gpr_ctx *ctx, double *dt_out, const double *u, const int32_t *n
 /* solvers.ader_stepper(u, ub, nX, dt, dX, STIFF, FLUX, MP, mask) (steppers.h:7-8) */
 int gpr_ader_stepper(gpr_ctx *ctx, double *u, const double *ub, const int32_t *nX, double dt,
                      const double *dX, int stiff, int flux, const gpr_par *MP, const uint8_t *mask);
+/* gpr_ader_stepper on device buffers without allocating.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * d_u (in/out, [cells][V]), d_ub ([ghost-extended cells][V]) and d_mask
+ * ([cells + 1 ghost layer], uint8) are device pointers; nX, dX and MP are
+ * host pointers.  The step's work buffers (WENO x-pass, wh, qh, face sums,
+ * failure counter) live in a file-scope device array of
+ * GPR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a grid
+ * needing more (2D, N = 3, V = 14: about 4.9 KB per cell, so up to about
+ * 13,000 cells) fails with GPR_E_ARG before any launch.  The call is
+ * asynchronous on the context's stream; the number of cells whose predictor
+ * converged in neither phase is written to the device int *d_nfail.
+ * Every call on a GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (contexts on different streams) race and must be serialised by the
+ * caller.  Values equal gpr_ader_stepper's for the same inputs. */
+#define GPR_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gpr_ader_stepper_noalloc(gpr_ctx *ctx, double *d_u, const double *d_ub, const int32_t *nX,
+                             double dt, const double *dX, int stiff, int flux, const gpr_par *MP,
+                             const uint8_t *d_mask, int32_t *d_nfail);
 /* solvers.split_stepper(u, ub, nX, dt, dX, HALF_STEP, FLUX, MP, mask)
  * (steppers.h:13-14, steppers.cpp:50-66): half-step source ODE on u and on ub
  * (both written in place), WENO of ub, the midstepper if HALF_STEP, the FV

@@ -34,7 +34,7 @@ SYMBOLS = [
     "gpr_batch_status", "gpr_dev_steps_dt", "gpr_adj_forward", "gpr_adj_backward",
     "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats", "gpr_adj_grad_kappa",
     "gpr_split_stepper", "gpr_ode_launcher", "gpr_analytic_solver", "gpr_midstepper",
-    "gpr_fv_centers", "gpr_fv_interfs",
+    "gpr_fv_centers", "gpr_fv_interfs", "gpr_ader_stepper_noalloc",
 ]
 
 
@@ -85,6 +85,7 @@ ARGTYPES = {
     "gpr_fv_launcher": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _i, _M, _Up],
     "gpr_timestep": [_P, _Dp, _Dp, _Ip, _Dp, _d, _d, _d, _l, _M],
     "gpr_ader_stepper": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
+    "gpr_ader_stepper_noalloc": [_P, _P, _P, _Ip, _d, _Dp, _i, _i, _M, _P, _P],
     "gpr_split_stepper": [_P, _Dp, _Dp, _Ip, _d, _Dp, _i, _i, _M, _Up],
     "gpr_ode_launcher": [_P, _Dp, _l, _d, _M],
     "gpr_analytic_solver": [_P, _Dp, _l, _d, _M, _i],

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -179,12 +180,53 @@ int make_mat(const gpr_par *p, Mat &m) {
 
 }  // namespace
 
+// ------------------------------------------------- allocation-free scratch
+// Device scratch of gpr_ader_stepper_noalloc: the WENO x-pass, wh, qh, the
+// face sums and a failure counter of one step, carved from one file-scope
+// array so that the call allocates nothing.  Each device holds its own copy
+// (GPR_NOALLOC_SCRATCH_BYTES) from the library's load until the process
+// ends.  The host may not use the array's name as an address: its device
+// address is taken once per device with hipGetSymbolAddress (at context
+// creation) and kept in g_scratch_dev[device].
+static __device__ __align__(256) unsigned char scratch[GPR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[MAX_DEVICES] = {};
+
+// the device address of `scratch` on `device` (the current device)
+int scratch_setup(int device, void **out) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(GPR_E_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GPR_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = p;
+  }
+  *out = g_scratch_dev[device];
+  return GPR_OK;
+}
+
+__global__ void k_copy_int(int32_t *dst, const int *src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+// double *p[0, n) = 0 as a kernel (graph capture records a kernel node)
+__global__ void k_zero_doubles(double *p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+}  // namespace
+
 // ---------------------------------------------------------------- context
 struct gpr_ctx {
   int N, ndim, thermal, V;
   gpr_solver_opts opts;
   hipStream_t stream = nullptr;
   int device = 0;
+  void *scratch = nullptr;  // this device's address of the no-alloc scratch array
   std::map<std::string, std::pair<void *, size_t>> bufs;
   StepState *d_st = nullptr;
   int *d_fail = nullptr;
@@ -274,7 +316,7 @@ template <int N, bool TH, int ND> struct BoundariesOp {
 // WENO in the type R (double: the plain method; float: the FP32-bulk form),
 // ub read in double, wh written in R.
 template <typename R, int N, bool TH, int ND>
-int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny) {
+int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny, R *ux_given = nullptr) {
   constexpr int V = Dims<TH>::V;
   const Basis &K = basis(N);
   if (ND == 1) {
@@ -284,7 +326,7 @@ int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny) {
   } else {
     int rc;
     long nux = (long)(nx + 2) * (ny + 2 * N) * N * V;
-    R *ux = (R *)buf(c, sizeof(R) == 8 ? "ux" : "uxf", nux * sizeof(R), rc);
+    R *ux = ux_given ? ux_given : (R *)buf(c, sizeof(R) == 8 ? "ux" : "uxf", nux * sizeof(R), rc);
     if (!ux) return rc;
     long t1 = (long)(nx + 2) * (ny + 2 * N) * V;
     hipLaunchKernelGGL((k_weno_x<R, N, V>), dim3(grid_for(t1, 256)), dim3(256), 0, c->stream, d_ub, ux, nx, ny, K);
@@ -296,8 +338,8 @@ int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny) {
 }
 
 template <int N, bool TH, int ND> struct WenoOp {
-  static int run(gpr_ctx *c, const double *d_ub, double *d_wh, int nx, int ny) {
-    return weno_run<double, N, TH, ND>(c, d_ub, d_wh, nx, ny);
+  static int run(gpr_ctx *c, const double *d_ub, double *d_wh, int nx, int ny, double *ux = nullptr) {
+    return weno_run<double, N, TH, ND>(c, d_ub, d_wh, nx, ny, ux);
   }
 };
 
@@ -687,6 +729,10 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
     gpr_ctx_destroy(c);
     return rc;
   }
+  if (int rc = scratch_setup(c->device, &c->scratch)) {
+    gpr_ctx_destroy(c);
+    return rc;
+  }
   *ctx = c;
   return GPR_OK;
 }
@@ -1050,6 +1096,66 @@ int gpr_ader_stepper(gpr_ctx *c, double *u, const double *ub, const int32_t *nX,
   HIP_TRY(hipMemcpyAsync(&nfail, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (nfail) return fail(GPR_E_PREDICTOR, std::to_string(nfail) + " cell(s): predictor converged in neither phase");
+  return GPR_OK;
+}
+
+int gpr_ader_stepper_noalloc(gpr_ctx *c, double *d_u, const double *d_ub, const int32_t *nX, double dt,
+                             const double *dX, int stiff, int flux, const gpr_par *MP, const uint8_t *d_mask,
+                             int32_t *d_nfail) {
+  if (!c || !d_u || !d_ub || !dX || !d_mask || !d_nfail) return fail(GPR_E_ARG, "null argument");
+  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  int rc = check_nX(c, nX);
+  if (rc) return rc;
+  Mat m;
+  rc = make_mat(MP, m);
+  if (rc) return rc;
+  if (!c->scratch) return fail(GPR_E_HIP, "the context has no scratch address");
+  const int ny = c->ndim == 2 ? nX[1] : 1;
+  const long ne = ext_cells(c, nX, 1);
+  const int NS = c->ndim == 2 ? c->N * c->N : c->N;
+  const long nw = ne * NS * c->V, nq = ne * NS * c->N * c->V;
+  const long nux = c->ndim == 2 ? (long)(nX[0] + 2) * (ny + 2 * c->N) * c->N * c->V : 0;
+  const long nfx = (long)(nX[0] + 1) * ny * 2 * c->V;
+  const long nfy = c->ndim == 2 ? (long)nX[0] * (ny + 1) * 2 * c->V : 2 * c->V;
+  // carve the scratch into 256-byte aligned pieces
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t o_fail = take(sizeof(int)), o_ux = take(nux * 8), o_wh = take(nw * 8), o_qh = take(nq * 8),
+               o_fx = take(nfx * 8), o_fy = take(nfy * 8);
+  if (off > (size_t)GPR_NOALLOC_SCRATCH_BYTES)
+    return fail(GPR_E_ARG, "grid too large for the allocation-free stepper: it needs " + std::to_string(off) +
+                               " bytes of scratch, the maximum is " + std::to_string(GPR_NOALLOC_SCRATCH_BYTES));
+  unsigned char *base = (unsigned char *)c->scratch;
+  int *dfail = (int *)(base + o_fail);
+  double *dux = (double *)(base + o_ux), *dwh = (double *)(base + o_wh), *dqh = (double *)(base + o_qh);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, dfail);
+  hipLaunchKernelGGL(k_zero_doubles, dim3(grid_for(nq, 256)), dim3(256), 0, c->stream, dqh, nq);
+  HIP_TRY(hipGetLastError());
+  rc = dispatch<WenoOp>(c, c, d_ub, dwh, nX[0], ny, dux);
+  if (rc) return rc;
+  PredArgs pa = pred_args(c, m, dt, dX);
+  if (stiff) pa.mode = 1;
+  pa.qh = dqh;
+  pa.wh = dwh;
+  pa.mask = d_mask;
+  pa.ncell = ne;
+  pa.fail = dfail;
+  rc = dispatch<PredictorOp>(c, c, pa);
+  if (rc) return rc;
+  FvArgs fa = fv_args(c, m, dt, dX, nX);
+  fa.u = d_u;
+  fa.qh = dqh;
+  fa.mask = d_mask;
+  fa.fx = (double *)(base + o_fx);
+  fa.fy = (double *)(base + o_fy);
+  rc = dispatch<FvOp>(c, c, fa);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, c->stream, d_nfail, (const int *)dfail);
+  HIP_TRY(hipGetLastError());
   return GPR_OK;
 }
 
