@@ -30,8 +30,27 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# units compiled more than once, with these macro sets (one object each)
+VARIANTS = {
+    "gpr_adj_heavy.hip": [{"GPR_ADJ_DUAL": d, "GPR_ADJ_TH": t} for d in (0, 1) for t in (0, 1)],
+    "gpr_pred.hip": [{"GPR_UNIT_N": n} for n in (2, 3)],
+    "gpr_fv.hip": [{"GPR_UNIT_N": n, "GPR_UNIT_ND": d} for n in (2, 3) for d in (1, 2)],
+}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def units():
+    """(source, object name, extra flags) for every object of the library"""
+    out = []
+    for src in sources():
+        base = os.path.basename(src)
+        for defs in VARIANTS.get(base, [{}]):
+            tag = "".join(f"_{k.split('_')[-1].lower()}{v}" for k, v in defs.items())
+            out.append((src, base + tag + ".o", [f"-D{k}={v}" for k, v in defs.items()]))
+    return out
 
 
 def build(verbose=False, jobs=None):
@@ -39,9 +58,9 @@ def build(verbose=False, jobs=None):
     procs = []
     bdir = os.path.join(HERE, "_build", os.path.splitext(os.path.basename(LIB))[0])
     os.makedirs(bdir, exist_ok=True)
-    for src in sources():
-        obj = os.path.join(bdir, os.path.basename(src) + ".o")
-        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", src, "-o", obj]
+    for src, oname, extra in units():
+        obj = os.path.join(bdir, oname)
+        cmd = [hipcc()] + HIPCC_FLAGS + extra + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), src))

@@ -1237,4 +1237,11 @@ __global__ void ka_boundaries(AdjArgs a) {
   }
 }
 
+// The transposed faces, cells and predictor of one (DUAL, TH) pair are
+// compiled in a translation unit of their own (gpr_adj_heavy.hip, built once
+// per pair), which also holds its own copy of the basis table.
+template <bool DUAL, bool TH>
+hipError_t adj_heavy_launch(dim3 gf, dim3 gc, hipStream_t s, const AdjArgs &a, long ne);
+template <bool DUAL, bool TH> hipError_t adj_heavy_upload(const Basis *hb);
+
 }  // namespace gpr

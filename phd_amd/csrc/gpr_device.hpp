@@ -38,8 +38,9 @@ struct Basis {
 // The basis tables for N = 2 and N = 3 in the constant segment, written at
 // context creation.  Kernels index them per lane; the same per-lane index
 // into a table held in a by-value kernel argument would make the compiler
-// copy the whole argument to the stack.
-__constant__ Basis c_basis[2];
+// copy the whole argument to the stack.  Each translation unit that launches
+// kernels has its own copy (internal linkage), written by that unit.
+static __constant__ Basis c_basis[2];
 template <int N> __device__ __forceinline__ const Basis &basis_dev() { return c_basis[N - 2]; }
 
 // *p if p is non-null, else v.  The load is atomic so that the compiler cannot

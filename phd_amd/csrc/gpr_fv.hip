@@ -1,0 +1,48 @@
+// Launches of the FV corrector's face and cell kernels (gpr_kernels.hpp) for
+// N = GPR_UNIT_N in GPR_UNIT_ND dimensions; phd_amd/build.py compiles this
+// unit once per pair.
+#include <hip/hip_runtime.h>
+
+#include "gpr_units.hpp"
+
+#ifndef GPR_UNIT_N
+#error "GPR_UNIT_N (2 or 3) must be defined"
+#endif
+#ifndef GPR_UNIT_ND
+#error "GPR_UNIT_ND (1 or 2) must be defined"
+#endif
+
+namespace gpr {
+
+// fv_launcher's faces and cells for one (TIME, SOURCES) pair
+template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_pair(hipStream_t s, const FvArgs &a) {
+  const long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
+  constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
+  if (a.parts & 2)
+    hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, s, a);
+  const long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
+  constexpr int CPW = CellShape<N, ND, TIME>::CPW;
+  hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, s, a);
+}
+
+template <int N, bool TH, int ND> hipError_t fv_launch(hipStream_t s, const FvArgs &a) {
+  if (a.time && a.sources)
+    fv_pair<N, TH, ND, true, true>(s, a);
+  else if (!a.time && !a.sources)
+    fv_pair<N, TH, ND, false, false>(s, a);
+  else if (a.time)
+    fv_pair<N, TH, ND, true, false>(s, a);
+  else
+    fv_pair<N, TH, ND, false, true>(s, a);
+  return hipGetLastError();
+}
+
+template <int N, int ND> hipError_t fv_upload(const Basis *hb) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_basis), hb, 2 * sizeof(Basis));
+}
+
+template hipError_t fv_launch<GPR_UNIT_N, false, GPR_UNIT_ND>(hipStream_t, const FvArgs &);
+template hipError_t fv_launch<GPR_UNIT_N, true, GPR_UNIT_ND>(hipStream_t, const FvArgs &);
+template hipError_t fv_upload<GPR_UNIT_N, GPR_UNIT_ND>(const Basis *);
+
+}  // namespace gpr

@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void k_reset(double *u, long ncell, double dx,
 // dt for the coming step from the CFL maximum (timestep, iterator.cpp:74-82).
 // first: the maximum was computed by k_cfl into maxspeed; otherwise by the
 // previous step's post work (maxspeed, or maxspeed2 after a reset).
-__global__ void k_finalize_dt(StepState *st) {
+static __global__ void k_finalize_dt(StepState *st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long bits = st->contorted ? st->maxspeed2 : st->maxspeed;
   double MAX = __longlong_as_double((long long)bits);
@@ -665,12 +665,12 @@ __global__ void k_finalize_dt(StepState *st) {
 }
 
 // *p = 0 as a kernel (a node that graph capture records reliably)
-__global__ void k_zero_int(int *p) {
+static __global__ void k_zero_int(int *p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
 // t += dt, count += 1 (iterator.cpp:142-143)
-__global__ void k_advance(StepState *st) {
+static __global__ void k_advance(StepState *st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   st->t += st->dt;
   st->count += 1;

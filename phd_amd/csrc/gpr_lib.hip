@@ -15,6 +15,7 @@
 #include "../../include/gpr_mi355x.h"
 #include "gpr_adjoint.hpp"
 #include "gpr_batch.hpp"
+#include "gpr_units.hpp"
 
 using namespace gpr;
 
@@ -349,61 +350,32 @@ template <int N, bool TH, int ND> struct WenoF32Op {
   }
 };
 
+// The predictor and FV kernels are compiled in units of their own
+// (gpr_pred.hip per N, gpr_fv.hip per (N, dimensions)); see gpr_units.hpp.
 template <int N, bool TH, int ND> struct PredictorOp {
   static int run(gpr_ctx *c, PredArgs a) {
-    constexpr int CPW = PredShape<N, ND>::CPW;
-    long blocks = (a.ncell + CPW - 1) / CPW;
-    hipLaunchKernelGGL((k_predictor<N, TH, ND>), dim3(blocks), dim3(WAVE), 0, c->stream, a);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY((pred_launch<N, TH, ND>(c->stream, a, PRED_ONE_LAUNCH)));
     return GPR_OK;
   }
 };
 
-// The predictor as two launches: Picard in R over a.mask (wh elements WIn),
-// appending the cells it leaves to a.list; then the FP64 preconditioned
-// fallback over the list, CPW listed cells per wavefront.  R = double gives
-// the plain method's values; R = float is the FP32-bulk form.
-template <int N, bool TH, int ND, typename R, typename WIn> int predictor_split(gpr_ctx *c, PredArgs a) {
-  constexpr int CPW = PredShape<N, ND>::CPW;
-  long blocks = (a.ncell + CPW - 1) / CPW;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, a.list_count);
-  hipLaunchKernelGGL((k_predictor<N, TH, ND, R, WIn, false, KIND_PICARD>), dim3(blocks), dim3(WAVE), 0, c->stream,
-                     a);
-  hipLaunchKernelGGL((k_predictor<N, TH, ND, double, WIn, false, KIND_FALLBACK>), dim3(blocks), dim3(WAVE), 0,
-                     c->stream, a);
-  HIP_TRY(hipGetLastError());
-  return GPR_OK;
-}
-
 template <int N, bool TH, int ND> struct PredictorF32Op {
-  static int run(gpr_ctx *c, PredArgs a) { return predictor_split<N, TH, ND, float, float>(c, a); }
+  static int run(gpr_ctx *c, PredArgs a) {
+    HIP_TRY((pred_launch<N, TH, ND>(c->stream, a, PRED_SPLIT_FP32)));
+    return GPR_OK;
+  }
 };
 
 template <int N, bool TH, int ND> struct PredictorSplitOp {
-  static int run(gpr_ctx *c, PredArgs a) { return predictor_split<N, TH, ND, double, double>(c, a); }
+  static int run(gpr_ctx *c, PredArgs a) {
+    HIP_TRY((pred_launch<N, TH, ND>(c->stream, a, PRED_SPLIT_FP64)));
+    return GPR_OK;
+  }
 };
-
-// fv_launcher's faces and cells for one (TIME, SOURCES) pair
-template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_launch(gpr_ctx *c, const FvArgs &a) {
-  long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
-  constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
-  if (a.parts & 2)
-    hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, c->stream, a);
-  long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
-  constexpr int CPW = CellShape<N, ND, TIME>::CPW;
-  hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, c->stream, a);
-}
 
 template <int N, bool TH, int ND> struct FvOp {
   static int run(gpr_ctx *c, FvArgs a) {
-    if (a.time && a.sources)
-      fv_launch<N, TH, ND, true, true>(c, a);
-    else if (!a.time && !a.sources)
-      fv_launch<N, TH, ND, false, false>(c, a);
-    else if (a.time)
-      fv_launch<N, TH, ND, true, false>(c, a);
-    else
-      fv_launch<N, TH, ND, false, true>(c, a);
+    HIP_TRY((fv_launch<N, TH, ND>(c->stream, a)));
     long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
     if (a.post && a.ode)
       hipLaunchKernelGGL((k_post<TH, ND, true>), dim3(grid_for(nc, 256)), dim3(256), 0, c->stream, a.u, nc, a.dx,
@@ -723,6 +695,12 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
     // every context, so concurrent writers agree)
     const Basis hb[2] = {basis(2), basis(3)};
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_basis), hb, sizeof(hb));
+    // the copies of the translation units that launch the adjoint's kernels
+    if (e == hipSuccess) e = adj_heavy_upload<false, false>(hb);
+    if (e == hipSuccess) e = adj_heavy_upload<false, true>(hb);
+    if (e == hipSuccess) e = adj_heavy_upload<true, false>(hb);
+    if (e == hipSuccess) e = adj_heavy_upload<true, true>(hb);
+    if (e == hipSuccess) e = units_upload_basis(hb);
   }
   if (e != hipSuccess) {
     int rc = fail(GPR_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -2134,15 +2112,11 @@ int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *
   constexpr int FPW = FaceShape<N, TH, 2>::FPW;
   constexpr int CPW = PredShape<N, 2>::CPW;
   const dim3 gf((nfx + nfy + FPW - 1) / FPW), gc((ne + CPW - 1) / CPW);
-  if (A->dual_local) {
-    hipLaunchKernelGGL((ka_faces<N, 2, true, TH>), gf, dim3(WAVE), 0, c->stream, a);
-    hipLaunchKernelGGL((ka_cells<N, 2, true, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
-    hipLaunchKernelGGL((ka_predictor<N, 2, true, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
-  } else {
-    hipLaunchKernelGGL((ka_faces<N, 2, false, TH>), gf, dim3(WAVE), 0, c->stream, a);
-    hipLaunchKernelGGL((ka_cells<N, 2, false, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
-    hipLaunchKernelGGL((ka_predictor<N, 2, false, TH>), gc, dim3(WAVE), 0, c->stream, a, ne);
-  }
+  static_assert(N == 3, "the adjoint's transposed faces, cells and predictor are built for N = 3");
+  if (A->dual_local)
+    HIP_TRY((adj_heavy_launch<true, TH>(gf, gc, c->stream, a, ne)));
+  else
+    HIP_TRY((adj_heavy_launch<false, TH>(gf, gc, c->stream, a, ne)));
   hipLaunchKernelGGL((ka_weno_y<N, V>), dim3(grid_for(ne * N * V, 256)), dim3(256), 0, c->stream, a);
   hipLaunchKernelGGL((ka_weno_x<N, V>), dim3(grid_for(nux * V, 256)), dim3(256), 0, c->stream, a);
   HIP_TRY(hipMemcpyAsync(a.lam0, a.lam_fv, sizeof(double) * nc * V, hipMemcpyDeviceToDevice, c->stream));
