@@ -2,8 +2,10 @@
 
     python -m phd_amd.build            # or phd_amd.build.build()
 
-Compiles phd_amd/csrc/*.hip with hipcc for gfx950 only.  Packed FP32 VALU
-instructions are disabled (-packed-fp32-ops) for every device compile.
+Compiles phd_amd/csrc/*.hip with hipcc for gfx950 only, each unit (and each
+variant of a unit compiled more than once) as its own object, in parallel.
+Packed FP32 VALU instructions are disabled (-packed-fp32-ops) for every
+device compile.
 """
 
 import os
@@ -20,6 +22,10 @@ HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
+    # fuse a*b+c into an FMA only within one source expression, so that every
+    # kernel instantiation of the same code rounds alike (the one- and
+    # two-launch predictors, the batched and single-domain steps)
+    "-ffp-contract=on",
 ] + os.environ.get("GPR_EXTRA_FLAGS", "").split()
 
 

@@ -22,7 +22,7 @@
 
 // waves per SIMD the predictor is compiled for (register budget 512 / waves)
 #ifndef GPR_PRED_WAVES
-#define GPR_PRED_WAVES 1
+#define GPR_PRED_WAVES 2
 #endif
 
 namespace gpr {
@@ -187,6 +187,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Scheduling fence every GPR_PRED_CHUNK variables of an unrolled contraction:
+// without it the scheduler issues all of a contraction's LDS reads before its
+// arithmetic, which needs more registers than two wavefronts per SIMD have.
+#ifndef GPR_PRED_CHUNK
+#define GPR_PRED_CHUNK 4
+#endif
+#define GPR_CHUNK_FENCE(v) \
+  if ((v) % GPR_PRED_CHUNK == GPR_PRED_CHUNK - 1) __builtin_amdgcn_sched_barrier(0)
+
+// Materialise x here: an empty asm that reads and writes x keeps the
+// arithmetic producing x ahead of this point.  Without it, LLVM sinks a
+// contraction's arithmetic below the next conditional LDS-write block, to
+// where its result is next used, while the LDS reads stay behind the wave
+// fence above: every value read stays live across the blocks in between.
+template <typename T> __device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
+
 // a / b to within a few ulps: hardware reciprocal refined by one Newton step
 __device__ __forceinline__ double fast_div(double a, double b) {
   double r = __builtin_amdgcn_rcp(b);
@@ -239,10 +255,16 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   constexpr int PRE = TH ? 27 : 18;
   constexpr bool FB = KIND != KIND_PICARD;  // this instantiation has the fallback
   static_assert(!FB || sizeof(R) == sizeof(double), "the fallback computes in double");
-  __shared__ R sQ[CPW * NN * V];   // iterate; in the fallback update also P g
-  __shared__ R sF[CPW * NN * V];   // x fluxes, then rhs or g
-  __shared__ R sG[ND == 2 ? CPW * NN * V : 1];  // y fluxes
+  // LDS per wavefront (17.5 KB for N = 3, 2D, V = 14, so that two
+  // wavefronts per SIMD fit in 160 KB).  sQ holds the iterate (the
+  // derivative contractions and the fallback's time column read it), then in
+  // the fallback P g on the distortion rows; sF holds the x fluxes, then the
+  // y fluxes, then the time solve's right-hand side.
+  __shared__ R sQ[CPW * NN * V];
+  __shared__ R sF[CPW * NN * V];
   __shared__ double sPre[FB ? CPW * NS * PRE : 1];
+  __shared__ R sW[CPW * NS * V];       // the WENO state of each spatial node
+  __shared__ R sB[4 * N * N];          // der, dgder, dgmatinv, dgmat
   __shared__ R sRat[WAVE];
 
   const Basis &K = basis_dev<N>();
@@ -292,9 +314,25 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   const int sl = lane_ok ? slot : 0;
   R *myQ = sQ + (sl * NN) * V;
   R *myF = sF + (sl * NN) * V;
-  R *myG = sG + (ND == 2 ? (sl * NN) * V : 0);
+  R *myW = sW + (sl * NS) * V;
   double *myPre = sPre + (FB ? (sl * NS + s) * PRE : 0);
-  const WIn *wsrc = (const WIn *)a.wh + (cell * NS + s) * V;
+
+  // basis tables and the WENO state into LDS
+  if (lane < 4 * N * N) {
+    const int tb = lane / (N * N), e = lane - tb * N * N;
+    const double *src = tb == 0 ? &K.der[0][0] : tb == 1 ? &K.dgder[0][0] : tb == 2 ? &K.dgmatinv[0][0] : &K.dgmat[0][0];
+    sB[lane] = R(src[(e / N) * MAXN + e % N]);
+  }
+  R q0[V];
+  {
+    const WIn *wsrc = (const WIn *)a.wh + (cell * NS + s) * V;
+#pragma unroll
+    for (int v = 0; v < V; v++) q0[v] = active ? R(wsrc[v]) : R(v == 0 || v == 5 || v == 9 || v == 13 ? 1 : 0);
+    if (lane_ok && t == 0)
+#pragma unroll
+      for (int v = 0; v < V; v++) myW[s * V + v] = q0[v];
+  }
+  const R *Bder = sB, *Bdgder = sB + N * N, *Bmi = sB + 2 * N * N, *Bmt = sB + 3 * N * N;
 
   const R dtR = R(dt);
   const R Ws = R(ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib]);
@@ -302,24 +340,6 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   const R Wt = R(K.w[t]);
   const R cW = R(K.endv[0][t] * (ND == 1 ? K.w[ia] : K.w[ia] * K.w[ib]));  // psi_t(0) W_s
   const R inv_dx = R(1.0 / dxv), inv_dy = R(1.0 / a.dy);
-  // this lane's rows of the basis matrices
-  R cdx[N], cdy[N], cgx[N], cgy[N], cmi[N], cmt[N];
-#pragma unroll
-  for (int j = 0; j < N; j++) {
-    cdx[j] = R(K.der[ia][j]);
-    cdy[j] = R(K.der[ib][j]);
-    cgx[j] = R(K.dgder[ia][j]);
-    cgy[j] = R(K.dgder[ib][j]);
-    cmi[j] = R(K.dgmatinv[t][j]);
-    cmt[j] = R(K.dgmat[t][j]);
-  }
-
-  R q0[V], w[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    w[v] = active ? R(wsrc[v]) : R(v == 0 || v == 5 || v == 9 || v == 13 ? 1 : 0);
-    q0[v] = w[v];
-  }
 
   // phase: 0 Picard, 1 preconditioned, 2 converged, 3 idle / given up
   int phase = active ? ((KIND == KIND_FALLBACK || (FB && a.mode == 1)) ? 1 : 0) : 3;
@@ -327,10 +347,16 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   R prev = R(1e30);
   const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * NN);
   bool need_pre = phase == 1;
+  wave_sync();  // sB, sW
 
   while (__any(phase == 0 || phase == 1)) {
     if constexpr (FB) {
-      if (need_pre && t == 0 && lane_ok) stiff_precond<N, TH>(w, M, dt, myPre);
+      if (need_pre && t == 0 && lane_ok) {
+        R w[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) w[v] = myW[s * V + v];
+        stiff_precond<N, TH>(w, M, dt, myPre);
+      }
     }
     need_pre = false;
     // (1) share the iterate
@@ -340,87 +366,123 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     wave_sync();
 
     // (2) spatial derivatives, source, non-conservative products, fluxes
-    R rhs[V];
+    R rhs[V], G[ND == 2 ? V : 1];
     {
       NodeState<R, TH> st;
       node_state<R, TH>(q0, M, st);
       source<R, TH>(q0, st, M, rhs);
       // B_d(q) . dq/dx_d reads only the distortion rows of dq (equations.cpp:189-247)
       R dq[14];
-#pragma unroll
-      for (int v = 5; v < 14; v++) {
-        R acc = 0;
-#pragma unroll
-        for (int j = 0; j < N; j++) acc += cdx[j] * myQ[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
-        dq[v] = acc * inv_dx;
-      }
-      bdot_sub(st.v, dq, 0, R(1), rhs);
-      if (ND == 2) {
+      {
+        R c0 = Bder[ia * N + 0], c1 = Bder[ia * N + 1], c2 = N > 2 ? Bder[ia * N + N - 1] : R(0);
 #pragma unroll
         for (int v = 5; v < 14; v++) {
           R acc = 0;
 #pragma unroll
-          for (int j = 0; j < N; j++) acc += cdy[j] * myQ[(t * NS + ia * N + j) * V + v];
+          for (int j = 0; j < N; j++)
+            acc += (j == 0 ? c0 : j == 1 ? c1 : c2) * myQ[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
+          dq[v] = acc * inv_dx;
+          GPR_CHUNK_FENCE(v);
+        }
+      }
+      bdot_sub(st.v, dq, 0, R(1), rhs);
+#pragma unroll
+      for (int v = 5; v < 14; v++) pin(rhs[v]);
+      if (ND == 2) {
+        R c0 = Bder[ib * N + 0], c1 = Bder[ib * N + 1], c2 = N > 2 ? Bder[ib * N + N - 1] : R(0);
+#pragma unroll
+        for (int v = 5; v < 14; v++) {
+          R acc = 0;
+#pragma unroll
+          for (int j = 0; j < N; j++) acc += (j == 0 ? c0 : j == 1 ? c1 : c2) * myQ[(t * NS + ia * N + j) * V + v];
           dq[v] = acc * inv_dy;
+          GPR_CHUNK_FENCE(v);
         }
         bdot_sub(st.v, dq, 1, R(1), rhs);
       }
       const R wts = Wt * Ws;
 #pragma unroll
-      for (int v = 0; v < V; v++) rhs[v] *= wts;
+      for (int v = 0; v < V; v++) {
+        rhs[v] *= wts;
+        pin(rhs[v]);
+      }
       R F[V];
       flux<R, TH>(q0, st, M, 0, F);
       const R wb = R(ND == 2 ? K.w[ib] : 1.0);
       if (lane_ok)
 #pragma unroll
         for (int v = 0; v < V; v++) myF[node * V + v] = F[v] * wb;
-      if (ND == 2) {
-        flux<R, TH>(q0, st, M, 1, F);
-        if (lane_ok)
+      if constexpr (ND == 2) {
+        flux<R, TH>(q0, st, M, 1, G);
 #pragma unroll
-          for (int v = 0; v < V; v++) myG[node * V + v] = F[v];
+        for (int v = 0; v < V; v++) pin(G[v]);
       }
     }
     wave_sync();
-    // (3) flux contractions (rhs2, dg2.cpp:173-201): x from sF, y from sG
+    // (3) flux contractions (rhs2, dg2.cpp:173-201): x, then y through the
+    // same buffer
     {
       const R cx = Wt * inv_dx;
+      R c0 = Bdgder[ia * N + 0], c1 = Bdgder[ia * N + 1], c2 = N > 2 ? Bdgder[ia * N + N - 1] : R(0);
 #pragma unroll
       for (int v = 0; v < V; v++) {
         R sx = 0;
 #pragma unroll
-        for (int j = 0; j < N; j++) sx += cgx[j] * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
+        for (int j = 0; j < N; j++)
+          sx += (j == 0 ? c0 : j == 1 ? c1 : c2) * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
         rhs[v] -= cx * sx;
+        pin(rhs[v]);
+        GPR_CHUNK_FENCE(v);
       }
     }
-    if (ND == 2) {
+    if constexpr (ND == 2) {
+      wave_sync();
+      if (lane_ok)
+#pragma unroll
+        for (int v = 0; v < V; v++) myF[node * V + v] = G[v];
+      wave_sync();
       const R cy = Wt * R(K.w[ia]) * inv_dy;
+      R c0 = Bdgder[ib * N + 0], c1 = Bdgder[ib * N + 1], c2 = N > 2 ? Bdgder[ib * N + N - 1] : R(0);
 #pragma unroll
       for (int v = 0; v < V; v++) {
         R sy = 0;
 #pragma unroll
-        for (int j = 0; j < N; j++) sy += cgy[j] * myG[(t * NS + ia * N + j) * V + v];
+        for (int j = 0; j < N; j++) sy += (j == 0 ? c0 : j == 1 ? c1 : c2) * myF[(t * NS + ia * N + j) * V + v];
         rhs[v] -= cy * sy;
+        pin(rhs[v]);
+        GPR_CHUNK_FENCE(v);
       }
     }
 #pragma unroll
-    for (int v = 0; v < V; v++) rhs[v] = rhs[v] * dtR + cW * w[v];
+    for (int v = 0; v < V; v++) {
+      rhs[v] = rhs[v] * dtR + cW * myW[s * V + v];
+      pin(rhs[v]);
+    }
     // (4) time solve.  Picard: q1 = (DG_MAT (x) W_s)^-1 rhs.  Fallback:
-    // g = DG_MAT q0 - rhs / W_s (q0 of the time column from sQ), P g in sQ.
+    // g = DG_MAT q0 - rhs / W_s (q0 of the time column from sQ); sF gets
+    // g - P g on the distortion rows and g elsewhere, and P g replaces the
+    // distortion rows of q0 in sQ.
     [[maybe_unused]] R P[9];
     if constexpr (FB) {
       if (phase == 1) {
+        {
+          R m0 = Bmt[t * N + 0], m1 = Bmt[t * N + 1], m2 = N > 2 ? Bmt[t * N + N - 1] : R(0);
 #pragma unroll
-        for (int v = 0; v < V; v++) {
-          R acc = 0;
+          for (int v = 0; v < V; v++) {
+            R acc = 0;
 #pragma unroll
-          for (int k = 0; k < N; k++) acc += cmt[k] * myQ[(k * NS + s) * V + v];
-          rhs[v] = acc - rhs[v] * iWs;  // rhs now holds g
+            for (int k = 0; k < N; k++) acc += (k == 0 ? m0 : k == 1 ? m1 : m2) * myQ[(k * NS + s) * V + v];
+            rhs[v] = acc - rhs[v] * iWs;  // rhs now holds g
+            pin(rhs[v]);
+            GPR_CHUNK_FENCE(v);
+          }
         }
         proj_dev_sym(myPre + 9, rhs + 5, P);
+#pragma unroll
+        for (int k = 0; k < 9; k++) rhs[5 + k] -= P[k];
       }
     }
-    wave_sync();
+    wave_sync();  // every lane has read its time column of q0
     if (lane_ok) {
 #pragma unroll
       for (int v = 0; v < V; v++) myF[node * V + v] = rhs[v];
@@ -431,36 +493,43 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       }
     }
     wave_sync();
-    if (FB && phase == 1) {
-      if constexpr (FB) {
-        const double *Ainv = myPre, *Jinv = myPre + 18;
+    {
+      R m0 = Bmi[t * N + 0], m1 = Bmi[t * N + 1], m2 = N > 2 ? Bmi[t * N + N - 1] : R(0);
+      if (FB && phase == 1) {
+        if constexpr (FB) {
+          const double *Ainv = myPre, *Jinv = myPre + 18;
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            R d = 0;
+            if (v >= 5 && v < 14) {
+#pragma unroll
+              for (int k = 0; k < N; k++) {
+                R hk = myF[(k * NS + s) * V + v];
+                R pk = myQ[(k * NS + s) * V + v];
+                d += (k == 0 ? m0 : k == 1 ? m1 : m2) * hk + Ainv[t * N + k] * pk;
+              }
+            } else if (v >= 14) {
+#pragma unroll
+              for (int k = 0; k < N; k++) d += Jinv[t * N + k] * myF[(k * NS + s) * V + v];
+            } else {
+#pragma unroll
+              for (int k = 0; k < N; k++) d += (k == 0 ? m0 : k == 1 ? m1 : m2) * myF[(k * NS + s) * V + v];
+            }
+            rhs[v] = q0[v] - d;  // rhs now holds q1
+            pin(rhs[v]);
+            GPR_CHUNK_FENCE(v);
+          }
+        }
+      } else {
 #pragma unroll
         for (int v = 0; v < V; v++) {
-          R d = 0;
-          if (v >= 5 && v < 14) {
+          R acc = 0;
 #pragma unroll
-            for (int k = 0; k < N; k++) {
-              R gk = myF[(k * NS + s) * V + v];
-              R pk = myQ[(k * NS + s) * V + v];
-              d += cmi[k] * (gk - pk) + Ainv[t * N + k] * pk;
-            }
-          } else if (v >= 14) {
-#pragma unroll
-            for (int k = 0; k < N; k++) d += Jinv[t * N + k] * myF[(k * NS + s) * V + v];
-          } else {
-#pragma unroll
-            for (int k = 0; k < N; k++) d += cmi[k] * myF[(k * NS + s) * V + v];
-          }
-          rhs[v] = q0[v] - d;  // rhs now holds q1
+          for (int k = 0; k < N; k++) acc += (k == 0 ? m0 : k == 1 ? m1 : m2) * myF[(k * NS + s) * V + v];
+          rhs[v] = acc * iWs;  // rhs now holds q1
+          pin(rhs[v]);
+          GPR_CHUNK_FENCE(v);
         }
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        R acc = 0;
-#pragma unroll
-        for (int k = 0; k < N; k++) acc += cmi[k] * myF[(k * NS + s) * V + v];
-        rhs[v] = acc * iWs;  // rhs now holds q1
       }
     }
     // (5) stopping rule over the cell's whole block (dg2.cpp:278-297)
@@ -483,12 +552,12 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
       ratio = cr;
     }
+#pragma unroll
+    for (int v = 0; v < V; v++) q0[v] = rhs[v];  // the next iterate (reset below on a restart)
     if (phase == 0) {
       it++;
       bool to_fallback = false;
       if (cmore) {
-#pragma unroll
-        for (int v = 0; v < V; v++) q0[v] = rhs[v];
         grow = ratio > prev ? grow + 1 : 0;
         prev = ratio;
         if (it >= a.dg_it || (a.bailout > 0 && grow >= a.bailout)) to_fallback = true;
@@ -512,7 +581,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
           pit = 0;
           need_pre = true;
 #pragma unroll
-          for (int v = 0; v < V; v++) q0[v] = w[v];
+          for (int v = 0; v < V; v++) q0[v] = myW[s * V + v];
         } else {
           // solved again by the KIND_FALLBACK launch over a.list
           if (node == 0) {
@@ -524,8 +593,6 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       }
     } else if (FB && phase == 1) {
       pit++;
-#pragma unroll
-      for (int v = 0; v < V; v++) q0[v] = rhs[v];
       if (!cmore || pit >= a.stiff_it) {
 #pragma unroll
         for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = rhs[v];
@@ -543,7 +610,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         phase = ok ? 2 : 3;
       }
     }
-    if (a.bailout > 0) wave_sync();  // sRat is rewritten next iteration
+    wave_sync();  // sQ, sF and sRat are rewritten next iteration
   }
 }
 
