@@ -63,24 +63,54 @@ template <bool TH> struct Dims {
   static constexpr int V = TH ? 17 : 14;
 };
 
+// Material kinds a kernel can be compiled for.  MK_ANY reads the EOS and the
+// relaxation model from the parameters at run time; the others fix them at
+// compile time, so that a kernel carries no code (and keeps no hoisted
+// constants in registers) for the branches its material never takes.
+enum { MK_ANY = 0, MK_GAS_NEWTONIAN = 1, MK_GR_PLASTIC = 2 };
+template <int MK> struct MatKind {
+  // -1: read from the parameters
+  static constexpr int eos = MK == MK_GAS_NEWTONIAN ? GPR_EOS_STIFFENED_GAS
+                             : MK == MK_GR_PLASTIC  ? GPR_EOS_GODUNOV_ROMENSKI
+                                                    : -1;
+  static constexpr int solid = MK == MK_GAS_NEWTONIAN ? 0 : MK == MK_GR_PLASTIC ? 1 : -1;
+  static constexpr int power_law = MK == MK_GAS_NEWTONIAN ? 0 : MK == MK_GR_PLASTIC ? 1 : -1;
+  static constexpr int bingham = MK == MK_ANY ? -1 : 0;
+  static constexpr int beta_zero = MK == MK_GAS_NEWTONIAN ? 1 : -1;
+};
+template <int MK> __host__ __device__ __forceinline__ int eos_code(const gpr_par &p) {
+  return MatKind<MK>::eos >= 0 ? MatKind<MK>::eos : p.EOS;
+}
+template <int MK> __host__ __device__ __forceinline__ bool mk_flag(int fixed, int runtime) {
+  return fixed >= 0 ? fixed != 0 : runtime != 0;
+}
+// The most specific material kind a parameter set fits (MK_ANY if none).
+inline int material_kind(const Mat &m) {
+  const gpr_par &p = m.p;
+  if (p.BINGHAM) return MK_ANY;
+  if (p.EOS == GPR_EOS_STIFFENED_GAS && !p.SOLID && !p.POWER_LAW && m.beta_zero) return MK_GAS_NEWTONIAN;
+  if (p.EOS == GPR_EOS_GODUNOV_ROMENSKI && p.SOLID && p.POWER_LAW) return MK_GR_PLASTIC;
+  return MK_ANY;
+}
+
 // ------------------------------------------------------------------ EOS
 template <typename R> __device__ __forceinline__ R dpow(R a, R b) { return pow(a, b); }
 
-template <typename R> __device__ __forceinline__ R Gamma_MG(R r, const gpr_par &p) {
-  switch (p.EOS) {
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R Gamma_MG(R r, const gpr_par &p) {
+  switch (eos_code<MK>(p)) {
   case GPR_EOS_STIFFENED_GAS: return R(p.gamma - 1);
   case GPR_EOS_SHOCK_MG: return R(p.Gamma0 * p.rho0) / r;
   case GPR_EOS_GODUNOV_ROMENSKI: return R(p.gamma);
   default: return R(p.Gamma0);
   }
 }
-template <typename R> __device__ __forceinline__ R dGamma_MG(R r, const gpr_par &p) {
-  if (p.EOS == GPR_EOS_SHOCK_MG) return -R(p.Gamma0 * p.rho0) / (r * r);
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R dGamma_MG(R r, const gpr_par &p) {
+  if (eos_code<MK>(p) == GPR_EOS_SHOCK_MG) return -R(p.Gamma0 * p.rho0) / (r * r);
   return R(0);
 }
-template <typename R> __device__ __forceinline__ R p_ref(R r, const gpr_par &p) {
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R p_ref(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
-  switch (p.EOS) {
+  switch (eos_code<MK>(p)) {
   case GPR_EOS_STIFFENED_GAS: return -R(p.pINF);
   case GPR_EOS_SHOCK_MG:
     if (r > r0) {
@@ -102,12 +132,12 @@ template <typename R> __device__ __forceinline__ R p_ref(R r, const gpr_par &p) 
   }
   }
 }
-template <typename R> __device__ __forceinline__ R e_ref(R r, const gpr_par &p) {
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R e_ref(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
-  switch (p.EOS) {
+  switch (eos_code<MK>(p)) {
   case GPR_EOS_STIFFENED_GAS: return R(p.pINF) / r;
   case GPR_EOS_SHOCK_MG:
-    if (r > r0) return R(0.5) * p_ref(r, p) * (R(1) / r0 - R(1) / r);
+    if (r > r0) return R(0.5) * p_ref<R, MK>(r, p) * (R(1) / r0 - R(1) / r);
     return R(0);
   case GPR_EOS_GODUNOV_ROMENSKI: {
     R t = p.alpha == 1.0 ? r / r0 : dpow(r / r0, R(p.alpha));
@@ -124,9 +154,9 @@ template <typename R> __device__ __forceinline__ R e_ref(R r, const gpr_par &p) 
   }
   }
 }
-template <typename R> __device__ __forceinline__ R dp_ref(R r, const gpr_par &p) {
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R dp_ref(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
-  switch (p.EOS) {
+  switch (eos_code<MK>(p)) {
   case GPR_EOS_STIFFENED_GAS: return R(0);
   case GPR_EOS_SHOCK_MG:
     if (r > r0) {
@@ -148,8 +178,8 @@ template <typename R> __device__ __forceinline__ R dp_ref(R r, const gpr_par &p)
   }
   }
 }
-template <typename R> __device__ __forceinline__ R de_ref(R r, const gpr_par &p) {
-  if (p.EOS == GPR_EOS_SHOCK_MG) {
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R de_ref(R r, const gpr_par &p) {
+  if (eos_code<MK>(p) == GPR_EOS_SHOCK_MG) {
     R r0 = R(p.rho0);
     if (r > r0) {
       R t = R(p.s) * (r - r0) - r;
@@ -157,25 +187,26 @@ template <typename R> __device__ __forceinline__ R de_ref(R r, const gpr_par &p)
     }
     return R(0);
   }
-  return p_ref(r, p) / (r * r);
+  return p_ref<R, MK>(r, p) / (r * r);
 }
-template <typename R> __device__ __forceinline__ R phi_int(R r, const gpr_par &p) {
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R phi_int(R r, const gpr_par &p) {
   R r0 = R(p.rho0);
-  switch (p.EOS) {
+  switch (eos_code<MK>(p)) {
   case GPR_EOS_STIFFENED_GAS: return dpow(r / r0, R(p.gamma - 1));
   case GPR_EOS_SHOCK_MG: return exp(R(p.Gamma0) * (R(1) - r0 / r));
   case GPR_EOS_GODUNOV_ROMENSKI: return dpow(r / r0, R(p.gamma));
   default: return dpow(r / r0, R(p.Gamma0));
   }
 }
-template <typename R> __device__ __forceinline__ R temperature_mg(R r, R pp, const gpr_par &p) {
-  R G = Gamma_MG(r, p);
-  R T = (pp - p_ref(r, p)) / (r * G * R(p.cv));
-  if (p.Tref != 0.0) T += phi_int(r, p) * R(p.Tref);
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R temperature_mg(R r, R pp, const gpr_par &p) {
+  R G = Gamma_MG<R, MK>(r, p);
+  R T = (pp - p_ref<R, MK>(r, p)) / (r * G * R(p.cv));
+  if (p.Tref != 0.0) T += phi_int<R, MK>(r, p) * R(p.Tref);
   return T;
 }
-template <typename R> __device__ __forceinline__ R c_s2(R r, const Mat &m) {
-  return m.beta_zero ? R(m.p.b02) : R(m.p.b02) * dpow(r / R(m.p.rho0), R(m.p.beta));
+template <typename R, int MK = MK_ANY> __device__ __forceinline__ R c_s2(R r, const Mat &m) {
+  return mk_flag<MK>(MatKind<MK>::beta_zero, m.beta_zero) ? R(m.p.b02)
+                                                          : R(m.p.b02) * dpow(r / R(m.p.rho0), R(m.p.beta));
 }
 
 template <typename R> __device__ __forceinline__ R det3(const R *A) {
@@ -192,7 +223,7 @@ template <typename R, bool TH> struct NodeState {
   R devGsq;
 };
 
-template <typename R, bool TH>
+template <typename R, bool TH, int MK = MK_ANY>
 __device__ __forceinline__ void node_state(const R *q, const Mat &m, NodeState<R, TH> &s) {
   const gpr_par &p = m.p;
   s.r = q[0];
@@ -216,7 +247,7 @@ __device__ __forceinline__ void node_state(const R *q, const Mat &m, NodeState<R
   G[0] = d0;
   G[4] = d1;
   G[8] = d2;
-  s.cs2 = c_s2(s.r, m);
+  s.cs2 = c_s2<R, MK>(s.r, m);
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -232,7 +263,7 @@ __device__ __forceinline__ void node_state(const R *q, const Mat &m, NodeState<R
       s.sig[i * 3 + j] = x;
       s.sig[j * 3 + i] = x;
     }
-  R Gm = Gamma_MG(s.r, p);
+  R Gm = Gamma_MG<R, MK>(s.r, p);
   R C0 = (R(1) - R(p.beta) / Gm) * s.cs2;
   R e = q[1] * s.ir - (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]) / R(2) -
         C0 / R(4) * s.devGsq;
@@ -240,19 +271,19 @@ __device__ __forceinline__ void node_state(const R *q, const Mat &m, NodeState<R
     R J0 = q[14] * s.ir, J1 = q[15] * s.ir, J2 = q[16] * s.ir;
     e -= R(p.calpha2) * (J0 * J0 + J1 * J1 + J2 * J2) / R(2);
   }
-  s.p = (e - e_ref(s.r, p)) * s.r * Gm + p_ref(s.r, p);
-  s.T = TH ? temperature_mg(s.r, s.p, p) : R(0);
+  s.p = (e - e_ref<R, MK>(s.r, p)) * s.r * Gm + p_ref<R, MK>(s.r, p);
+  s.T = TH ? temperature_mg<R, MK>(s.r, s.p, p) : R(0);
 }
 
 // Relaxation rate of the distortion (variables/sources.cpp:13-81).
-template <typename R, bool TH>
+template <typename R, bool TH, int MK = MK_ANY>
 __device__ __forceinline__ R theta1inv(const R *q, const NodeState<R, TH> &s, const Mat &m) {
   const gpr_par &p = m.p;
   R dA = det3(q + 5);
   R c13 = cbrt(dA);
   R A53 = dA * c13 * c13;
-  if (p.SOLID) {
-    if (p.POWER_LAW) {
+  if (mk_flag<MK>(MatKind<MK>::solid, p.SOLID)) {
+    if (mk_flag<MK>(MatKind<MK>::power_law, p.POWER_LAW)) {
       const R *S = s.sig;
       R a = S[0] - S[4], b = S[4] - S[8], c = S[8] - S[0];
       R sn = sqrt(R(0.5) * (a * a + b * b + c * c) + R(3) * (S[1] * S[1] + S[5] * S[5] + S[6] * S[6]));
@@ -261,7 +292,7 @@ __device__ __forceinline__ R theta1inv(const R *q, const NodeState<R, TH> &s, co
     }
     return R(0);
   }
-  if (p.BINGHAM) {
+  if (mk_flag<MK>(MatKind<MK>::bingham, p.BINGHAM)) {
     R f = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) f += s.sig[i] * s.sig[i];
@@ -270,7 +301,7 @@ __device__ __forceinline__ R theta1inv(const R *q, const NodeState<R, TH> &s, co
     return R(0);
   }
   R c = A53 * R(m.mu_fac);
-  if (p.POWER_LAW) {
+  if (mk_flag<MK>(MatKind<MK>::power_law, p.POWER_LAW)) {
     R f = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) f += s.sig[i] * s.sig[i];
@@ -281,7 +312,7 @@ __device__ __forceinline__ R theta1inv(const R *q, const NodeState<R, TH> &s, co
 }
 
 // Algebraic source (equations.cpp:50-67).
-template <typename R, bool TH>
+template <typename R, bool TH, int MK = MK_ANY>
 __device__ __forceinline__ void source(const R *q, const NodeState<R, TH> &s, const Mat &m, R *S) {
   constexpr int V = Dims<TH>::V;
   S[0] = R(0);
@@ -289,7 +320,7 @@ __device__ __forceinline__ void source(const R *q, const NodeState<R, TH> &s, co
   S[2] = R(m.p.dp[0]);
   S[3] = R(m.p.dp[1]);
   S[4] = R(m.p.dp[2]);
-  R th = theta1inv(q, s, m);
+  R th = theta1inv<R, TH, MK>(q, s, m);
 #pragma unroll
   for (int i = 0; i < 9; i++) S[5 + i] = -s.EA[i] * th;
   if (TH) {

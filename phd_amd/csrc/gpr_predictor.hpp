@@ -75,11 +75,54 @@ struct PredArgs {
 };
 
 // One wavefront of 64 lanes holds CPW cells, one space-time node per lane.
+// A cell of more than 16 nodes (N = 3 in 2D: 27) takes a 32-lane half of
+// the wavefront, so that reductions over its lanes run within the half
+// (cross-lane DPP moves); smaller cells are packed densely.
 template <int N, int ND> struct PredShape {
   static constexpr int NS = ND == 1 ? N : N * N;  // spatial nodes per cell
   static constexpr int NN = NS * N;                // space-time nodes per cell
-  static constexpr int CPW = WAVE / NN;            // cells per wavefront
+  static constexpr int LPC = NN > 16 ? 32 : NN;    // lanes per cell slot
+  static constexpr int CPW = WAVE / LPC;           // cells per wavefront
 };
+
+// The flux components that can be nonzero in direction d (equations.cpp:11-48):
+// mass, energy, momentum, column d of the distortion (F[5 + d + 3k]) and the
+// thermal rows.  The other six distortion components are zero, so the
+// predictor neither stores nor contracts them.
+template <bool TH> struct FluxPack {
+  static constexpr int NF = TH ? 11 : 8;        // nonzero components
+  static constexpr int STRIDE = TH ? 14 : 10;   // LDS doubles per node (16-byte aligned records)
+};
+__host__ __device__ constexpr int flux_var(int i, int d) { return i < 5 ? i : i < 8 ? 5 + d + 3 * (i - 5) : 6 + i; }
+
+// max over each 32-lane half of the wavefront of a non-negative double (NaN
+// counts as 0, as fmax ignores it), for every lane of the half.  Doubles >= 0
+// order as their bit patterns, so the maximum is taken on (hi, lo) words
+// moved by DPP: prefix maxima within each 16-lane row (row_shr 1, 2, 4, 8),
+// then row 0's (row 2's) maximum into row 1 (row 3) by row_bcast:15, which
+// leaves the half's maximum in lanes 31 and 63.  Every lane must be active.
+__device__ __forceinline__ double half_wave_max(double x) {
+  unsigned long long b = x == x ? (unsigned long long)__double_as_longlong(x) : 0ull;
+  unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+#define GPR_DPP_MAX_STEP(ctrl, rmask)                                                \
+  {                                                                                 \
+    const unsigned l2 = __builtin_amdgcn_update_dpp(0u, lo, ctrl, rmask, 0xf, false); \
+    const unsigned h2 = __builtin_amdgcn_update_dpp(0u, hi, ctrl, rmask, 0xf, false); \
+    const bool gt = h2 > hi || (h2 == hi && l2 > lo);                                \
+    lo = gt ? l2 : lo;                                                              \
+    hi = gt ? h2 : hi;                                                              \
+  }
+  GPR_DPP_MAX_STEP(0x111, 0xf)  // row_shr:1
+  GPR_DPP_MAX_STEP(0x112, 0xf)  // row_shr:2
+  GPR_DPP_MAX_STEP(0x114, 0xf)  // row_shr:4
+  GPR_DPP_MAX_STEP(0x118, 0xf)  // row_shr:8
+  GPR_DPP_MAX_STEP(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+#undef GPR_DPP_MAX_STEP
+  const bool upper = (threadIdx.x & 32) != 0;
+  const unsigned l = upper ? __builtin_amdgcn_readlane(lo, 63) : __builtin_amdgcn_readlane(lo, 31);
+  const unsigned h = upper ? __builtin_amdgcn_readlane(hi, 63) : __builtin_amdgcn_readlane(hi, 31);
+  return __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
+}
 
 // Preconditioner of one spatial node, frozen at its WENO state ws:
 // out[0..8] = (DG_MAT + dt kappa W)^-1, out[9..17] = polar rotation of A,
@@ -87,14 +130,14 @@ template <int N, int ND> struct PredShape {
 // A = s Rot the distortion source has Jacobian -kappa Pi with
 // Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3);
 // the thermal source -(c_alpha^2 T / (kappa rho)) rho J has rate rJ.
-template <int N, bool TH>
+template <int N, bool TH, int MK = MK_ANY>
 __device__ __forceinline__ void stiff_precond(const double *ws, const Mat &m, double dt, double *out) {
   const Basis &K = basis_dev<N>();
   NodeState<double, TH> s;
-  node_state<double, TH>(ws, m, s);
+  node_state<double, TH, MK>(ws, m, s);
   double dA = det3(ws + 5);
   double s13 = cbrt(dA);
-  double kap = 2.0 * theta1inv<double, TH>(ws, s, m) * s.cs2 * s13 * s13;
+  double kap = 2.0 * theta1inv<double, TH, MK>(ws, s, m) * s.cs2 * s13 * s13;
   if (!(kap >= 0.0) || !(kap < 1e300)) kap = 0.0;
   double rJ = 0.0;
   if (TH) {
@@ -245,13 +288,16 @@ enum { KIND_BOTH = 0, KIND_PICARD = 1, KIND_FALLBACK = 2 };
 // form, KIND_PICARD only).  WIn: the element type of wh.
 // BATCH: the launch covers a batch of 1D instances (a.binst, a.pblk, a.bst)
 // with per-instance material, dx and dt; every extended cell is predicted.
+// MK: the material kind the physics is compiled for (MK_ANY: any material).
 template <int N, bool TH, int ND, typename R = double, typename WIn = double, bool BATCH = false,
-          int KIND = KIND_BOTH>
+          int KIND = KIND_BOTH, int MK = MK_ANY>
 __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(float) ? 2 : GPR_PRED_WAVES) void k_predictor(PredArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = PredShape<N, ND>::NS;
   constexpr int NN = PredShape<N, ND>::NN;
   constexpr int CPW = PredShape<N, ND>::CPW;
+  constexpr int LPC = PredShape<N, ND>::LPC;
+  constexpr int NF = FluxPack<TH>::NF, FS = FluxPack<TH>::STRIDE;
   constexpr int PRE = TH ? 27 : 18;
   constexpr bool FB = KIND != KIND_PICARD;  // this instantiation has the fallback
   static_assert(!FB || sizeof(R) == sizeof(double), "the fallback computes in double");
@@ -265,17 +311,17 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   __shared__ double sPre[FB ? CPW * NS * PRE : 1];
   __shared__ R sW[CPW * NS * V];       // the WENO state of each spatial node
   __shared__ R sB[4 * N * N];          // der, dgder, dgmatinv, dgmat
-  __shared__ R sRat[WAVE];
+  __shared__ R sRat[LPC == 32 ? 1 : WAVE];
 
   const Basis &K = basis_dev<N>();
   const int lane = threadIdx.x;
-  const int slot = lane / NN;
-  const int node = lane - slot * NN;
+  const int slot = lane / LPC;
+  const bool lane_ok = slot < CPW && lane - slot * LPC < NN;  // lanes outside a whole cell write no LDS
+  const int node = lane_ok ? lane - slot * LPC : 0;
   const int t = node / NS;
   const int s = node - t * NS;
   const int ia = ND == 1 ? s : s / N;
   const int ib = ND == 1 ? 0 : s - (s / N) * N;
-  const bool lane_ok = slot < CPW;  // lanes past the last whole cell write no LDS
   long cell;
   bool valid;
   double dt, dxv;
@@ -345,7 +391,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   int phase = active ? ((KIND == KIND_FALLBACK || (FB && a.mode == 1)) ? 1 : 0) : 3;
   int it = 0, pit = 0, grow = 0;
   R prev = R(1e30);
-  const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * NN);
+  const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * LPC);
   bool need_pre = phase == 1;
   wave_sync();  // sB, sW
 
@@ -355,7 +401,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         R w[V];
 #pragma unroll
         for (int v = 0; v < V; v++) w[v] = myW[s * V + v];
-        stiff_precond<N, TH>(w, M, dt, myPre);
+        stiff_precond<N, TH, MK>(w, M, dt, myPre);
       }
     }
     need_pre = false;
@@ -369,8 +415,8 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     R rhs[V], G[ND == 2 ? V : 1];
     {
       NodeState<R, TH> st;
-      node_state<R, TH>(q0, M, st);
-      source<R, TH>(q0, st, M, rhs);
+      node_state<R, TH, MK>(q0, M, st);
+      source<R, TH, MK>(q0, st, M, rhs);
       // B_d(q) . dq/dx_d reads only the distortion rows of dq (equations.cpp:189-247)
       R dq[14];
       {
@@ -411,11 +457,11 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       const R wb = R(ND == 2 ? K.w[ib] : 1.0);
       if (lane_ok)
 #pragma unroll
-        for (int v = 0; v < V; v++) myF[node * V + v] = F[v] * wb;
+        for (int i = 0; i < NF; i++) myF[node * FS + i] = F[flux_var(i, 0)] * wb;
       if constexpr (ND == 2) {
         flux<R, TH>(q0, st, M, 1, G);
 #pragma unroll
-        for (int v = 0; v < V; v++) pin(G[v]);
+        for (int i = 0; i < NF; i++) pin(G[flux_var(i, 1)]);
       }
     }
     wave_sync();
@@ -425,32 +471,32 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       const R cx = Wt * inv_dx;
       R c0 = Bdgder[ia * N + 0], c1 = Bdgder[ia * N + 1], c2 = N > 2 ? Bdgder[ia * N + N - 1] : R(0);
 #pragma unroll
-      for (int v = 0; v < V; v++) {
+      for (int i = 0; i < NF; i++) {
         R sx = 0;
 #pragma unroll
         for (int j = 0; j < N; j++)
-          sx += (j == 0 ? c0 : j == 1 ? c1 : c2) * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * V + v];
-        rhs[v] -= cx * sx;
-        pin(rhs[v]);
-        GPR_CHUNK_FENCE(v);
+          sx += (j == 0 ? c0 : j == 1 ? c1 : c2) * myF[(t * NS + (ND == 1 ? j : j * N + ib)) * FS + i];
+        rhs[flux_var(i, 0)] -= cx * sx;
+        pin(rhs[flux_var(i, 0)]);
+        GPR_CHUNK_FENCE(i);
       }
     }
     if constexpr (ND == 2) {
       wave_sync();
       if (lane_ok)
 #pragma unroll
-        for (int v = 0; v < V; v++) myF[node * V + v] = G[v];
+        for (int i = 0; i < NF; i++) myF[node * FS + i] = G[flux_var(i, 1)];
       wave_sync();
       const R cy = Wt * R(K.w[ia]) * inv_dy;
       R c0 = Bdgder[ib * N + 0], c1 = Bdgder[ib * N + 1], c2 = N > 2 ? Bdgder[ib * N + N - 1] : R(0);
 #pragma unroll
-      for (int v = 0; v < V; v++) {
+      for (int i = 0; i < NF; i++) {
         R sy = 0;
 #pragma unroll
-        for (int j = 0; j < N; j++) sy += (j == 0 ? c0 : j == 1 ? c1 : c2) * myF[(t * NS + ia * N + j) * V + v];
-        rhs[v] -= cy * sy;
-        pin(rhs[v]);
-        GPR_CHUNK_FENCE(v);
+        for (int j = 0; j < N; j++) sy += (j == 0 ? c0 : j == 1 ? c1 : c2) * myF[(t * NS + ia * N + j) * FS + i];
+        rhs[flux_var(i, 1)] -= cy * sy;
+        pin(rhs[flux_var(i, 1)]);
+        GPR_CHUNK_FENCE(i);
       }
     }
 #pragma unroll
@@ -545,12 +591,16 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     }
     const unsigned long long bm = __ballot(more), bn = __ballot(nan);
     const bool cmore = (bm & cellbits) != 0, cnan = (bn & cellbits) != 0;
-    if (a.bailout > 0) {
-      sRat[lane] = ratio;
-      wave_sync();
-      R cr = 0;
-      for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
-      ratio = cr;
+    if (a.bailout > 0) {  // the cell's largest ratio, in each of its lanes
+      if constexpr (LPC == 32) {
+        ratio = R(half_wave_max(lane_ok ? double(ratio) : 0.0));
+      } else {
+        sRat[lane] = ratio;
+        wave_sync();
+        R cr = 0;
+        for (int k = 0; k < NN; k++) cr = fmax(cr, sRat[sl * NN + k]);
+        ratio = cr;
+      }
     }
 #pragma unroll
     for (int v = 0; v < V; v++) q0[v] = rhs[v];  // the next iterate (reset below on a restart)
