@@ -34,35 +34,36 @@ sys.path.insert(0, ROOT)
 METRIC = "ADER-WENO cell-updates/s, 2D 512² P2: plain FP64 vs batched/FP32/adjoint"
 
 # ---------------------------------------------------------------------------
-# Algorithmic FP64 operation counts (add, sub, mul, div, sqrt, cbrt = 1 flop;
-# fused multiply-add = 2) of one DG-predictor iteration at one space-time
-# node, 2D, V=14, Newtonian ideal gas, counted from the algorithm's formulas
-# (dg2.cpp:8-71 right-hand side, the DG_U2 solve in its factorised form,
-# the stopping test dg2.cpp:280).  Each entry is the count of the named part
-# as the formulas state it, not of the compiled instructions.
-FLOPS_NODE = {
-    "state (pressure, A^T A, dev G, dE/dA, stress)": 160,
-    "source (theta1^-1, S_A)": 30,
-    "spatial derivatives (2 x V x N FMA, scaling)": 196,
-    "non-conservative B_x, B_y products": 66,
-    "space-time weights": 14,
-    "fluxes F, G (+ y-weight)": 76,
-    "flux contractions, dt, initial term": 269,
-    "time solve (DG_MAT^-1 (x) W^-1)": 98,
-    "stopping test": 84,
+# Algorithmic FP64 operation counts, frozen from phd_amd/csrc/flopcount.cpp
+# (built as phd_amd/flopcount by phd_amd.build; tests/test_flopcount.py checks
+# that these numbers are what it prints).  That program instantiates the
+# kernels' own templated pointwise physics with a counting number type and
+# replays the contractions between nodes as the kernels do them per lane:
+# +, -, *, / count 1 each (a fused multiply-add 2), sqrt, cbrt, pow, exp,
+# cos, acos 1 each; loop-invariant set-up is not counted per iteration.
+# Per space-time node and iteration of the predictor (P2, 2D, V = 14), per
+# spatial node for the fallback's set-up, and per cell-update for the other
+# stages at the interior of the grid.
+FLOPS = {
+    "gas_newtonian_2d_p2": {"picard_node": 788, "fallback_node": 1083, "fallback_setup_spatial": 611,
+                            "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
+                            "fv_face_lane": 2133, "fv_cell_node": 408, "fv_cell": 49410,
+                            "post_cell": 1235},
+    "gr_plastic_2d_p2": {"picard_node": 824, "fallback_node": 1119, "fallback_setup_spatial": 647,
+                         "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
+                         "fv_face_lane": 2077, "fv_cell_node": 444, "fv_cell": 49374,
+                         "post_cell": 1296},
 }
-PICARD_NODE = sum(FLOPS_NODE.values())
-# preconditioned update: residual g (112) + projector (120) + update (165)
-# replace the plain time solve (98)
-FALLBACK_NODE = PICARD_NODE - 98 + 112 + 120 + 165
-# preconditioner set-up per spatial node: state (160), theta (21), two 3x3
-# inverses (~100), polar rotation (8 Newton steps x ~60)
-FALLBACK_SETUP_SPATIAL = 760
 
 
-def predictor_flops(picard_iters, fallback_iters, fallback_cells, nodes=27, spatial=9):
-    return (nodes * (PICARD_NODE * picard_iters + FALLBACK_NODE * fallback_iters)
-            + spatial * FALLBACK_SETUP_SPATIAL * fallback_cells)
+def predictor_flops(picard_iters, fallback_iters, fallback_cells, nodes=27, spatial=9,
+                    material="gas_newtonian_2d_p2"):
+    """FP64 operations of one predictor launch from its iteration totals:
+    nodes x (Picard iterations x per-node cost + fallback iterations x theirs)
+    + spatial nodes x set-up per fallback cell."""
+    f = FLOPS[material]
+    return (nodes * (f["picard_node"] * picard_iters + f["fallback_node"] * fallback_iters)
+            + spatial * f["fallback_setup_spatial"] * fallback_cells)
 
 
 MI355X_FP64_TFLOPS = 78.6   # dense FP64 (vector and matrix) peak
@@ -113,6 +114,63 @@ def cpu_port(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, bailout):
     for _ in range(nsteps):
         u, t, c, dt, rc, st = O.step(u, nX, dX, 0.8, bcs, t, c, 1e300, MP)
     return time.perf_counter() - tic
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_leg(u_w, MP, nX, dX, bcs, st0, args, threads, gpu_value):
+    """The CPU rate over the GPU's own timed window: the device states at the
+    window's first, middle and last steps (taken from a second device run
+    from the same start), one oracle step from each on the host cores (the
+    same algorithm and bailout as the GPU: kind "port"), and the reference's
+    own step from the middle one.  Returns the cpu_baseline object."""
+    from phd_amd.runner import DeviceRun
+    n = nX[0]
+    cells = n * nX[1]
+    w0, K = args.warmup, args.steps
+    picks = sorted({w0, w0 + K // 2, w0 + K - 1})
+    run = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout, mode=args.mode,
+                    t0=st0["t"], count0=st0["count"])
+    states = []
+    for step in picks:
+        run.steps(step - run.status()["count"])
+        s = run.status()
+        states.append((run.host(), s["t"], s["count"]))
+    del run
+    secs = [cpu_port(u, MP, nX, dX, bcs, t, c, 1, threads, args.bailout) for u, t, c in states]
+    pval = cells * len(secs) / sum(secs)
+    u, t, c = states[len(states) // 2]
+    ref = cpu_reference(u, MP, nX, dX, bcs, t, c, 1, threads)
+    out = {
+        "value": pval, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+        "cpu_model": cpu_model(),
+        "sample": f"one step from each of steps {picks} of the GPU's timed window (steps {w0}-"
+                  f"{w0 + K - 1}, device states), oracle/liboracle.so (the CPU restatement of the same "
+                  f"algorithm, Picard bailout {args.bailout}) with OpenMP over {threads} threads; "
+                  f"seconds per step {[round(x, 3) for x in secs]}",
+        "gpu_over_cpu": gpu_value / pval,
+    }
+    if ref is not None and "error" not in ref:
+        rval = cells * ref["steps"] / ref["sec"]
+        out["reference_value"] = rval
+        out["reference_sample"] = (
+            f"the reference C++ (oracle/_ref, built from its sources, -O3 -march=x86-64-v3, OpenMP "
+            f"over {threads} threads) on step {c} of the same window from the device state; its "
+            f"Picard runs to DG_IT = 50 where it diverges and leaves those cells unassigned, so its "
+            f"state after the step is {'non-finite' if ref['status'] else 'finite'}")
+        out["gpu_over_reference"] = gpu_value / rval
+    elif ref is not None:
+        out["reference_error"] = ref["error"]
+    return out
 
 
 def copy_probe(mib):
@@ -451,7 +509,6 @@ def main():
                     help="steps of the whole-job run from t = 0 (0 skips it)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-faithful", action="store_true",
                     help="skip the second timed run with bailout = 0")
@@ -506,8 +563,10 @@ def main():
     flops_launch = predictor_flops(tm["picard_iters"] / K, tm["fallback_iters"] / K,
                                    tm["fallback_cells"] / K)
     achieved = flops_launch / (pred_ms * 1e-3) / 1e12
+    f = FLOPS["gas_newtonian_2d_p2"]
+    pred_cell = flops_launch / cells
     roofline = {
-        "bound": "mfma", "pipe": "fp64 VALU (peak equal to dense FP64 MFMA on MI355X)",
+        "bound": "fp64-valu", "pipe": "FP64 vector ALU (peak 78.6 TFLOP/s, the same as dense FP64 MFMA)",
         "kernel": "k_predictor<3,false,2>", "achieved": round(achieved, 3),
         "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / MI355X_FP64_TFLOPS, 4),
@@ -517,6 +576,12 @@ def main():
         "fallback_iters_per_cell": tm["fallback_iters"] / max(tm["cells"], 1),
         "fallback_cell_fraction": tm["fallback_cells"] / max(tm["cells"], 1),
         "phase_ms_per_step": {k: v / K for k, v in tm["ms"].items()},
+        "flops_per_cell_update": {
+            "weno": f["weno_cell"], "predictor": pred_cell, "fv": f["fv_cell"],
+            "post_and_cfl": f["post_cell"],
+            "total": f["weno_cell"] + pred_cell + f["fv_cell"] + f["post_cell"],
+            "source": "phd_amd/csrc/flopcount.cpp (instrumented); the predictor's from the "
+                      "iteration totals counted on the device in the timed launches"},
         "iterations_first_timed_step": iteration_histogram(u_w, MP, nX, dX, bcs, st0["t"],
                                                            st0["count"], args.bailout, args.mode),
     }
@@ -560,30 +625,8 @@ def main():
         progress("split leg done")
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ref = cpu_reference(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps,
-                            threads)
-        progress("CPU reference timed")
-        psec = cpu_port(u_w, MP, nX, dX, bcs, st0["t"], st0["count"], args.cpu_steps, threads,
-                        args.bailout)
-        pval = cells * args.cpu_steps / psec
-        if ref is not None and "error" not in ref:
-            cval = cells * ref["steps"] / ref["sec"]
-            out["cpu_baseline"] = {
-                "value": cval, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
-                "sample": f"{ref['steps']} step(s) of the same {n}x{n} run from the GPU's timed "
-                          f"starting state (step {st0['count']}), the reference C++ built from its "
-                          f"sources (oracle/_ref, -O3 -march=x86-64-v3, OpenMP); its predictor "
-                          f"leaves the cells whose Picard iteration does not converge unassigned, "
-                          f"so its state after the step is "
-                          f"{'non-finite' if ref['status'] else 'finite'}",
-                "gpu_over_cpu": value / cval, "port_value": pval}
-        else:
-            out["cpu_baseline"] = {
-                "value": pval, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-                "sample": f"{args.cpu_steps} step(s) of the same {n}x{n} run from the GPU's timed "
-                          f"starting state, oracle/liboracle.so with OpenMP"
-                          + (f" (the reference's own run failed: {ref['error']})" if ref else ""),
-                "gpu_over_cpu": value / pval}
+        out["cpu_baseline"] = cpu_baseline_leg(u_w, MP, nX, dX, bcs, st0, args, threads, value)
+        progress("CPU baseline timed")
     print(json.dumps(out))
 
 

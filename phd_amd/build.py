@@ -82,7 +82,22 @@ def build(verbose=False, jobs=None):
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_flopcount()
     return LIB
+
+
+FLOPCOUNT = os.path.join(HERE, "flopcount")
+
+
+def build_flopcount():
+    """The host program that counts the kernels' FP64 operations
+    (csrc/flopcount.cpp, the physics headers compiled for the host)."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no host C++ compiler for flopcount")
+    subprocess.check_call([cxx, "-std=c++17", "-O1", "-DGPR_HOST_ONLY", "-Wno-unknown-pragmas",
+                           os.path.join(CSRC, "flopcount.cpp"), "-o", FLOPCOUNT])
+    return FLOPCOUNT
 
 
 if __name__ == "__main__":

@@ -948,7 +948,7 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   bool failed = false, have_pre = false, used_pre = phase == 1;
   while (__ballot(phase < 2) != 0ull) {
     if (!have_pre && __ballot(phase == 1) != 0ull) {
-      if (active && t == 0) stiff_precond<N, TH>(a.wh + (e * NS + s) * V, a.m, dt, sPre + (sl * NS + s) * PRE);
+      if (active && t == 0) stiff_precond<N, TH>(a.wh + (e * NS + s) * V, a.m, dt, K, sPre + (sl * NS + s) * PRE);
       have_pre = true;
       __syncthreads();
     }

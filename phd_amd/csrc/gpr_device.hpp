@@ -11,7 +11,17 @@
 // reference does.  State layout per cell: [ρ, ρE, ρv(3), A row-major (9), ρJ(3)].
 #pragma once
 
+#ifdef GPR_HOST_ONLY
+// host compile of the pointwise physics (flopcount.cpp): the templated
+// functions below are instantiated with a counting number type
+#include <cmath>
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __constant__
+#else
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "../../include/gpr_par.h"
@@ -47,9 +57,11 @@ template <int N> __device__ __forceinline__ const Basis &basis_dev() { return c_
 // merge the two sources into a load through a selected pointer, which would
 // take the address of the by-value kernel argument holding v and copy the
 // whole argument to the stack.
+#ifndef GPR_HOST_ONLY
 __device__ __forceinline__ double dev_or(const double *p, double v) {
   return p ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : v;
 }
+#endif
 
 // Material parameters plus the few derived scalars the kernels use often.
 struct Mat {
@@ -355,6 +367,16 @@ __device__ __forceinline__ void flux(const R *q, const NodeState<R, TH> &s, cons
     F[14 + d] += s.T;
   }
 }
+
+// The flux components that can be nonzero in direction d (equations.cpp:11-48):
+// mass, energy, momentum, column d of the distortion (F[5 + d + 3k]) and the
+// thermal rows.  The other six distortion components are zero, so the
+// predictor neither stores nor contracts them.
+template <bool TH> struct FluxPack {
+  static constexpr int NF = TH ? 11 : 8;        // nonzero components
+  static constexpr int STRIDE = TH ? 14 : 10;   // LDS doubles per node (16-byte aligned records)
+};
+__host__ __device__ constexpr int flux_var(int i, int d) { return i < 5 ? i : i < 8 ? 5 + d + 3 * (i - 5) : 6 + i; }
 
 // out -= scale * B_d(q) . x  on the distortion rows (equations.cpp:189-267).
 template <typename R>

@@ -18,8 +18,11 @@ namespace gpr {
 template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_pair(hipStream_t s, const FvArgs &a) {
   const long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
   constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
+  // k_fv_faces maps block b to work item (b % 8) * M + b / 8 with
+  // M = ceil(items / 8): the grid is 8 M blocks, those past the items idle
+  const long items = (nf + FPW - 1) / FPW;
   if (a.parts & 2)
-    hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3((nf + FPW - 1) / FPW), dim3(WAVE), 0, s, a);
+    hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3(8 * ((items + 7) / 8)), dim3(WAVE), 0, s, a);
   const long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
   constexpr int CPW = CellShape<N, ND, TIME>::CPW;
   hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, s, a);

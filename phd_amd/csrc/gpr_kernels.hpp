@@ -16,6 +16,7 @@
 
 #include <type_traits>
 
+#include "gpr_pointwise.hpp"
 #include "gpr_relax.hpp"
 
 namespace gpr {
@@ -120,87 +121,6 @@ __global__ void k_boundaries(const double *__restrict__ u, double *__restrict__ 
   }
 }
 
-// ---------------------------------------------------------------------- WENO
-// Reconstruction of one variable over one stencil line (coeffs, weno.cpp:17-73)
-// in difference form about the central value c: w_s = c + M_s^-1 (d - c), whose
-// oscillation indicator and weighted mean are those of the direct form exactly
-// (the basis sums to one, so M_s^-1 maps constants to themselves and SIG
-// annihilates them) but do not cancel when |c| >> |d - c|.
-template <typename R, int N>
-__device__ __forceinline__ void weno_point(const R *d, const Basis &K, int ncell, int gap, int ind, R *out) {
-  constexpr int FN2 = (N - 1) / 2;
-  R c = d[N - 1];
-  R dd[2 * N - 1];
-#pragma unroll
-  for (int r = 0; r < 2 * N - 1; r++) dd[r] = d[r] - c;
-  R ws[3][N];
-#pragma unroll
-  for (int a = 0; a < N; a++) {
-    R sl = 0, sr = 0, sc = 0;
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      sl += R(K.mLinv[a][k]) * dd[k];
-      sr += R(K.mRinv[a][k]) * dd[N - 1 + k];
-      sc += R(K.mCLinv[a][k]) * dd[FN2 + k];
-    }
-    ws[0][a] = sl;
-    ws[1][a] = sr;
-    ws[2][a] = sc;
-  }
-  constexpr int NSTEN = N > 2 ? 3 : 2;
-  bool useL = ind >= gap, useR = ind <= ncell - gap - 1, useC = N > 2 && ind + FN2 >= gap;
-  R o[3], tt[3];
-#pragma unroll
-  for (int s = 0; s < NSTEN; s++) {
-    R qv = 0;
-#pragma unroll
-    for (int i = 0; i < N; i++)
-#pragma unroll
-      for (int j = 0; j < N; j++) qv += ws[s][i] * R(K.sig[i][j]) * ws[s][j];
-    tt[s] = qv + R(1e-14);
-  }
-  if constexpr (!std::is_same<R, float>::value) {
-#pragma unroll
-    for (int s = 0; s < NSTEN; s++) {
-      R t2 = tt[s] * tt[s], t4 = t2 * t2;
-      o[s] = R(s == 2 ? 1e5 : 1.0) / (t4 * t4);
-    }
-  } else {
-    // lambda_s / t_s^8 scaled by the smallest t^8 among the stencils in use:
-    // the normalised weights are unchanged, and t^8 (down to 1e-112) would
-    // underflow in single precision
-    R tmin = R(3e38);
-    if (useL) tmin = fmin(tmin, tt[0]);
-    if (useR) tmin = fmin(tmin, tt[1]);
-    if (useC) tmin = fmin(tmin, tt[NSTEN - 1]);
-#pragma unroll
-    for (int s = 0; s < NSTEN; s++) {
-      R r = tmin / tt[s], r2 = r * r, r4 = r2 * r2;
-      o[s] = R(s == 2 ? 1e5 : 1.0) * (r4 * r4);
-    }
-  }
-  R num[N], den = 0;
-#pragma unroll
-  for (int a = 0; a < N; a++) num[a] = 0;
-  if (useL) {
-#pragma unroll
-    for (int a = 0; a < N; a++) num[a] += ws[0][a] * o[0];
-    den += o[0];
-  }
-  if (useR) {
-#pragma unroll
-    for (int a = 0; a < N; a++) num[a] += ws[1][a] * o[1];
-    den += o[1];
-  }
-  if (useC) {
-#pragma unroll
-    for (int a = 0; a < N; a++) num[a] += ws[2][a] * o[2];
-    den += o[2];
-  }
-#pragma unroll
-  for (int a = 0; a < N; a++) out[a] = c + num[a] / den;
-}
-
 // 1D: wh[(nx+2)][N][V] from ub[(nx+2N)][V] (weno_launcher ndim 1, weno.cpp:145-147)
 template <typename R, int N, int V>
 __global__ void k_weno_1d(const double *__restrict__ ub, R *__restrict__ wh, int nx, Basis K) {
@@ -259,64 +179,6 @@ __global__ void k_weno_y(const R *__restrict__ ux, R *__restrict__ wh, int nx, i
   }
 }
 
-// ------------------------------------------------------------------------ FV
-// Rusanov flux D_RUS (fluxes.cpp:66-75) and the path integral Bint
-// (fluxes.cpp:9-20) between two states, accumulated as
-//   lo += k (b + f),  hi += k (b - f).
-template <bool TH, typename R = double>
-__device__ __forceinline__ void rusanov(const R *qL, const R *qR, int d, const Mat &m, R *f) {
-  constexpr int V = Dims<TH>::V;
-  R sL = max_abs_eigs<TH, R>(qL, m, d), sR = max_abs_eigs<TH, R>(qR, m, d);
-  R smax = (sL < sR) ? sR : sL;  // std::max(max1, max2): a NaN on the right is dropped
-  R F[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) f[v] = smax * (qL[v] - qR[v]);
-  {
-    NodeState<R, TH> st;
-    node_state<R, TH>(qL, m, st);
-    flux<R, TH>(qL, st, m, d, F);
-#pragma unroll
-    for (int v = 0; v < V; v++) f[v] += F[v];
-    node_state<R, TH>(qR, m, st);
-    flux<R, TH>(qR, st, m, d, F);
-#pragma unroll
-    for (int v = 0; v < V; v++) f[v] += F[v];
-  }
-}
-
-template <int N, bool TH, typename R = double>
-__device__ __forceinline__ void bint(const R *qL, const R *qR, int d, const Basis &K, R *b) {
-  constexpr int V = Dims<TH>::V;
-#pragma unroll
-  for (int v = 0; v < V; v++) b[v] = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    R xi = K.x[i];
-    R r = qL[0] + xi * (qR[0] - qL[0]);
-    R vv[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) vv[c] = (qL[2 + c] + xi * (qR[2 + c] - qL[2 + c])) / r;
-    R dq[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) dq[v] = qR[v] - qL[v];
-    bdot_sub(vv, dq, d, R(-K.w[i]), b);
-  }
-}
-
-template <int N, bool TH, typename R = double>
-__device__ __forceinline__ void riemann_accum(const R *qL, const R *qR, int d, const Mat &m,
-                                              const Basis &K, double k, R *lo, R *hi) {
-  constexpr int V = Dims<TH>::V;
-  R f[V], b[V];
-  rusanov<TH, R>(qL, qR, d, m, f);
-  bint<N, TH, R>(qL, qR, d, K, b);
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    lo[v] += k * (b[v] + f[v]);
-    hi[v] += k * (b[v] - f[v]);
-  }
-}
-
 struct FvArgs {
   double *u;             // [cells][V]
   const double *qh;      // [(nx+2)(ny+2)][NN][V]
@@ -367,21 +229,43 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   const int lane = threadIdx.x;
   const int fl = lane / NT, r = lane - fl * NT;
   const int t = r / NSF, sf = r - t * NSF;
-  const long id0 = (long)blockIdx.x * FPW;
-  const long id = id0 + fl;
-  const bool ok = fl < FPW && id < nfx + nfy;
-  const int d = id < nfx ? 0 : 1;
-  long e0 = 0, e1 = 0;
+  // Work order.  Faces are numbered row by row: row i's x-faces (between
+  // cell rows i and i+1, ny of them), then its y-faces (ny + 1), for
+  // i = 0..nx-1, then the last row of x-faces; every cell's qh is read by
+  // its four faces within about two rows of work.  Blocks are dispatched to
+  // the XCDs round-robin (block b on XCD b % 8), so block b takes item
+  // (b % 8) * M + b / 8: each XCD works through one contiguous band of rows,
+  // and the re-reads of a cell's qh hit that XCD's L2.
+  const long nitems = (nfx + nfy + FPW - 1) / FPW;
+  const long M = (nitems + 7) / 8;
+  const long b = blockIdx.x;
+  long item = (b % 8) * M + b / 8;
+  const bool item_ok = item < nitems;
+  const long id0 = item * FPW;
+  const long id = id0 + fl;  // the face in row order
+  const bool ok = item_ok && fl < FPW && id < nfx + nfy;
+  int d = 0;
+  long e0 = 0, e1 = 0, fid = 0;  // fid: the face's slot in fx (d = 0) or fy (d = 1)
   if (ok) {
-    if (d == 0) {
-      int i = id / ny, j = id % ny;
-      e0 = ND == 2 ? (long)i * (ny + 2) + (j + 1) : i;
-      e1 = ND == 2 ? (long)(i + 1) * (ny + 2) + (j + 1) : i + 1;
+    if (ND == 1) {
+      fid = id;
+      e0 = id;
+      e1 = id + 1;
     } else {
-      long k = id - nfx;
-      int i = k / (ny + 1), j = k % (ny + 1);
-      e0 = (long)(i + 1) * (ny + 2) + j;
-      e1 = e0 + 1;
+      const long per = 2L * ny + 1;  // faces of one row
+      const long i = id / per, k = id - i * per;
+      if (i == nx || k < ny) {  // x-face (i, j)
+        const long j = k;
+        fid = i * ny + j;
+        e0 = i * (ny + 2) + (j + 1);
+        e1 = (i + 1) * (ny + 2) + (j + 1);
+      } else {  // y-face (i, j) of row i
+        d = 1;
+        const long j = k - ny;
+        fid = i * (ny + 1) + j;
+        e0 = (i + 1) * (ny + 2) + j;
+        e1 = e0 + 1;
+      }
     }
   }
   double lo[V], hi[V];
@@ -416,6 +300,7 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
     else
       riemann_accum<N, TH>(qL, qR, 1, a.m, K, kk, lo, hi);
   }
+  __shared__ long sOut[FPW];  // each face's output slot: fid in fx, or nfx + fid in fy
   if (fl < FPW) {
     double *my = sR + (fl * NT + r) * 2 * V;
 #pragma unroll
@@ -423,17 +308,18 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
       my[v] = lo[v];
       my[V + v] = hi[v];
     }
+    if (r == 0) sOut[fl] = ok ? (d == 0 ? fid : nfx + fid) : -1;
   }
   __syncthreads();
   // one lane per (face, output): sum the face's NT lanes in a fixed order
   for (int o = lane; o < FPW * 2 * V; o += WAVE) {
     const int f = o / (2 * V), k = o - f * 2 * V;
-    const long fid = id0 + f;
-    if (fid >= nfx + nfy) continue;
+    const long slot = sOut[f];
+    if (slot < 0) continue;
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < NT; q++) acc += sR[(f * NT + q) * 2 * V + k];
-    double *out = fid < nfx ? a.fx + fid * 2 * V : a.fy + (fid - nfx) * 2 * V;
+    double *out = slot < nfx ? a.fx + slot * 2 * V : a.fy + (slot - nfx) * 2 * V;
     out[k] = acc;
   }
 }

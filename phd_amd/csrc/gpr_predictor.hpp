@@ -18,7 +18,7 @@
 // which solves the same equations and reduces to Picard when kappa = rJ = 0.
 #pragma once
 
-#include "gpr_device.hpp"
+#include "gpr_pointwise.hpp"
 
 // waves per SIMD the predictor is compiled for (register budget 512 / waves)
 #ifndef GPR_PRED_WAVES
@@ -85,15 +85,6 @@ template <int N, int ND> struct PredShape {
   static constexpr int CPW = WAVE / LPC;           // cells per wavefront
 };
 
-// The flux components that can be nonzero in direction d (equations.cpp:11-48):
-// mass, energy, momentum, column d of the distortion (F[5 + d + 3k]) and the
-// thermal rows.  The other six distortion components are zero, so the
-// predictor neither stores nor contracts them.
-template <bool TH> struct FluxPack {
-  static constexpr int NF = TH ? 11 : 8;        // nonzero components
-  static constexpr int STRIDE = TH ? 14 : 10;   // LDS doubles per node (16-byte aligned records)
-};
-__host__ __device__ constexpr int flux_var(int i, int d) { return i < 5 ? i : i < 8 ? 5 + d + 3 * (i - 5) : 6 + i; }
 
 // max over each 32-lane half of the wavefront of a non-negative double (NaN
 // counts as 0, as fmax ignores it), for every lane of the half.  Doubles >= 0
@@ -122,103 +113,6 @@ __device__ __forceinline__ double half_wave_max(double x) {
   const unsigned l = upper ? __builtin_amdgcn_readlane(lo, 63) : __builtin_amdgcn_readlane(lo, 31);
   const unsigned h = upper ? __builtin_amdgcn_readlane(hi, 63) : __builtin_amdgcn_readlane(hi, 31);
   return __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
-}
-
-// Preconditioner of one spatial node, frozen at its WENO state ws:
-// out[0..8] = (DG_MAT + dt kappa W)^-1, out[9..17] = polar rotation of A,
-// out[18..26] = (DG_MAT + dt rJ W)^-1 (thermal only).  At a relaxed state
-// A = s Rot the distortion source has Jacobian -kappa Pi with
-// Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3);
-// the thermal source -(c_alpha^2 T / (kappa rho)) rho J has rate rJ.
-template <int N, bool TH, int MK = MK_ANY>
-__device__ __forceinline__ void stiff_precond(const double *ws, const Mat &m, double dt, double *out) {
-  const Basis &K = basis_dev<N>();
-  NodeState<double, TH> s;
-  node_state<double, TH, MK>(ws, m, s);
-  double dA = det3(ws + 5);
-  double s13 = cbrt(dA);
-  double kap = 2.0 * theta1inv<double, TH, MK>(ws, s, m) * s.cs2 * s13 * s13;
-  if (!(kap >= 0.0) || !(kap < 1e300)) kap = 0.0;
-  double rJ = 0.0;
-  if (TH) {
-    rJ = m.p.calpha2 * s.T / (m.p.kappa * s.r);
-    if (!(rJ >= 0.0) || !(rJ < 1e300)) rJ = 0.0;
-  }
-#pragma unroll
-  for (int which = 0; which < (TH ? 2 : 1); which++) {
-    double rate = which == 0 ? kap : rJ;
-    // Gauss-Jordan with partial pivoting.  Every index is a compile-time
-    // constant (the pivot row is swapped in by predicated exchanges), so the
-    // matrix stays in registers rather than in scratch memory.
-    double M[N][2 * N];
-#pragma unroll
-    for (int i = 0; i < N; i++)
-#pragma unroll
-      for (int j = 0; j < 2 * N; j++)
-        M[i][j] = j < N ? K.dgmat[i][j] + (i == j ? dt * rate * K.w[i] : 0.0) : (j - N == i ? 1.0 : 0.0);
-#pragma unroll
-    for (int c = 0; c < N; c++) {
-      int piv = c;
-      double best = fabs(M[c][c]);
-#pragma unroll
-      for (int r = c + 1; r < N; r++)
-        if (fabs(M[r][c]) > best) {
-          best = fabs(M[r][c]);
-          piv = r;
-        }
-#pragma unroll
-      for (int r = c + 1; r < N; r++)
-        if (piv == r)
-#pragma unroll
-          for (int j = 0; j < 2 * N; j++) {
-            double tt = M[c][j];
-            M[c][j] = M[r][j];
-            M[r][j] = tt;
-          }
-      double dg = M[c][c];
-#pragma unroll
-      for (int j = 0; j < 2 * N; j++) M[c][j] /= dg;
-#pragma unroll
-      for (int r = 0; r < N; r++) {
-        if (r == c) continue;
-        double f = M[r][c];
-#pragma unroll
-        for (int j = 0; j < 2 * N; j++) M[r][j] -= f * M[c][j];
-      }
-    }
-    double *dst = which == 0 ? out : out + 18;
-#pragma unroll
-    for (int i = 0; i < N; i++)
-#pragma unroll
-      for (int j = 0; j < N; j++) dst[i * N + j] = M[i][N + j];
-  }
-  double X[9];
-  for (int i = 0; i < 9; i++) X[i] = ws[5 + i];
-  if (dA > 0) {
-    for (int it = 0; it < 40; it++) {  // X <- (X + X^-T) / 2
-      double d = det3(X), inv[9];
-      inv[0] = (X[4] * X[8] - X[5] * X[7]) / d;
-      inv[1] = (X[2] * X[7] - X[1] * X[8]) / d;
-      inv[2] = (X[1] * X[5] - X[2] * X[4]) / d;
-      inv[3] = (X[5] * X[6] - X[3] * X[8]) / d;
-      inv[4] = (X[0] * X[8] - X[2] * X[6]) / d;
-      inv[5] = (X[2] * X[3] - X[0] * X[5]) / d;
-      inv[6] = (X[3] * X[7] - X[4] * X[6]) / d;
-      inv[7] = (X[1] * X[6] - X[0] * X[7]) / d;
-      inv[8] = (X[0] * X[4] - X[1] * X[3]) / d;
-      double ch = 0;
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-          double nx = 0.5 * (X[i * 3 + j] + inv[j * 3 + i]);
-          ch = fmax(ch, fabs(nx - X[i * 3 + j]));
-          X[i * 3 + j] = nx;
-        }
-      if (ch < 1e-15) break;
-    }
-  } else {
-    for (int i = 0; i < 9; i++) X[i] = (i % 4 == 0) ? 1.0 : 0.0;
-  }
-  for (int i = 0; i < 9; i++) out[9 + i] = X[i];
 }
 
 // The predictor's workgroup is exactly one wavefront, so exchanging values
@@ -253,26 +147,6 @@ __device__ __forceinline__ double fast_div(double a, double b) {
   return a * r;
 }
 __device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-
-// Pi(X) = Rot dev(sym(Rot^T X))
-__device__ __forceinline__ void proj_dev_sym(const double *Rot, const double *X, double *out) {
-  double Y[9], S[9];
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-      Y[i * 3 + j] = Rot[0 * 3 + i] * X[0 * 3 + j] + Rot[1 * 3 + i] * X[1 * 3 + j] + Rot[2 * 3 + i] * X[2 * 3 + j];
-  double tr = (Y[0] + Y[4] + Y[8]) / 3.0;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) S[i * 3 + j] = 0.5 * (Y[i * 3 + j] + Y[j * 3 + i]) - (i == j ? tr : 0.0);
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-      out[i * 3 + j] = Rot[i * 3 + 0] * S[0 * 3 + j] + Rot[i * 3 + 1] * S[1 * 3 + j] + Rot[i * 3 + 2] * S[2 * 3 + j];
-}
 
 // What one launch does (KIND):
 //   KIND_BOTH      Picard, and the preconditioned fallback in the same
@@ -401,7 +275,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         R w[V];
 #pragma unroll
         for (int v = 0; v < V; v++) w[v] = myW[s * V + v];
-        stiff_precond<N, TH, MK>(w, M, dt, myPre);
+        stiff_precond<N, TH, MK>(w, M, dt, K, myPre);
       }
     }
     need_pre = false;
