@@ -45,6 +45,7 @@ void orc_default_cfg(orc_cfg *c, int N, int thermal, int ndim) {
   c->stiff_tol = 6e-6;
   c->bailout = 0;
   c->fallback = 0;
+  c->flux = 0;
 }
 
 /* ------------------------------------------------------------------ constants
@@ -830,6 +831,437 @@ void orc_bint(double *ret, const double *qL, const double *qR, int d,
     for (int i = 0; i < V; i++)
       ret[i] += K->w[s] * tmp[i];
   }
+}
+
+/* ---------------------------------------------------------- Roe and Osher
+ * The system matrix M_d = dF_d/dP . dP/dQ + B_d (equations.cpp:169-176,
+ * jacobians.cpp:10-152) with P = (rho, p, v, A, J) the primitive variables,
+ * and the two Riemann solvers that take |M| from its eigendecomposition
+ * (fluxes.cpp:22-64). */
+
+/* dE/drho at constant p and A, one material (derivatives.cpp:11-45) */
+static double dEdrho_Q(const double *Q, double pp, const gpr_par *p) {
+  return dedrho(Q[0], pp, p) + dC_0drho(Q[0], p) / 4 * devGsq(Q + 5);
+}
+
+/* dsigma_ij / dA_mn at constant rho (state.cpp:71-84); A, G = A^T A and
+ * A dev(G) row-major */
+static double dsig_dA(double r, double cs2, const double *A, const double *G,
+                      const double *AdG, int i, int j, int m, int n) {
+  double t = A[m * 3 + i] * G[j * 3 + n] + A[m * 3 + j] * G[i * 3 + n] -
+             2. / 3. * G[i * 3 + j] * A[m * 3 + n];
+  if (i == n)
+    t += AdG[m * 3 + j];
+  if (j == n)
+    t += AdG[m * 3 + i];
+  return -r * cs2 * t;
+}
+
+/* dF_d/dP (jacobians.cpp:10-106), V x V row-major */
+static void jac_dFdP(const double *Q, int d, const gpr_par *p, int th, double *M) {
+  int V = th ? 17 : 14;
+  for (int i = 0; i < V * V; i++)
+    M[i] = 0.0;
+#define MM(i, j) M[(i) * V + (j)]
+  double r = Q[0], E = Q[1] / r, pp = orc_pressure(Q, p, th);
+  double v[3] = {Q[2] / r, Q[3] / r, Q[4] / r};
+  const double *A = Q + 5;
+  double Er = dEdrho_Q(Q, pp, p), Ep = dedp(r, p);
+  double vd = v[d], rvd = r * vd;
+  MM(0, 0) = vd;
+  MM(0, 2 + d) = r;
+  MM(1, 0) = (E + r * Er) * vd;
+  MM(1, 1) = (r * Ep + 1) * vd;
+  for (int k = 0; k < 3; k++)
+    MM(1, 2 + k) = r * vd * v[k];
+  MM(1, 2 + d) += r * E + pp;
+  for (int k = 0; k < 3; k++) {
+    MM(2 + k, 0) = vd * v[k];
+    MM(2 + k, 2 + k) = rvd;
+  }
+  for (int k = 0; k < 3; k++)
+    MM(2 + k, 2 + d) += r * v[k];
+  MM(2 + d, 1) = 1.0;
+
+  /* viscous part: the stress column sigma_{.d}, its rho derivative at
+   * constant A, and dE/dA = C_0 A dev(G) */
+  double sd[3], sr[3], G[9], AdG[9], psi[9];
+  double cs2 = c_s2(r, p), C0 = C_0(r, p);
+  sigma_col(Q, p, d, sd);
+  for (int k = 0; k < 3; k++)
+    sr[k] = (1 / r + dc_s2drho(r, p) / cs2) * sd[k];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      G[i * 3 + j] = A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j];
+  AdevG(A, AdG);
+  for (int k = 0; k < 9; k++)
+    psi[k] = C0 * AdG[k];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double t = 0.;
+      for (int k = 0; k < 3; k++) {
+        double ds = dsig_dA(r, cs2, A, G, AdG, d, k, i, j);
+        MM(2 + k, 5 + i * 3 + j) = -ds;
+        t -= v[k] * ds;
+      }
+      MM(1, 5 + i * 3 + j) = t;
+    }
+  MM(1, 0) -= sr[0] * v[0] + sr[1] * v[1] + sr[2] * v[2];
+  for (int k = 0; k < 3; k++)
+    MM(1, 2 + k) -= sd[k];
+  for (int k = 0; k < 9; k++)
+    MM(1, 5 + k) += r * vd * psi[k];
+  for (int k = 0; k < 3; k++)
+    MM(2 + k, 0) -= sr[k];
+  for (int row = 0; row < 3; row++)
+    for (int k = 0; k < 3; k++) {
+      MM(5 + 3 * row + d, 2 + k) = A[row * 3 + k];
+      MM(5 + 3 * row + d, 5 + 3 * row + k) = v[k];
+    }
+
+  if (th) {
+    double T = temperature_mg(r, pp, p), Tr = dTdrho(r, pp, p), Tp = dTdp(r, p);
+    double J[3] = {Q[14] / r, Q[15] / r, Q[16] / r};
+    double H[3] = {p->calpha2 * Q[14] / r, p->calpha2 * Q[15] / r, p->calpha2 * Q[16] / r};
+    MM(1, 0) += Tr * H[d];
+    MM(1, 1) += Tp * H[d];
+    for (int k = 0; k < 3; k++)
+      MM(1, 14 + k) = rvd * H[k];
+    MM(1, 14 + d) += p->calpha2 * T;
+    for (int k = 0; k < 3; k++)
+      MM(14 + k, 0) = vd * J[k];
+    MM(14 + d, 0) += Tr;
+    MM(14 + d, 1) = Tp;
+    for (int k = 0; k < 3; k++) {
+      MM(14 + k, 2 + d) = r * J[k];
+      MM(14 + k, 14 + k) = rvd;
+    }
+  }
+#undef MM
+}
+
+/* dP/dQ (jacobians.cpp:108-152), V x V row-major */
+static void jac_dPdQ(const double *Q, const gpr_par *p, int th, double *M) {
+  int V = th ? 17 : 14;
+  for (int i = 0; i < V * V; i++)
+    M[i] = i % (V + 1) == 0 ? 1.0 : 0.0;
+#define MM(i, j) M[(i) * V + (j)]
+  double r = Q[0], E = Q[1] / r, pp = orc_pressure(Q, p, th);
+  double v[3] = {Q[2] / r, Q[3] / r, Q[4] / r};
+  double Er = dEdrho_Q(Q, pp, p), Ep = dedp(r, p);
+  double Gi = 1 / (r * Ep);
+  double t = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - (E + r * Er);
+  double J[3] = {0, 0, 0};
+  if (th) {
+    for (int k = 0; k < 3; k++)
+      J[k] = Q[14 + k] / r;
+    t += p->calpha2 * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+  }
+  MM(1, 0) = Gi * t;
+  MM(1, 1) = Gi;
+  for (int k = 0; k < 3; k++) {
+    MM(1, 2 + k) = -Gi * v[k];
+    MM(2 + k, 0) = -v[k] / r;
+    MM(2 + k, 2 + k) = 1 / r;
+  }
+  double AdG[9];
+  AdevG(Q + 5, AdG);
+  double C0 = C_0(r, p);
+  for (int k = 0; k < 9; k++)
+    MM(1, 5 + k) = -Gi * r * (C0 * AdG[k]);
+  if (th)
+    for (int k = 0; k < 3; k++) {
+      MM(1, 14 + k) = -Gi * (p->calpha2 * Q[14 + k] / r);
+      MM(14 + k, 0) = -J[k] / r;
+      MM(14 + k, 14 + k) = 1 / r;
+    }
+#undef MM
+}
+
+/* equations.cpp:169-176 with block() (equations.cpp:69-87) */
+void orc_system_matrix(double *ret, const double *Q, int d, const gpr_par *p, int thermal) {
+  int V = thermal ? 17 : 14;
+  double F[MAXV * MAXV], P[MAXV * MAXV];
+  jac_dFdP(Q, d, p, thermal, F);
+  jac_dPdQ(Q, p, thermal, P);
+  for (int i = 0; i < V; i++)
+    for (int j = 0; j < V; j++) {
+      double s = 0;
+      for (int k = 0; k < V; k++)
+        s += F[i * V + k] * P[k * V + j];
+      ret[i * V + j] = s;
+    }
+  double r = Q[0], vd = Q[2 + d] / r;
+  for (int i = 5; i < 14; i++)
+    ret[i * V + i] += vd;
+  for (int row = 0; row < 3; row++)
+    for (int k = 0; k < 3; k++)
+      ret[(5 + 3 * row + d) * V + 5 + 3 * row + k] -= Q[2 + k] / r;
+}
+
+/* out = Re(R |Lambda| R^-1 x) for a real n x n row-major J = R Lambda R^-1,
+ * the product D_ROE and D_OSH form from Eigen's EigenSolver and a pivoted QR
+ * solve (fluxes.cpp:34-37,56-59).  Here: Householder reduction to Hessenberg
+ * form, complex single-shift QR to a Schur form J = Z T Z^H (Wilkinson
+ * shifts, an exceptional shift every 10th iteration), the eigenvectors of T
+ * by back substitution (U unit upper triangular; a pivot below eps |T| is
+ * raised to it, as in EISPACK's hqr2), R = Z U, and R b = x by Gaussian
+ * elimination with partial pivoting.  Returns 0, or -1 if QR does not
+ * converge in 60 n iterations. */
+static int eig_abs_apply(const double *J, int n, const double *x, double *out) {
+  double complex H[MAXV][MAXV], Z[MAXV][MAXV], U[MAXV][MAXV], R[MAXV][MAXV], b[MAXV];
+  double Hr[MAXV][MAXV], Q[MAXV][MAXV];
+  const double eps = 2.220446049250313e-16;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      Hr[i][j] = J[i * n + j];
+      Q[i][j] = i == j;
+    }
+  /* Householder reduction to upper Hessenberg form, Q accumulated */
+  for (int k = 0; k < n - 2; k++) {
+    double v[MAXV], nrm = 0, tail = 0;
+    for (int i = k + 1; i < n; i++) {
+      v[i] = Hr[i][k];
+      nrm += v[i] * v[i];
+      if (i > k + 1)
+        tail += v[i] * v[i];
+    }
+    if (tail == 0.0)
+      continue;
+    nrm = sqrt(nrm);
+    double alpha = v[k + 1] >= 0 ? -nrm : nrm;
+    v[k + 1] -= alpha;
+    double vv = 0;
+    for (int i = k + 1; i < n; i++)
+      vv += v[i] * v[i];
+    double beta = 2.0 / vv;
+    for (int j = 0; j < n; j++) { /* H <- P H */
+      double s = 0;
+      for (int i = k + 1; i < n; i++)
+        s += v[i] * Hr[i][j];
+      s *= beta;
+      for (int i = k + 1; i < n; i++)
+        Hr[i][j] -= s * v[i];
+    }
+    for (int i = 0; i < n; i++) { /* H <- H P, Q <- Q P */
+      double s = 0, t = 0;
+      for (int j = k + 1; j < n; j++) {
+        s += Hr[i][j] * v[j];
+        t += Q[i][j] * v[j];
+      }
+      s *= beta;
+      t *= beta;
+      for (int j = k + 1; j < n; j++) {
+        Hr[i][j] -= s * v[j];
+        Q[i][j] -= t * v[j];
+      }
+    }
+    Hr[k + 1][k] = alpha;
+    for (int i = k + 2; i < n; i++)
+      Hr[i][k] = 0.0;
+  }
+  double norm = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      H[i][j] = Hr[i][j];
+      Z[i][j] = Q[i][j];
+      norm += fabs(Hr[i][j]);
+    }
+  if (norm == 0.0) {
+    for (int i = 0; i < n; i++)
+      out[i] = 0.0;
+    return 0;
+  }
+  /* complex QR on the active window [l, hi], bulge chased by rotations */
+  int hi = n - 1, its = 0, total = 0;
+  while (hi > 0) {
+    int l = hi;
+    for (; l > 0; l--) {
+      double s = cabs(H[l - 1][l - 1]) + cabs(H[l][l]);
+      if (s == 0.0)
+        s = norm;
+      if (cabs(H[l][l - 1]) <= eps * s) {
+        H[l][l - 1] = 0.0;
+        break;
+      }
+    }
+    if (l == hi) {
+      hi--;
+      its = 0;
+      continue;
+    }
+    if (++total > 60 * n)
+      return -1;
+    its++;
+    double complex mu;
+    if (its % 10 == 0) {
+      mu = H[hi][hi] + 0.75 * fabs(creal(H[hi][hi - 1]));
+    } else {
+      double complex a = H[hi - 1][hi - 1], bb = H[hi - 1][hi], c = H[hi][hi - 1],
+                     dd = H[hi][hi];
+      double complex tr2 = (a + dd) / 2, disc = csqrt((a - dd) * (a - dd) / 4 + bb * c);
+      double complex l1 = tr2 + disc, l2 = tr2 - disc;
+      mu = cabs(l1 - dd) <= cabs(l2 - dd) ? l1 : l2;
+    }
+    for (int k = l; k < hi; k++) {
+      double complex xx = k == l ? H[l][l] - mu : H[k][k - 1];
+      double complex yy = k == l ? H[l + 1][l] : H[k + 1][k - 1];
+      double rr = sqrt(creal(xx * conj(xx)) + creal(yy * conj(yy)));
+      if (rr == 0.0)
+        continue;
+      double complex c = xx / rr, s = yy / rr;
+      if (k > l) {
+        H[k][k - 1] = rr;
+        H[k + 1][k - 1] = 0.0;
+      }
+      for (int j = k; j < n; j++) { /* rows k, k+1 <- G rows */
+        double complex h0 = H[k][j], h1 = H[k + 1][j];
+        H[k][j] = conj(c) * h0 + conj(s) * h1;
+        H[k + 1][j] = -s * h0 + c * h1;
+      }
+      int top = k + 2 < hi ? k + 2 : hi;
+      for (int i = 0; i <= top; i++) { /* columns k, k+1 <- columns G^H */
+        double complex h0 = H[i][k], h1 = H[i][k + 1];
+        H[i][k] = h0 * c + h1 * s;
+        H[i][k + 1] = -h0 * conj(s) + h1 * conj(c);
+      }
+      for (int i = 0; i < n; i++) {
+        double complex z0 = Z[i][k], z1 = Z[i][k + 1];
+        Z[i][k] = z0 * c + z1 * s;
+        Z[i][k + 1] = -z0 * conj(s) + z1 * conj(c);
+      }
+    }
+  }
+  /* eigenvectors of the triangular T = H */
+  double tn = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = i; j < n; j++)
+      tn += cabs(H[i][j]);
+  double small = eps * tn;
+  for (int k = 0; k < n; k++) {
+    for (int j = 0; j < n; j++)
+      U[j][k] = 0.0;
+    U[k][k] = 1.0;
+    for (int j = k - 1; j >= 0; j--) {
+      double complex s = 0;
+      for (int m = j + 1; m <= k; m++)
+        s += H[j][m] * U[m][k];
+      double complex den = H[j][j] - H[k][k];
+      if (cabs(den) < small)
+        den = small;
+      U[j][k] = -s / den;
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      double complex s = 0;
+      for (int m = 0; m <= j; m++)
+        s += Z[i][m] * U[m][j];
+      R[i][j] = s;
+    }
+  /* R b = x, partial pivoting (R is overwritten) */
+  double complex A[MAXV][MAXV];
+  for (int i = 0; i < n; i++) {
+    b[i] = x[i];
+    for (int j = 0; j < n; j++)
+      A[i][j] = R[i][j];
+  }
+  for (int k = 0; k < n; k++) {
+    int piv = k;
+    for (int i = k + 1; i < n; i++)
+      if (cabs(A[i][k]) > cabs(A[piv][k]))
+        piv = i;
+    if (piv != k) {
+      for (int j = 0; j < n; j++) {
+        double complex t = A[k][j];
+        A[k][j] = A[piv][j];
+        A[piv][j] = t;
+      }
+      double complex t = b[k];
+      b[k] = b[piv];
+      b[piv] = t;
+    }
+    for (int i = k + 1; i < n; i++) {
+      double complex f = A[i][k] / A[k][k];
+      for (int j = k; j < n; j++)
+        A[i][j] -= f * A[k][j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double complex s = b[i];
+    for (int j = i + 1; j < n; j++)
+      s -= A[i][j] * b[j];
+    b[i] = s / A[i][i];
+  }
+  for (int i = 0; i < n; i++)
+    b[i] *= cabs(H[i][i]);
+  for (int i = 0; i < n; i++) {
+    double complex s = 0;
+    for (int j = 0; j < n; j++)
+      s += R[i][j] * b[j];
+    out[i] = creal(s);
+  }
+  return 0;
+}
+
+/* fluxes.cpp:45-64: |M(qbar)| (qL - qR) + F(qL) + F(qR), with
+ * M(qbar) = sum_i W_i M(qR + x_i (qL - qR)) */
+int orc_d_roe(double *ret, const double *qL, const double *qR, int d, const gpr_par *p,
+              const orc_cfg *c) {
+  const consts_t *K = get_consts(c->N);
+  int th = c->thermal, V = th ? 17 : 14;
+  double dq[MAXV], q[MAXV], M[MAXV * MAXV], Mb[MAXV * MAXV];
+  for (int i = 0; i < V; i++)
+    dq[i] = qL[i] - qR[i];
+  for (int i = 0; i < V * V; i++)
+    Mb[i] = 0.0;
+  for (int s = 0; s < c->N; s++) {
+    for (int i = 0; i < V; i++)
+      q[i] = qR[i] + K->x[s] * dq[i];
+    orc_system_matrix(M, q, d, p, th);
+    for (int i = 0; i < V * V; i++)
+      Mb[i] += K->w[s] * M[i];
+  }
+  int rc = eig_abs_apply(Mb, V, dq, ret);
+  orc_flux(ret, qL, d, p, th);
+  orc_flux(ret, qR, d, p, th);
+  return rc;
+}
+
+/* fluxes.cpp:22-43: sum_i W_i |M(qR + x_i (qL - qR))| (qL - qR) + F(qL) + F(qR) */
+int orc_d_osh(double *ret, const double *qL, const double *qR, int d, const gpr_par *p,
+              const orc_cfg *c) {
+  const consts_t *K = get_consts(c->N);
+  int th = c->thermal, V = th ? 17 : 14;
+  double dq[MAXV], q[MAXV], M[MAXV * MAXV], t[MAXV];
+  int rc = 0;
+  for (int i = 0; i < V; i++) {
+    dq[i] = qL[i] - qR[i];
+    ret[i] = 0.0;
+  }
+  for (int s = 0; s < c->N; s++) {
+    for (int i = 0; i < V; i++)
+      q[i] = qR[i] + K->x[s] * dq[i];
+    orc_system_matrix(M, q, d, p, th);
+    rc |= eig_abs_apply(M, V, dq, t);
+    for (int i = 0; i < V; i++)
+      ret[i] += K->w[s] * t[i];
+  }
+  orc_flux(ret, qL, d, p, th);
+  orc_flux(ret, qR, d, p, th);
+  return rc;
+}
+
+/* the Riemann solver the configuration selects (fv.cpp:129-139,202-212) */
+static void riemann_flux(double *ret, const double *qL, const double *qR, int d,
+                         const gpr_par *p, const orc_cfg *c) {
+  if (c->flux == 1)
+    orc_d_roe(ret, qL, qR, d, p, c);
+  else if (c->flux == 2)
+    orc_d_osh(ret, qL, qR, d, p, c);
+  else
+    orc_d_rus(ret, qL, qR, d, p, c);
 }
 
 /* ------------------------------------------------------------------ grid */
@@ -1625,7 +2057,7 @@ void orc_fv_parts(double *u, const double *qh, const int32_t *nX, double dt,
           L[v] = a;
           R[v] = b;
         }
-        orc_d_rus(f, L, R, 0, p, c);
+        riemann_flux(f, L, R, 0, p, c);
         orc_bint(bb, L, R, 0, p, c);
         if (i > 0)
           for (int v = 0; v < V; v++)
@@ -1708,7 +2140,7 @@ void orc_fv_parts(double *u, const double *qh, const int32_t *nX, double dt,
               L[v] = a;
               R[v] = b;
             }
-            orc_d_rus(f, L, R, d, p, c);
+            riemann_flux(f, L, R, d, p, c);
             orc_bint(bb, L, R, d, p, c);
             double xw = kk * K->w[s];
             int lo = d == 0 ? i > 0 : j > 0;

@@ -37,6 +37,7 @@ typedef struct orc_cfg {
   int fallback;  /* 0: preconditioned Picard (what the device code runs);
                     1: Newton on the local source with lagged transport (an
                        independent solver of the same equations, for tests) */
+  int flux;      /* Riemann solver of the FV interfaces: 0 Rusanov, 1 Roe, 2 Osher */
 } orc_cfg;
 
 /* per-call predictor statistics */
@@ -68,6 +69,13 @@ double orc_max_abs_eigs(const double *Q, int d, const gpr_par *p, int thermal);
 void orc_d_rus(double *ret, const double *qL, const double *qR, int d,
                const gpr_par *p, const orc_cfg *c);
 void orc_bint(double *ret, const double *qL, const double *qR, int d,
+              const gpr_par *p, const orc_cfg *c);
+/* system matrix dF_d/dQ + B_d, V x V row-major */
+void orc_system_matrix(double *ret, const double *Q, int d, const gpr_par *p, int thermal);
+/* Roe and Osher fluxes; 0, or -1 if an eigendecomposition did not converge */
+int orc_d_roe(double *ret, const double *qL, const double *qR, int d,
+              const gpr_par *p, const orc_cfg *c);
+int orc_d_osh(double *ret, const double *qL, const double *qR, int d,
               const gpr_par *p, const orc_cfg *c);
 /* general real eigenvalues of an n x n row-major matrix (n <= 4) */
 int orc_eigenvalues(const double *a, int n, double *wr, double *wi);

@@ -21,7 +21,8 @@ class Cfg(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("thermal", ctypes.c_int), ("ndim", ctypes.c_int),
                 ("dg_it", ctypes.c_int), ("dg_tol", ctypes.c_double),
                 ("stiff_it", ctypes.c_int), ("stiff_tol", ctypes.c_double),
-                ("bailout", ctypes.c_int), ("fallback", ctypes.c_int)]
+                ("bailout", ctypes.c_int), ("fallback", ctypes.c_int),
+                ("flux", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -67,7 +68,7 @@ class Oracle:
     """The restatement configured for (N, thermal, ndim)."""
 
     def __init__(self, N=3, thermal=False, ndim=2, dg_it=50, dg_tol=6e-6,
-                 stiff_it=200, stiff_tol=6e-6, bailout=0, fallback=0):
+                 stiff_it=200, stiff_tol=6e-6, bailout=0, fallback=0, flux=0):
         self.L = lib()
         self.cfg = Cfg()
         self.L.orc_default_cfg(ctypes.byref(self.cfg), N, int(thermal), ndim)
@@ -77,6 +78,7 @@ class Oracle:
         self.cfg.stiff_tol = stiff_tol
         self.cfg.bailout = bailout
         self.cfg.fallback = fallback
+        self.cfg.flux = flux
         self.N, self.thermal, self.ndim = N, int(thermal), ndim
         self.V = 17 if thermal else 14
 
@@ -143,6 +145,26 @@ class Oracle:
         r = np.zeros(self.V)
         self.L.orc_d_rus(_dp(r), _dp(qL), _dp(qR), d, ctypes.byref(MP),
                          ctypes.byref(self.cfg))
+        return r
+
+    def system_matrix(self, Q, d, MP):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        r = np.zeros((self.V, self.V))
+        self.L.orc_system_matrix(_dp(r), _dp(Q), d, ctypes.byref(MP), self.thermal)
+        return r
+
+    def D_ROE(self, qL, qR, d, MP):
+        return self._eig_flux(self.L.orc_d_roe, qL, qR, d, MP)
+
+    def D_OSH(self, qL, qR, d, MP):
+        return self._eig_flux(self.L.orc_d_osh, qL, qR, d, MP)
+
+    def _eig_flux(self, fn, qL, qR, d, MP):
+        qL = np.ascontiguousarray(qL, dtype=np.float64)
+        qR = np.ascontiguousarray(qR, dtype=np.float64)
+        r = np.zeros(self.V)
+        if fn(_dp(r), _dp(qL), _dp(qR), d, ctypes.byref(MP), ctypes.byref(self.cfg)) != 0:
+            raise RuntimeError("eigendecomposition did not converge")
         return r
 
     def Bint(self, qL, qR, d, MP):

@@ -418,6 +418,111 @@ def fv_parts():
     save("fv_parts.npz", **out)
 
 
+def _workload(tag, n):
+    """(u, MP, tf, dX, bcs, nX) of a small fixture workload."""
+    if tag == "dsl":
+        u, MP, tf, dX, bcs = problems.double_shear_2d(n, μ=0.1)
+        return u.ravel(), MP, 1e9, dX, bcs, [n, n]
+    if tag == "plastic":
+        u, MP, tf, dX, bcs = problems.plastic_pulse_2d(n)
+        return u.ravel(), MP, 1e9, dX, bcs, [n, n]
+    if tag == "grf_th":
+        u, MP, tf, dX, bcs = problems.grf_density_2d(n, seed=3, μ=1e-2, p_amp=0.05, thermal=True)
+        return u.ravel(), MP, 1e9, dX, bcs, [n, n]
+    u, MP, tf, dX, bcs = problems.sod_1d(n, thermal=True, μ=1e-2)
+    return u.ravel(), MP, tf, dX, bcs, [n]
+
+
+def eig_fluxes():
+    """system_matrix (equations.cpp:169-176), D_ROE and D_OSH (fluxes.cpp:22-64)
+    on neighbouring cells of four workloads after two reference steps (one
+    for the plastic solid: the reference aborts in its second step, where
+    Picard ends non-finite and its 2D Newton-Krylov call fails an Eigen size
+    assertion), in x and y, and on the same pairs with 2% noise; with the
+    no-FMA build's values."""
+    out = {}
+    rng = np.random.default_rng(61)
+    for tag, thermal, n in (("dsl", False, 12), ("plastic", False, 12), ("grf_th", True, 12),
+                            ("sod_th", True, 48)):
+        R = RefSolver(3, thermal)
+        R2 = RefSolver(3, thermal, nofma=True)
+        u, MP, tf, dX, bcs, nX = _workload(tag, n)
+        u2, _, _, _, _, st = R.run(u, nX, dX, 0.8, bcs, tf, 1 if tag == "plastic" else 2, MP)
+        assert st == 0
+        V = R.V
+        c = u2.reshape(*nX, V)
+        qL, qR, ds = [], [], []
+        for k in range(24):
+            d = k % len(nX)
+            if len(nX) == 1:
+                i = rng.integers(n - 1)
+                a, b = c[i], c[i + 1]
+            else:
+                i, j = rng.integers(n - 1, size=2)
+                a, b = (c[i, j], c[i + 1, j]) if d == 0 else (c[i, j], c[i, j + 1])
+            for noise in (0.0, 0.02):
+                qL.append(a * (1 + noise * rng.standard_normal(V)))
+                qR.append(b * (1 + noise * rng.standard_normal(V)))
+                ds.append(d)
+        qL, qR, ds = np.array(qL), np.array(qR), np.array(ds, np.int32)
+        out[f"{tag}_par"] = np.array(par_json(MP))
+        out[f"{tag}_qL"], out[f"{tag}_qR"], out[f"{tag}_d"] = qL, qR, ds
+        for name, fn in (("sm", lambda S, a, b, d: S.system_matrix(a, d, MP)),
+                         ("roe", lambda S, a, b, d: S.D_ROE(a, b, d, MP)),
+                         ("osh", lambda S, a, b, d: S.D_OSH(a, b, d, MP))):
+            out[f"{tag}_{name}"] = np.array([fn(R, a, b, int(d)) for a, b, d in zip(qL, qR, ds)])
+            out[f"{tag}_{name}_nofma"] = np.array([fn(R2, a, b, int(d))
+                                                   for a, b, d in zip(qL, qR, ds)])
+    save("eig_fluxes.npz", **out)
+
+
+def eig_steps():
+    """Full steps (iterator.cpp:110-165) and the interface half of the FV
+    update (interfs1/2, fv.cpp:111-279) with FLUX = 1 (Roe) and 2 (Osher):
+    three steps of three workloads from their initial states (one of the
+    plastic solid, where the reference aborts in the second), with and
+    without FMA; interfs on the FV-halves fixture inputs."""
+    out = {}
+    for tag, thermal, n in (("dsl", False, 12), ("plastic", False, 12), ("sod_th", True, 48)):
+        R = RefSolver(3, thermal)
+        R2 = RefSolver(3, thermal, nofma=True)
+        u, MP, tf, dX, bcs, nX = _workload(tag, n)
+        out[f"{tag}_par"] = np.array(par_json(MP))
+        out[f"{tag}_u"], out[f"{tag}_nX"], out[f"{tag}_dX"] = u, np.array(nX), np.array(dX)
+        out[f"{tag}_bcs"], out[f"{tag}_tf"] = np.array(bcs), np.array(tf)
+        for flux in (1, 2):
+            R.set_flux(flux)
+            R2.set_flux(flux)
+            steps, steps2, dts = [], [], []
+            ua, t, c = u.copy(), 0.0, 0
+            for _ in range(1 if tag == "plastic" else 3):
+                u2, t2, c2, d1, _, st = R.run(ua, nX, dX, 0.8, bcs, tf, 1, MP, t=t, count=c)
+                u3 = R2.run(ua, nX, dX, 0.8, bcs, tf, 1, MP, t=t, count=c)[0]
+                assert st == 0
+                steps.append(u2)
+                steps2.append(u3)
+                dts.append(d1[0])
+                ua, t, c = u2, t2, c2
+            out[f"{tag}_f{flux}_steps"] = np.array(steps)
+            out[f"{tag}_f{flux}_steps_nofma"] = np.array(steps2)
+            out[f"{tag}_f{flux}_dts"] = np.array(dts)
+        R.set_flux(0)
+        R2.set_flux(0)
+    g = np.load(os.path.join(HERE, "fv_parts.npz"))
+    from phd_amd.params import Par
+    for tag, thermal in (("2d_dsl12", False), ("1d_sod48_th", True)):
+        s = np.load(os.path.join(HERE, f"stages_{tag}.npz"))
+        MP = Par.from_dict(json.loads(str(s["par"])))
+        R = RefSolver(3, thermal)
+        R2 = RefSolver(3, thermal, nofma=True)
+        for flux in (1, 2):
+            for name, rec, time in (("interfs", g[f"{tag}_qh"], 1), ("interfs_t0", g[f"{tag}_wh"], 0)):
+                args = (s["u"], rec, s["nX"], float(s["dt"]), s["dX"], MP, s["mask"], 0, time, 2)
+                out[f"{tag}_f{flux}_{name}"] = R.fv_part(*args, flux=flux)
+                out[f"{tag}_f{flux}_{name}_nofma"] = R2.fv_part(*args, flux=flux)
+    save("eig_steps.npz", **out)
+
+
 if __name__ == "__main__":
     constants()
     pointwise()
@@ -433,3 +538,5 @@ if __name__ == "__main__":
     split_steps("2d_plastic16", False, 2, 16, None, 3)
     split_steps("1d_sod48_th", True, 1, 48, 1e-2, 4)
     split_steps("1d_sod48", False, 1, 48, 1e-2, 4)
+    eig_fluxes()
+    eig_steps()
