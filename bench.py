@@ -32,6 +32,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "ADER-WENO cell-updates/s, 2D 512² P2: plain FP64 vs batched/FP32/adjoint"
+# Picard exit of every leg: a cell that has not converged after 6 Picard
+# iterations leaves for the stiff fallback the first time its stopping
+# statistic grows (bail_from), and no other early exit (bailout = 0).  On
+# the double shear layer this moves no cell off the reference's own
+# Picard path (the faithful leg checks it on the timed window).
+PICARD_EXIT = {"bailout": 0, "bail_from": 6}
 
 # ---------------------------------------------------------------------------
 # Algorithmic FP64 operation counts, frozen from phd_amd/csrc/flopcount.cpp
@@ -102,12 +108,12 @@ def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0
     return {"sec": float(np.sum(r["secs"])), "steps": len(r["secs"]), "status": r["status"]}
 
 
-def cpu_port(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, bailout):
+def cpu_port(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, exit_opts):
     """The oracle (CPU restatement of the same algorithm) on the same steps."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from orcbind import Oracle
-    O = Oracle(3, False, 2, bailout=bailout)
+    O = Oracle(3, False, 2, **exit_opts)
     u = u_start.copy()
     t, c = t0, count0
     tic = time.perf_counter()
@@ -138,15 +144,15 @@ def cpu_baseline_leg(u_w, MP, nX, dX, bcs, st0, args, threads, gpu_value):
     cells = n * nX[1]
     w0, K = args.warmup, args.steps
     picks = sorted({w0, w0 + K // 2, w0 + K - 1})
-    run = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout, mode=args.mode,
-                    t0=st0["t"], count0=st0["count"])
+    run = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, mode=args.mode, t0=st0["t"], count0=st0["count"],
+                    **exit_of(args))
     states = []
     for step in picks:
         run.steps(step - run.status()["count"])
         s = run.status()
         states.append((run.host(), s["t"], s["count"]))
     del run
-    secs = [cpu_port(u, MP, nX, dX, bcs, t, c, 1, threads, args.bailout) for u, t, c in states]
+    secs = [cpu_port(u, MP, nX, dX, bcs, t, c, 1, threads, exit_of(args)) for u, t, c in states]
     pval = cells * len(secs) / sum(secs)
     u, t, c = states[len(states) // 2]
     ref = cpu_reference(u, MP, nX, dX, bcs, t, c, 1, threads)
@@ -155,7 +161,7 @@ def cpu_baseline_leg(u_w, MP, nX, dX, bcs, st0, args, threads, gpu_value):
         "cpu_model": cpu_model(),
         "sample": f"one step from each of steps {picks} of the GPU's timed window (steps {w0}-"
                   f"{w0 + K - 1}, device states), oracle/liboracle.so (the CPU restatement of the same "
-                  f"algorithm, Picard bailout {args.bailout}) with OpenMP over {threads} threads; "
+                  f"algorithm, the same Picard exit {exit_of(args)}) with OpenMP over {threads} threads; "
                   f"seconds per step {[round(x, 3) for x in secs]}",
         "gpu_over_cpu": gpu_value / pval,
     }
@@ -203,7 +209,7 @@ def fp32_bulk_leg(n, steps):
            "cells": n * n, "steps": steps}
     states = {}
     for name, prec in (("fp64", 0), ("fp32_bulk", 1)):
-        run = DeviceRun(u0.ravel(), MP, [n, n], dX, bcs, CFL=0.8, bailout=5, precision=prec)
+        run = DeviceRun(u0.ravel(), MP, [n, n], dX, bcs, CFL=0.8, precision=prec, **PICARD_EXIT)
         sec, tm = timed_run(run, steps)
         out[name] = {"value": n * n * steps / sec, "ms_per_step": 1e3 * sec / steps,
                      "phase_ms_per_step": {k: v / tm["steps"] for k, v in tm["ms"].items()},
@@ -229,7 +235,7 @@ def fp32_bulk_leg(n, steps):
     # run's last): FP32 rounding of the stiff solid's state sits near DG_TOL
     out["iterations_last_state"] = {
         name: iteration_histogram(states["fp64"].ravel(), MP, [n, n], dX, bcs, end["t"], end["count"],
-                                  5, 0, precision=prec)
+                                  PICARD_EXIT, 0, precision=prec)
         for name, prec in (("fp64", 0), ("fp32_bulk", 1))}
     return out
 
@@ -246,7 +252,7 @@ def batched_leg(count, seed, sample):
     from phd_amd.batch import Batch
     from phd_amd.runner import DeviceRun
     inst = problems.shock_tube_batch(count, seed=seed)
-    b = Batch(inst, bailout=5)
+    b = Batch(inst, **PICARD_EXIT)
     tune = b.autotune()
     b.reset()
     tic = time.perf_counter()
@@ -267,7 +273,7 @@ def batched_leg(count, seed, sample):
     pw, ps = 0, 0.0
     for u0, MP, tf, dX, bcs in inst[:sample]:
         r = DeviceRun(np.asarray(u0).ravel(), MP, [u0.shape[0]], dX, bcs, CFL=0.8, tf=tf,
-                      bailout=5)
+                      **PICARD_EXIT)
         r.sync()
         tic = time.perf_counter()
         while r.status()["t"] < tf:
@@ -302,7 +308,7 @@ def adjoint_case(n, steps, K, p_amp):
     target = tgt_run.host().reshape(-1, 14)[:, 0].copy()
     tgt_run.close()
     # the plain form (its default predictor tolerances) over the same steps
-    plain = DeviceRun(u0, MP, nX, dX, bcs, CFL=0.8, bailout=5)
+    plain = DeviceRun(u0, MP, nX, dX, bcs, CFL=0.8, **PICARD_EXIT)
     psec, _ = timed_run(plain, steps)
     plain.close()
     run = AdjointRun(u0, MP, nX, dX, bcs)
@@ -410,7 +416,12 @@ def split_leg(n, steps, warmup, threads, cpu):
     return out
 
 
-def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode, precision=0):
+def exit_of(args):
+    """The Picard exit options of the main leg."""
+    return {"bailout": args.bailout, "bail_from": args.bail_from}
+
+
+def iteration_histogram(u, MP, nX, dX, bcs, t, count, exit_opts, mode, precision=0):
     """Per-cell predictor iterations of one step from state u (the timed
     leg's starting state), through the host entry point gpr_predictor with
     the same solver options: a histogram of Picard iterations over the cells
@@ -418,8 +429,9 @@ def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode, precision=0
     from phd_amd.gprpy import load
     G = load(3, False)
     ctx = G.ctx(2)
-    saved = {"bailout": ctx.opts.bailout, "mode": ctx.opts.mode, "precision": ctx.opts.precision}
-    ctx.set_opts(bailout=bailout, mode=mode, precision=precision)
+    saved = {"bailout": ctx.opts.bailout, "bail_from": ctx.opts.bail_from, "mode": ctx.opts.mode,
+             "precision": ctx.opts.precision}
+    ctx.set_opts(mode=mode, precision=precision, **exit_opts)
     try:
         nXa = np.array(nX, np.int32)
         ne = int(np.prod(nXa + 2))
@@ -445,16 +457,16 @@ def iteration_histogram(u, MP, nX, dX, bcs, t, count, bailout, mode, precision=0
             "picard_histogram": {int(k): int(v) for k, v in zip(*np.unique(pic, return_counts=True))},
             "fallback_histogram": {int(k): int(v) for k, v in zip(*np.unique(fb, return_counts=True))},
             "note": "Picard iterations of the cells Picard solves; cells it leaves (after "
-                    "growing `bailout` times in a row) are counted by their fallback iterations"}
+                    "an early exit sends to the fallback) are counted by their fallback iterations"}
 
 
-def full_run_leg(u0, MP, nX, dX, bcs, total, seg, bailout, mode):
+def full_run_leg(u0, MP, nX, dX, bcs, total, seg, exit_opts, mode):
     """The whole job of the configured workload: `total` steps from t = 0 in
     segments of `seg` steps, each timed as the main line is (stream
     synchronisation around it, HIP-event phase timing inside), so that the
     segments show how the cost moves as the shear layers roll up."""
     from phd_amd.runner import DeviceRun
-    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=bailout, mode=mode)
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, mode=mode, **exit_opts)
     cells = int(np.prod(nX))
     segs, done, wall = [], 0, 0.0
     while done < total:
@@ -512,7 +524,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-faithful", action="store_true",
                     help="skip the second timed run with bailout = 0")
-    ap.add_argument("--bailout", type=int, default=5)
+    ap.add_argument("--bailout", type=int, default=PICARD_EXIT["bailout"],
+                    help="leave Picard after the stopping statistic grew this many times in a row")
+    ap.add_argument("--bail-from", type=int, default=PICARD_EXIT["bail_from"],
+                    help="after this many Picard iterations, leave on the first growth (0: off)")
     ap.add_argument("--mode", type=int, default=0,
                     help="predictor mode: 0 Picard then the fallback in the same launch; 2 the "
                          "fallback as a second launch over the cells Picard leaves (same values)")
@@ -545,7 +560,7 @@ def main():
     nX = [n, n]
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
-    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=args.bailout, mode=args.mode)
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, mode=args.mode, **exit_of(args))
     run.steps(args.warmup)
     st0 = run.status()
     if st0["pred_fail"] or st0["nonfinite"]:
@@ -553,6 +568,7 @@ def main():
     u_w = run.host()
 
     sec, tm = timed_run(run, args.steps)
+    u_end = run.host()
     progress(f"plain {n}x{n}: {args.steps} steps in {sec:.3f} s")
 
     cells = n * n
@@ -583,7 +599,7 @@ def main():
             "source": "phd_amd/csrc/flopcount.cpp (instrumented); the predictor's from the "
                       "iteration totals counted on the device in the timed launches"},
         "iterations_first_timed_step": iteration_histogram(u_w, MP, nX, dX, bcs, st0["t"],
-                                                           st0["count"], args.bailout, args.mode),
+                                                           st0["count"], exit_of(args), args.mode),
     }
 
     out = {
@@ -593,22 +609,25 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"2D GPR double shear layer {n}x{n} periodic, P2 (N=3), FP64, "
                                "mu=1e-3, cs=8, CFL 0.8, steps after warm-up", "cells": cells,
-                   "N": 3, "V": 14, "bailout": args.bailout, "mode": args.mode,
+                   "N": 3, "V": 14, "bailout": args.bailout, "bail_from": args.bail_from,
+                   "mode": args.mode,
                    "parallelism": "single GPU"},
         "roofline": roofline,
     }
     if args.full_steps > 0:
-        out["full_run"] = full_run_leg(u0, MP, nX, dX, bcs, args.full_steps, 500, args.bailout,
+        out["full_run"] = full_run_leg(u0, MP, nX, dX, bcs, args.full_steps, 500, exit_of(args),
                                        args.mode)
         progress(f"full run: {args.full_steps} steps in {out['full_run']['seconds']:.2f} s")
-    if args.bailout != 0 and not args.no_faithful:
+    if (args.bailout != 0 or args.bail_from != 0) and not args.no_faithful:
         # the same steps with Picard run to DG_IT = 50 for every cell, as the
-        # reference does, from the same starting state
-        run0 = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=0, mode=args.mode, t0=st0["t"],
-                         count0=st0["count"])
+        # reference does, from the same starting state; equal end states mean
+        # the early exit moved no cell off the reference's Picard path
+        run0 = DeviceRun(u_w, MP, nX, dX, bcs, CFL=0.8, bailout=0, bail_from=0, mode=args.mode,
+                         t0=st0["t"], count0=st0["count"])
         sec0, _ = timed_run(run0, args.steps)
         out["bailout0_value"] = cells * args.steps / sec0
         out["bailout0_ms_per_step"] = 1e3 * sec0 / args.steps
+        out["bailout0_same_state"] = bool(np.array_equal(run0.host(), u_end))
         progress("bailout 0 leg done")
         del run0
     if not args.no_fp32:

@@ -64,6 +64,9 @@ typedef struct gpr_solver_opts {
   int32_t split;      /* device loop: 0 the ADER step; 1 the split-WENO step
                          (steppers.cpp:50-66, iterator's SPLIT), FP64 only */
   int32_t half_step;  /* with split: the midstepper between WENO and FV (HALF_STEP) */
+  int32_t bail_from;  /* 0: off; j > 0: a cell that has run j Picard iterations
+                         without converging leaves Picard the first time its
+                         stopping statistic grows (in addition to bailout) */
 } gpr_solver_opts;
 
 const char *gpr_last_error(void);

@@ -46,6 +46,7 @@ void orc_default_cfg(orc_cfg *c, int N, int thermal, int ndim) {
   c->bailout = 0;
   c->fallback = 0;
   c->flux = 0;
+  c->bail_from = 0;
 }
 
 /* ------------------------------------------------------------------ constants
@@ -1943,6 +1944,8 @@ int orc_predictor(double *qh, const double *wh, double dt, const double *dX,
         grow = ratio > prev ? grow + 1 : 0;
         prev = ratio;
         if (c->bailout > 0 && grow >= c->bailout)
+          break;
+        if (c->bail_from > 0 && count + 1 > c->bail_from && grow > 0)
           break;
         continue;
       } else if (nan) {

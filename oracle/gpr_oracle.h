@@ -38,6 +38,8 @@ typedef struct orc_cfg {
                     1: Newton on the local source with lagged transport (an
                        independent solver of the same equations, for tests) */
   int flux;      /* Riemann solver of the FV interfaces: 0 Rusanov, 1 Roe, 2 Osher */
+  int bail_from; /* 0: off; j > 0: after j Picard iterations without convergence,
+                    leave Picard the first time its stopping statistic grows */
 } orc_cfg;
 
 /* per-call predictor statistics */

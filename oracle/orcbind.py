@@ -22,7 +22,7 @@ class Cfg(ctypes.Structure):
                 ("dg_it", ctypes.c_int), ("dg_tol", ctypes.c_double),
                 ("stiff_it", ctypes.c_int), ("stiff_tol", ctypes.c_double),
                 ("bailout", ctypes.c_int), ("fallback", ctypes.c_int),
-                ("flux", ctypes.c_int)]
+                ("flux", ctypes.c_int), ("bail_from", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -68,7 +68,7 @@ class Oracle:
     """The restatement configured for (N, thermal, ndim)."""
 
     def __init__(self, N=3, thermal=False, ndim=2, dg_it=50, dg_tol=6e-6,
-                 stiff_it=200, stiff_tol=6e-6, bailout=0, fallback=0, flux=0):
+                 stiff_it=200, stiff_tol=6e-6, bailout=0, fallback=0, flux=0, bail_from=0):
         self.L = lib()
         self.cfg = Cfg()
         self.L.orc_default_cfg(ctypes.byref(self.cfg), N, int(thermal), ndim)
@@ -79,6 +79,7 @@ class Oracle:
         self.cfg.bailout = bailout
         self.cfg.fallback = fallback
         self.cfg.flux = flux
+        self.cfg.bail_from = bail_from
         self.N, self.thermal, self.ndim = N, int(thermal), ndim
         self.V = 17 if thermal else 14
 

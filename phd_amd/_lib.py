@@ -49,7 +49,8 @@ class SolverOpts(ctypes.Structure):
                 ("stiff_it", ctypes.c_int32), ("stiff_tol", ctypes.c_double),
                 ("bailout", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("contorted_tol", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("split", ctypes.c_int32), ("half_step", ctypes.c_int32)]
+                ("split", ctypes.c_int32), ("half_step", ctypes.c_int32),
+                ("bail_from", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

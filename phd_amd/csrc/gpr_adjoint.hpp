@@ -47,7 +47,8 @@ struct AdjArgs {
   double *gmu;          // d loss / d (the material parameter), accumulated
   double *gkap;         // d loss / d kappa (heat conduction), accumulated
   int *fail;            // cells whose transposed Picard iteration did not converge
-  unsigned long long *itc;  // [0] transposed predictor iterations, [1] cells, [2] cells in the preconditioned form
+  unsigned long long *itc;  // ITC_SLOTS x ITC_STRIDE words: [0] transposed predictor iterations, [1] cells,
+                            // [2] cells in the preconditioned form
   double dt, dx, dy;
   double par_fac;       // d S_A / d parameter = par_fac S_A: -1/mu (Newtonian), -n/sigmaY (power-law solid)
   double kappa;         // heat conductivity: d S_J / d kappa = -S_J / kappa
@@ -1118,9 +1119,10 @@ __global__ __launch_bounds__(64) void ka_predictor(AdjArgs a, long ncell_ext) {
   }
   if (failed && node == 0) atomicAdd(a.fail, 1);
   if (active && node == 0) {
-    atomicAdd(&a.itc[0], (unsigned long long)itot);
-    atomicAdd(&a.itc[1], 1ull);
-    if (used_pre) atomicAdd(&a.itc[2], 1ull);
+    unsigned long long *slot = a.itc + (blockIdx.x % ITC_SLOTS) * ITC_STRIDE;
+    atomicAdd(&slot[0], (unsigned long long)itot);
+    atomicAdd(&slot[1], 1ull);
+    if (used_pre) atomicAdd(&slot[2], 1ull);
   }
   // lam_wh(s) = sum_t psi_t(0) W_s m(t, s); d rhs / d mu through S
   __syncthreads();

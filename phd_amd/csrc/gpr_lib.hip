@@ -615,6 +615,7 @@ PredArgs pred_args(gpr_ctx *c, const Mat &m, double dt, const double *dX) {
   a.stiff_it = c->opts.stiff_it;
   a.stiff_tol = c->opts.stiff_tol;
   a.bailout = c->opts.bailout;
+  a.bail_from = c->opts.bail_from;
   a.mode = c->opts.mode;
   a.m = m;
   a.fail = c->d_fail;
@@ -685,11 +686,12 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
   c->opts.contorted_tol = 1.0;
   c->opts.split = 0;
   c->opts.half_step = 0;
+  c->opts.bail_from = 0;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_st, sizeof(StepState));
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_fail, sizeof(int) * 4);
-  if (e == hipSuccess) e = hipMalloc((void **)&c->d_itc, sizeof(unsigned long long) * 4);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_itc, sizeof(unsigned long long) * ITC_SLOTS * ITC_STRIDE);
   if (e == hipSuccess) {
     // basis tables of the constant segment on this device (same values for
     // every context, so concurrent writers agree)
@@ -735,7 +737,7 @@ int gpr_ctx_set_opts(gpr_ctx *c, const gpr_solver_opts *o) {
   if (!c || !o) return fail(GPR_E_ARG, "null argument");
   if (o->dg_it < 1 || o->stiff_it < 1 || !(o->dg_tol > 0) || !(o->stiff_tol > 0) || o->bailout < 0 ||
       o->mode < 0 || o->mode > 2 || o->precision < 0 || o->precision > 1 || o->split < 0 || o->split > 1 ||
-      o->half_step < 0 || o->half_step > 1)
+      o->half_step < 0 || o->half_step > 1 || o->bail_from < 0)
     return fail(GPR_E_ARG, "invalid solver options");
   c->opts = *o;
   return GPR_OK;
@@ -1450,7 +1452,7 @@ extern "C" {
 int gpr_dev_set_timing(gpr_ctx *c, int enable) {
   if (!c) return fail(GPR_E_ARG, "null context");
   if (enable) {
-    HIP_TRY(hipMemsetAsync(c->d_itc, 0, sizeof(unsigned long long) * 4, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_itc, 0, sizeof(unsigned long long) * ITC_SLOTS * ITC_STRIDE, c->stream));
     c->ev_used = 0;
     c->timed_steps = 0;
   }
@@ -1469,7 +1471,12 @@ int gpr_dev_timing(gpr_ctx *c, double *ms, int64_t *steps, int64_t *iters, int64
       tot[p] += f;
     }
   unsigned long long h[4] = {0, 0, 0, 0};
-  HIP_TRY(hipMemcpy(h, c->d_itc, sizeof h, hipMemcpyDeviceToHost));
+  {
+    std::vector<unsigned long long> slots((size_t)ITC_SLOTS * ITC_STRIDE);
+    HIP_TRY(hipMemcpy(slots.data(), c->d_itc, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int s = 0; s < ITC_SLOTS; s++)
+      for (int k = 0; k < 4; k++) h[k] += slots[(size_t)s * ITC_STRIDE + k];
+  }
   if (ms)
     for (int p = 0; p < 5; p++) ms[p] = tot[p];
   if (steps) *steps = c->timed_steps;
@@ -2102,7 +2109,8 @@ int adj_step_back(gpr_ctx *c, AdjHost *A, const double *u_n, double dt, double *
   a.start_pre = A->start_pre;
   a.it_max = 400;
   a.tol = A->tol;
-  if (!(a.itc = (unsigned long long *)abuf(A, 12, 3 * sizeof(unsigned long long), rc))) return rc;
+  if (!(a.itc = (unsigned long long *)abuf(A, 12, ITC_SLOTS * ITC_STRIDE * sizeof(unsigned long long), rc)))
+    return rc;
   a.bc = c->run_bc;
   a.m = c->run_mat;
   HIP_TRY(hipMemsetAsync(a.lam_wh, 0, sizeof(double) * ne * N * N * V, c->stream));
@@ -2218,9 +2226,10 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   HIP_TRY(hipMemcpyAsync(lam, lam_T, sz, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(d_gmu, 0, sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(c->d_fail, 0, sizeof(int), c->stream));
-  unsigned long long *d_itc = (unsigned long long *)abuf(A, 12, 3 * sizeof(unsigned long long), rc);
+  unsigned long long *d_itc =
+      (unsigned long long *)abuf(A, 12, ITC_SLOTS * ITC_STRIDE * sizeof(unsigned long long), rc);
   if (!d_itc) return rc;
-  HIP_TRY(hipMemsetAsync(d_itc, 0, 3 * sizeof(unsigned long long), c->stream));
+  HIP_TRY(hipMemsetAsync(d_itc, 0, ITC_SLOTS * ITC_STRIDE * sizeof(unsigned long long), c->stream));
   const int K = A->K;
   const int64_t nchk = (A->nsteps + K - 1) / K;
   // Keep each recomputed step's predictor result for its transposed step
@@ -2273,11 +2282,16 @@ int gpr_adj_backward(gpr_ctx *c, const double *lam_T, double *lam_0, double *gra
   HIP_TRY(hipMemcpyAsync(lam_0, lam, sz, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(&nf, c->d_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  unsigned long long itc[3];
-  HIP_TRY(hipMemcpyAsync(itc, d_itc, sizeof itc, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> itc_slots((size_t)ITC_SLOTS * ITC_STRIDE);
+  HIP_TRY(hipMemcpyAsync(itc_slots.data(), d_itc, itc_slots.size() * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(&A->grad_kappa, d_gkap, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 3; k++) A->stats[k] = (int64_t)itc[k];
+  for (int k = 0; k < 3; k++) {
+    unsigned long long tot = 0;
+    for (int s = 0; s < ITC_SLOTS; s++) tot += itc_slots[(size_t)s * ITC_STRIDE + k];
+    A->stats[k] = (int64_t)tot;
+  }
   if (unconverged) *unconverged = nf;
   return GPR_OK;
 }

@@ -29,6 +29,12 @@ namespace gpr {
 
 constexpr int WAVE = 64;
 
+// Iteration counters (PredArgs::itc and the adjoint's) are summed into
+// ITC_SLOTS slots of ITC_STRIDE words (one 128-byte line each), slot =
+// block % ITC_SLOTS, so that atomics from different workgroups rarely meet
+// on one address; the host adds the slots.
+constexpr int ITC_SLOTS = 256, ITC_STRIDE = 16;
+
 struct StepState;
 
 // A batch of independent 1D problems in one set of launches.  Instance k's
@@ -61,11 +67,13 @@ struct PredArgs {
   const uint8_t *mask;     // [ncell]
   int32_t *iters;          // optional per-cell iteration count (fallback: 1000 + its)
   int *fail;               // device counter of cells that converged in neither phase
-  unsigned long long *itc; // optional: [0] Picard, [1] fallback iterations, [2] cells, [3] fallback cells
+  unsigned long long *itc; // optional, ITC_SLOTS x ITC_STRIDE words: [0] Picard, [1] fallback iterations,
+                           // [2] cells, [3] fallback cells
   const double *dt_dev;    // if non-null, dt is read from here
   double dt, dx, dy;
   long ncell;
   int dg_it, stiff_it, bailout, mode;  // mode 1: fallback only; 0 or 2: Picard then fallback
+  int bail_from;           // > 0: after this many Picard iterations, one growth leaves Picard
   double dg_tol, stiff_tol;
   Mat m;
   // batched launches only
@@ -265,6 +273,8 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   int phase = active ? ((KIND == KIND_FALLBACK || (FB && a.mode == 1)) ? 1 : 0) : 3;
   int it = 0, pit = 0, grow = 0;
   R prev = R(1e30);
+  // this lane's contributions to a.itc (node 0 of a cell only), added once per wavefront
+  unsigned long long n_pic = 0, n_fb = 0, n_cells = 0, n_fbcells = 0;
   const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * LPC);
   bool need_pre = phase == 1;
   wave_sync();  // sB, sW
@@ -456,16 +466,20 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     const R tol = R(phase == 1 ? a.stiff_tol : a.dg_tol);
     bool more = false, nan = false;
     R ratio = 0;
+    // the stopping statistic feeds the growth tests only: every iteration for
+    // bailout, and from iteration bail_from on (its value there is the first
+    // one compared) for bail_from
+    const bool want_ratio = __any(a.bailout > 0 || (a.bail_from > 0 && phase == 0 && it + 1 >= a.bail_from));
 #pragma unroll
     for (int v = 0; v < V; v++) {
       R dd = fabs(rhs[v] - q0[v]), lim = tol * (R(1) + fabs(q0[v]));
       more |= dd > lim;
       nan |= rhs[v] != rhs[v];
-      if (a.bailout > 0) ratio = fmax(ratio, fast_div(dd, lim));
+      if (want_ratio) ratio = fmax(ratio, fast_div(dd, lim));
     }
     const unsigned long long bm = __ballot(more), bn = __ballot(nan);
     const bool cmore = (bm & cellbits) != 0, cnan = (bn & cellbits) != 0;
-    if (a.bailout > 0) {  // the cell's largest ratio, in each of its lanes
+    if (want_ratio) {  // the cell's largest ratio, in each of its lanes (wave-uniform branch)
       if constexpr (LPC == 32) {
         ratio = R(half_wave_max(lane_ok ? double(ratio) : 0.0));
       } else {
@@ -478,13 +492,15 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     }
 #pragma unroll
     for (int v = 0; v < V; v++) q0[v] = rhs[v];  // the next iterate (reset below on a restart)
+    [[maybe_unused]] bool push = false;  // KIND_PICARD: append this cell to a.list
     if (phase == 0) {
       it++;
       bool to_fallback = false;
       if (cmore) {
         grow = ratio > prev ? grow + 1 : 0;
         prev = ratio;
-        if (it >= a.dg_it || (a.bailout > 0 && grow >= a.bailout)) to_fallback = true;
+        if (it >= a.dg_it || (a.bailout > 0 && grow >= a.bailout) || (a.bail_from > 0 && it > a.bail_from && grow > 0))
+          to_fallback = true;
       } else if (cnan) {
         to_fallback = true;
       } else {
@@ -492,10 +508,8 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = rhs[v];
         if (node == 0) {
           if (a.iters) a.iters[cell] = it;
-          if (a.itc) {
-            atomicAdd(&a.itc[0], (unsigned long long)it);
-            atomicAdd(&a.itc[2], 1ull);
-          }
+          n_pic += it;
+          n_cells++;
         }
         phase = 2;
       }
@@ -509,8 +523,8 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         } else {
           // solved again by the KIND_FALLBACK launch over a.list
           if (node == 0) {
-            a.list[atomicAdd(a.list_count, 1)] = (int)cell;
-            if (a.itc) atomicAdd(&a.itc[0], (unsigned long long)it);
+            push = true;
+            n_pic += it;
           }
           phase = 3;
         }
@@ -524,17 +538,41 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         if (node == 0) {
           if (a.iters) a.iters[cell] = ok ? 1000 + pit : -1;
           if (!ok) atomicAdd(a.fail, 1);
-          if (a.itc) {
-            atomicAdd(&a.itc[0], (unsigned long long)it);
-            atomicAdd(&a.itc[1], (unsigned long long)pit);
-            atomicAdd(&a.itc[2], 1ull);
-            atomicAdd(&a.itc[3], 1ull);
-          }
+          n_pic += it;
+          n_fb += pit;
+          n_cells++;
+          n_fbcells++;
         }
         phase = ok ? 2 : 3;
       }
     }
+    if constexpr (!FB) {
+      // one atomic per wavefront for the cells it appends to a.list
+      const unsigned long long pm = __ballot(push);
+      if (pm) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(a.list_count, (int)__popcll(pm));
+        base = __shfl(base, 0);
+        if (push) a.list[base + __popcll(pm & ((1ull << lane) - 1))] = (int)cell;
+      }
+    }
     wave_sync();  // sQ, sF and sRat are rewritten next iteration
+  }
+  if (a.itc) {  // every lane is active here
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      n_pic += __shfl_xor(n_pic, o);
+      n_fb += __shfl_xor(n_fb, o);
+      n_cells += __shfl_xor(n_cells, o);
+      n_fbcells += __shfl_xor(n_fbcells, o);
+    }
+    if (lane == 0) {
+      unsigned long long *slot = a.itc + (blockIdx.x % ITC_SLOTS) * ITC_STRIDE;
+      if (n_pic) atomicAdd(&slot[0], n_pic);
+      if (n_fb) atomicAdd(&slot[1], n_fb);
+      if (n_cells) atomicAdd(&slot[2], n_cells);
+      if (n_fbcells) atomicAdd(&slot[3], n_fbcells);
+    }
   }
 }
 

@@ -26,7 +26,7 @@ def test_batch_matches_each_instance_alone():
     from phd_amd.runner import DeviceRun
     print("batch vs single runs", flush=True)
     inst = small_batch()
-    b = Batch(inst, bailout=5)
+    b = Batch(inst, bailout=0, bail_from=6)
     b.run(16)
     st = b.status()
     assert st["done"].all() and not st["nonfinite"].any() and st["pred_fail"] == 0
@@ -35,7 +35,7 @@ def test_batch_matches_each_instance_alone():
     worst = 0.0
     for k, (u0, MP, tf, dX, bcs) in enumerate(inst):
         r = DeviceRun(np.asarray(u0).ravel(), MP, [u0.shape[0]], dX, bcs, CFL=0.8, tf=tf,
-                      bailout=5)
+                      bailout=0, bail_from=6)
         while r.status()["t"] < tf:
             r.steps(1)
         s = r.status()
@@ -57,8 +57,8 @@ def test_batch_against_oracle_steps():
     from orcbind import Oracle
     from phd_amd.batch import Batch
     inst = small_batch(seed=5, count=3)
-    b = Batch(inst, bailout=5)
-    O = Oracle(3, False, 1, bailout=5)
+    b = Batch(inst, bailout=0, bail_from=6)
+    O = Oracle(3, False, 1, bailout=0, bail_from=6)
     us = [np.asarray(u0).ravel() for u0, *_ in inst]
     for step in range(6):
         b.reset(np.concatenate(us))
@@ -78,8 +78,8 @@ def test_graph_replays_equal_stream_steps():
     gives the stream loop's states."""
     from phd_amd.batch import Batch
     inst = small_batch(seed=7, count=5)
-    a = Batch(inst, bailout=5)
-    b = Batch(inst, bailout=5)
+    a = Batch(inst, bailout=0, bail_from=6)
+    b = Batch(inst, bailout=0, bail_from=6)
     for rep in range(4):
         a.run(8, max_graphs=1)
         b.steps(8)

@@ -55,8 +55,8 @@ def test_fp32_bulk_against_fp64(case):
         n, steps = 256, 10
         u0, MP, _, dX, bcs = problems.plastic_pulse_2d(n)
     nX = [n, n]
-    a, sa = run(u0, MP, nX, dX, bcs, steps, 1, bailout=5)
-    b, sb = run(u0, MP, nX, dX, bcs, steps, 0, bailout=5)
+    a, sa = run(u0, MP, nX, dX, bcs, steps, 1, bailout=0, bail_from=6)
+    b, sb = run(u0, MP, nX, dX, bcs, steps, 0, bailout=0, bail_from=6)
     assert sa["count"] == sb["count"] == steps
     # dt comes from the FP64 CFL reduction of slightly different states
     assert abs(sa["t"] - sb["t"]) / sb["t"] < 1e-6
@@ -82,7 +82,7 @@ def test_fp32_predictor_stage():
     print("fp32 predictor stage, plastic 256^2 after 10 steps", flush=True)
     n = 256
     u0, MP, _, dX, bcs = problems.plastic_pulse_2d(n)
-    u, st = run(u0, MP, [n, n], dX, bcs, 10, 0, bailout=5)
+    u, st = run(u0, MP, [n, n], dX, bcs, 10, 0, bailout=0, bail_from=6)
     G = load(3, False)
     ctx = G.ctx(2)
     nXa = np.array([n, n], np.int32)
@@ -95,10 +95,10 @@ def test_fp32_predictor_stage():
     G.solvers.weno.weno_launcher(wh, ub, nXa, MP)
     dt = G.solvers.timestep([u.ravel()], [None], dX, 0.8, st["t"], 1e300, st["count"], [MP], 1)
     out = {}
-    saved = {k: getattr(ctx.opts, k) for k in ("precision", "bailout", "mode")}
+    saved = {k: getattr(ctx.opts, k) for k in ("precision", "bailout", "bail_from", "mode")}
     try:
         for prec in (0, 1):
-            ctx.set_opts(precision=prec, bailout=5, mode=0)
+            ctx.set_opts(precision=prec, bailout=0, bail_from=6, mode=0)
             qh = np.zeros(ne * 27 * 14)
             iters = np.full(ne, -7, np.int32)
             G.solvers.dg.predictor(qh, wh, dt, dX, False, False, MP, mask, iters=iters)

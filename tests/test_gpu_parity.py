@@ -373,24 +373,29 @@ def test_iterator_global_reset(gp):
     assert relerr(ug, uo) < 1e-11
 
 
-@pytest.mark.parametrize("bailout", [5, 0])
-def test_full_size_512(gp, bailout):
+EXITS = {"bail_from6": {"bailout": 0, "bail_from": 6}, "bailout5": {"bailout": 5},
+         "reference": {"bailout": 0}}
+
+
+@pytest.mark.parametrize("exit_name", list(EXITS))
+def test_full_size_512(gp, exit_name):
     """The benchmark workload at its full size (2D double shear layer 512x512,
     periodic, mu = 1e-3, c_s = 8) through the device-resident loop, with the
-    benchmark's Picard bailout (5) and with the reference's rule (0: Picard
-    runs to DG_IT = 50): the first step against the oracle; steps 7 and 8
-    (the first full-dt steps, where Picard diverges in 10-15% of cells and
-    the stiff fallback runs) against the oracle from the device's own state;
-    with bailout 5 also step 21, inside the benchmark's timed window; a
-    repeated run is bit-identical."""
+    benchmark's Picard exit (bail_from 6), the grow-5 exit, and the
+    reference's rule (Picard runs to DG_IT = 50): the first step against the
+    oracle; steps 7 and 8 (the first full-dt steps, where Picard diverges in
+    10-15% of cells and the stiff fallback runs) against the oracle from the
+    device's own state; with an early exit also step 21, inside the
+    benchmark's timed window; a repeated run is bit-identical."""
     from phd_amd import problems
     from phd_amd.runner import DeviceRun
-    print(f"full-size 512x512 case, bailout {bailout}", flush=True)
+    ex = EXITS[exit_name]
+    print(f"full-size 512x512 case, Picard exit {ex}", flush=True)
     n = 512
     u0, MP, _, dX, bcs = problems.double_shear_2d(n)
     nX = [n, n]
-    O = oracle(3, False, 2, bailout=bailout)
-    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=bailout)
+    O = oracle(3, False, 2, **ex)
+    run = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, **ex)
     run.steps(1)
     u1 = run.host()
     uo, t, c, dt, rc, st = O.step(u0.ravel(), nX, dX, 0.8, bcs, 0.0, 0, 1e300, MP)
@@ -404,7 +409,7 @@ def test_full_size_512(gp, bailout):
     uo, t, c, dt, rc, st = O.step(u7, nX, dX, 0.8, bcs, s["t"], s["count"], 1e300, MP)
     assert rc == 0 and st["fallback_cells"] > 0.05 * n * n
     assert relerr(u8, uo) < TOL
-    if bailout == 5:
+    if exit_name != "reference":
         run.steps(12)
         s = run.status()
         u20 = run.host()
@@ -413,10 +418,39 @@ def test_full_size_512(gp, bailout):
         assert rc == 0 and st["fallback_cells"] > 0.05 * n * n
         assert relerr(run.host(), uo) < TOL
     run.close()
-    run2 = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, bailout=bailout)
+    run2 = DeviceRun(u0.ravel(), MP, nX, dX, bcs, CFL=0.8, **ex)
     run2.steps(8)
     assert np.array_equal(run2.host(), u8)
     run2.close()
+
+
+def test_bail_from_keeps_reference_picard_path_512():
+    """The benchmark's Picard exit (leave after 6 iterations on the first
+    growth of the stopping statistic) against the reference's rule (Picard
+    to DG_IT = 50) over the first 600 steps of the 512x512 double shear
+    layer, compared every 100 steps.  The fallback restarts from the
+    standard initial guess, so the states agree bit for bit exactly when no
+    cell that the reference's Picard would converge left Picard early."""
+    from phd_amd import problems
+    from phd_amd.runner import DeviceRun
+    print("512x512 double shear layer, bail_from 6 against Picard to DG_IT, 600 steps", flush=True)
+    n = 512
+    u0, MP, _, dX, bcs = problems.double_shear_2d(n)
+    a = DeviceRun(u0.ravel(), MP, [n, n], dX, bcs, CFL=0.8, bailout=0, bail_from=6)
+    b = DeviceRun(u0.ravel(), MP, [n, n], dX, bcs, CFL=0.8, bailout=0, bail_from=0)
+    for k in range(6):
+        a.set_timing(True)
+        b.set_timing(True)
+        a.steps(100)
+        b.steps(100)
+        ta, tb = a.timing(), b.timing()
+        assert np.array_equal(a.host(), b.host()), (k + 1) * 100
+        assert ta["fallback_cells"] == tb["fallback_cells"] > 0
+        assert ta["picard_iters"] < tb["picard_iters"]
+        print(f"  steps {(k + 1) * 100}: identical; Picard iterations {ta['picard_iters']} vs "
+              f"{tb['picard_iters']}", flush=True)
+    a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("V", [14, 17])

@@ -5,7 +5,7 @@ amplitude 0.5, transmissive boundaries (phd_amd.problems.plastic_pulse_2d).
 
 FP64: the device loop's steps against the oracle's step from the device's
 own state, at 1e-12 (max |a-b| / (1 + |b|)), with the benchmark's solver
-options (Picard bailout 5), at 256^2 (every step of the ramp and beyond,
+options (Picard exit bail_from 6), at 256^2 (every step of the ramp and beyond,
 and a step after the pulse has driven cells past yield) and at the
 benchmark size 1024^2 (the first step, the first full-dt step, and step 101).
 
@@ -33,7 +33,7 @@ def setup(n, **opts):
     from phd_amd.runner import DeviceRun
     u0, MP, _, dX, bcs = problems.plastic_pulse_2d(n)
     u0 = u0.ravel()
-    run = DeviceRun(u0, MP, [n, n], dX, bcs, CFL=0.8, bailout=5, **opts)
+    run = DeviceRun(u0, MP, [n, n], dX, bcs, CFL=0.8, bailout=0, bail_from=6, **opts)
     return u0, MP, dX, bcs, run
 
 
@@ -47,7 +47,7 @@ def test_plastic_256_steps_vs_oracle():
     n = 256
     print("plastic pulse 256x256 FP64, steps 1-8 and 60", flush=True)
     u0, MP, dX, bcs, run = setup(n)
-    O = oracle(bailout=5)
+    O = oracle(bailout=0, bail_from=6)
     u = u0
     for k in range(8):
         s = run.status()
@@ -75,7 +75,7 @@ def test_plastic_1024_steps_vs_oracle():
     n = 1024
     print("plastic pulse 1024x1024 FP64, steps 1, 7 and 101", flush=True)
     u0, MP, dX, bcs, run = setup(n)
-    O = oracle(bailout=5)
+    O = oracle(bailout=0, bail_from=6)
     s = run.status()
     run.steps(1)
     uo, _ = oracle_step(O, u0, n, dX, bcs, s, MP)
@@ -110,14 +110,14 @@ def test_plastic_fp32_bulk_vs_oracle_fp64():
     n = 256
     print("plastic pulse 256x256 FP32-bulk against the oracle's FP64", flush=True)
     u0, MP, dX, bcs, r64 = setup(n)
-    O = oracle(bailout=5)
+    O = oracle(bailout=0, bail_from=6)
     r64.steps(40)
     s = r64.status()
     u40 = r64.host()
     r64.close()
     from phd_amd.runner import DeviceRun
-    r32 = DeviceRun(u40, MP, [n, n], dX, bcs, CFL=0.8, bailout=5, precision=1, t0=s["t"],
-                    count0=s["count"])
+    r32 = DeviceRun(u40, MP, [n, n], dX, bcs, CFL=0.8, bailout=0, bail_from=6, precision=1,
+                    t0=s["t"], count0=s["count"])
     r32.steps(1)
     uo, _ = oracle_step(O, u40, n, dX, bcs, s, MP)
     l1, li = errors(r32.host(), uo)
