@@ -62,6 +62,14 @@ FLOPS = {
 }
 
 
+# The one-launch predictor of the 2D P2 ideal-gas Newtonian kind also runs
+# the FV corrector's cell integrand at each of a cell's 27 nodes and
+# extrapolates its 4 x 9 face states (V x N multiply-adds each): the fused
+# epilogue, once per real cell.
+def fused_epilogue_flops(cells, nodes=27, face_nodes=36, V=14, N=3, material="gas_newtonian_2d_p2"):
+    return cells * (nodes * FLOPS[material]["fv_cell_node"] + face_nodes * V * N * 2)
+
+
 def predictor_flops(picard_iters, fallback_iters, fallback_cells, nodes=27, spatial=9,
                     material="gas_newtonian_2d_p2"):
     """FP64 operations of one predictor launch from its iteration totals:
@@ -576,14 +584,18 @@ def main():
     ms_step = 1e3 * sec / args.steps
     K = tm["steps"]
     pred_ms = tm["ms"]["predictor"] / K
+    fused = args.mode == 0
     flops_launch = predictor_flops(tm["picard_iters"] / K, tm["fallback_iters"] / K,
                                    tm["fallback_cells"] / K)
+    if fused:
+        flops_launch += fused_epilogue_flops(cells)
     achieved = flops_launch / (pred_ms * 1e-3) / 1e12
     f = FLOPS["gas_newtonian_2d_p2"]
     pred_cell = flops_launch / cells
     roofline = {
         "bound": "fp64-valu", "pipe": "FP64 vector ALU (peak 78.6 TFLOP/s, the same as dense FP64 MFMA)",
         "kernel": "k_predictor<3,false,2>", "achieved": round(achieved, 3),
+        "fused_fv_epilogue": fused,
         "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / MI355X_FP64_TFLOPS, 4),
         "traffic": PREDICTOR_TRAFFIC_512 if n == 512 else None,
@@ -593,9 +605,11 @@ def main():
         "fallback_cell_fraction": tm["fallback_cells"] / max(tm["cells"], 1),
         "phase_ms_per_step": {k: v / K for k, v in tm["ms"].items()},
         "flops_per_cell_update": {
-            "weno": f["weno_cell"], "predictor": pred_cell, "fv": f["fv_cell"],
+            "weno": f["weno_cell"], "predictor": pred_cell,
+            "fv": f["fv_cell"] - (fused_epilogue_flops(1) if fused else 0),
             "post_and_cfl": f["post_cell"],
-            "total": f["weno_cell"] + pred_cell + f["fv_cell"] + f["post_cell"],
+            "total": f["weno_cell"] + pred_cell + f["fv_cell"] + f["post_cell"]
+                     - (fused_epilogue_flops(1) if fused else 0),
             "source": "phd_amd/csrc/flopcount.cpp (instrumented); the predictor's from the "
                       "iteration totals counted on the device in the timed launches"},
         "iterations_first_timed_step": iteration_histogram(u_w, MP, nX, dX, bcs, st0["t"],

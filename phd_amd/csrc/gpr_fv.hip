@@ -28,8 +28,20 @@ template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_pair(hipStream_t 
   hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, s, a);
 }
 
+// the fused form: faces from the predictor's face states, then the cell update
+template <int N, bool TH, int ND> void fv_fused(hipStream_t s, const FvArgs &a) {
+  const long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
+  constexpr int FPW = FaceShape<N, TH, ND, true>::FPW;
+  const long items = (nf + FPW - 1) / FPW;
+  hipLaunchKernelGGL((k_fv_faces<N, TH, ND, true, true>), dim3(8 * ((items + 7) / 8)), dim3(WAVE), 0, s, a);
+  const long tot = (long)a.nx * (ND == 2 ? a.ny : 1) * Dims<TH>::V;
+  hipLaunchKernelGGL((k_fv_update<TH, ND>), dim3((tot + 255) / 256), dim3(256), 0, s, a);
+}
+
 template <int N, bool TH, int ND> hipError_t fv_launch(hipStream_t s, const FvArgs &a) {
-  if (a.time && a.sources)
+  if (a.fs)
+    fv_fused<N, TH, ND>(s, a);
+  else if (a.time && a.sources)
     fv_pair<N, TH, ND, true, true>(s, a);
   else if (!a.time && !a.sources)
     fv_pair<N, TH, ND, false, false>(s, a);

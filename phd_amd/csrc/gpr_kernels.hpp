@@ -193,6 +193,11 @@ struct FvArgs {
   int sources;           // 1: the cell integral includes S(q)
   int ode;               // with post: the split step's closing half-step ODE first
   int parts;             // 1: cell integrals (centers1/2), 2: face terms (interfs1/2), 3: both
+  // fused form (ADER step, TIME and SOURCES): the predictor wrote each active
+  // extended cell's face states (FaceStates layout) and each real cell's
+  // cell integral; qh is then unused
+  const double *fs;
+  const double *cint;    // [nx][ny][V]
   Mat m;
 };
 
@@ -212,7 +217,9 @@ template <int N, bool TH, int ND, bool TIME = true> struct FaceShape {
   static constexpr int FPW = WAVE / NT;        // faces per wavefront
 };
 
-template <int N, bool TH, int ND, bool TIME = true>
+// FS: the face states come from a.fs (the predictor's fused epilogue)
+// instead of being extrapolated from qh here; the values are the same.
+template <int N, bool TH, int ND, bool TIME = true, bool FS = false>
 __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
@@ -279,19 +286,31 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
     const double kt = (TIME ? dt * K.w[t] : dt) / (2.0 * h);
     const double kk = ND == 2 ? kt * K.w[sf] : kt;
     double qL[V], qR[V];
-    const double *pl = a.qh + (e0 * NN + t * NS) * V;
-    const double *pr = a.qh + (e1 * NN + t * NS) * V;
+    if constexpr (FS) {
+      using F = FaceStates<N, ND, TIME>;
+      const int sl = 2 * d;  // e0's high side is sl + 1, e1's low side sl
+      const double *pl = a.fs + ((e0 * F::SIDES + sl + 1) * F::NPS + t * F::NSF + sf) * V;
+      const double *pr = a.fs + ((e1 * F::SIDES + sl) * F::NPS + t * F::NSF + sf) * V;
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      double l = 0, rr = 0;
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-        int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
-        l += K.endv[1][k] * pl[nl * V + v];
-        rr += K.endv[0][k] * pr[nl * V + v];
+      for (int v = 0; v < V; v++) {
+        qL[v] = pl[v];
+        qR[v] = pr[v];
       }
-      qL[v] = l;
-      qR[v] = rr;
+    } else {
+      const double *pl = a.qh + (e0 * NN + t * NS) * V;
+      const double *pr = a.qh + (e1 * NN + t * NS) * V;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double l = 0, rr = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
+          l += K.endv[1][k] * pl[nl * V + v];
+          rr += K.endv[0][k] * pr[nl * V + v];
+        }
+        qL[v] = l;
+        qR[v] = rr;
+      }
     }
     // the direction as a compile-time constant in each branch, so that the
     // physics indexes its small arrays statically (no stack frame)
@@ -371,38 +390,10 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   __syncthreads();
   double S[V];
   {
-    NodeState<double, TH> st;
-    node_state<double, TH>(q, a.m, st);
-    if (SRC) {
-      source<double, TH>(q, st, a.m, S);
-    } else {
-#pragma unroll
-      for (int v = 0; v < V; v++) S[v] = 0.0;
-    }
+    cell_integrand<N, TH, ND, TIME, SRC>(q, myQ, t, ia, ib, a.m, K, dt, a.dx, a.dy, S);
     const bool centers = a.parts & 1;
-    double dq[V];
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      double acc = 0;
-#pragma unroll
-      for (int k = 0; k < N; k++) acc += K.der[ia][k] * myQ[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
-      dq[v] = acc / a.dx;
-    }
-    bdot_sub(st.v, dq, 0, 1.0, S);
-    if (ND == 2) {
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        double acc = 0;
-#pragma unroll
-        for (int k = 0; k < N; k++) acc += K.der[ib][k] * myQ[(t * NS + ia * N + k) * V + v];
-        dq[v] = acc / a.dy;
-      }
-      bdot_sub(st.v, dq, 1, 1.0, S);
-    }
-    const double wt = TIME ? dt * K.w[t] : dt;
-    const double wgt = ND == 2 ? wt * K.w[ia] * K.w[ib] : wt * K.w[ia];
-#pragma unroll
-    for (int v = 0; v < V; v++) S[v] = active && centers ? wgt * S[v] : 0.0;
+    for (int v = 0; v < V; v++) S[v] = active && centers ? S[v] : 0.0;
   }
   __syncthreads();
   if (lane_ok)
@@ -426,6 +417,30 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
       du -= fb[V + v] + ft[v];
     }
     a.u[c * V + v] += du;
+  }
+}
+
+// Fused form's cell update: u += cint - (face terms), one thread per (cell,
+// variable), in the order k_fv_cells adds them.
+template <bool TH, int ND>
+__global__ __launch_bounds__(256) void k_fv_update(FvArgs a) {
+  constexpr int V = Dims<TH>::V;
+  const int nx = a.nx, ny = ND == 2 ? a.ny : 1;
+  const long total = (long)nx * ny * V;
+  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+    const long c = id / V;
+    const int v = (int)(id - c * V);
+    const long ci = ND == 2 ? c / ny : c, cj = ND == 2 ? c - ci * ny : 0;
+    double du = a.cint[id];
+    const double *fl = a.fx + (ci * ny + cj) * 2 * V;        // left face: hi side
+    const double *fr = a.fx + ((ci + 1) * ny + cj) * 2 * V;  // right face: lo side
+    du -= fl[V + v] + fr[v];
+    if (ND == 2) {
+      const double *fb = a.fy + (ci * (ny + 1) + cj) * 2 * V;
+      const double *ft = a.fy + (ci * (ny + 1) + cj + 1) * 2 * V;
+      du -= fb[V + v] + ft[v];
+    }
+    a.u[id] += du;
   }
 }
 

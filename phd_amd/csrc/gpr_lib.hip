@@ -1417,6 +1417,25 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
     pa.mask = dm;
     pa.ncell = ne;
     pa.itc = c->timing ? c->d_itc : nullptr;
+    // the one-launch FP64 predictor hands the FV corrector face states and
+    // cell integrals instead of qh (same values).  Measured per material
+    // kind: on the 2D P2 ideal-gas Newtonian fluid the epilogue costs less
+    // than the qh round trip it removes (FV 1.04 -> 0.71 ms, predictor
+    // +0.07 ms at 512^2); on the Godunov-Romenski plastic solid the epilogue
+    // costs more than k_fv_cells (1.85 against 1.31 ms at 1024^2), so that
+    // kind and the others keep the qh path.
+    const bool fused = !f32 && c->opts.mode == 0 && c->ndim == 2 && c->N == 3 && !c->thermal &&
+                       material_kind(c->run_mat) == MK_GAS_NEWTONIAN;
+    if (fused) {
+      const long nps = (long)c->N * (c->ndim == 2 ? c->N : 1);
+      pa.fs = (double *)buf(c, "fs", ne * 2 * c->ndim * nps * c->V * sizeof(double), rc);
+      if (!pa.fs) return rc;
+      pa.cint = (double *)buf(c, "cint", nc * c->V * sizeof(double), rc);
+      if (!pa.cint) return rc;
+      pa.fnx = c->run_nx;
+      pa.fny = c->ndim == 2 ? c->run_ny : 1;
+      pa.qh = nullptr;
+    }
     if (f32)
       rc = dispatch<PredictorF32Op>(c, c, pa);
     else if (c->opts.mode == 2)
@@ -1431,6 +1450,10 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
     fa.qh = dqh;
     fa.mask = dm;
     fa.post = 1;
+    if (fused) {
+      fa.fs = pa.fs;
+      fa.cint = pa.cint;
+    }
     rc = fv_buffers(c, fa);
     if (rc) return rc;
     rc = dispatch<FvOp>(c, c, fa);

@@ -155,6 +155,61 @@ __device__ __forceinline__ void riemann_accum(const R *qL, const R *qR, int d, c
 // out[0..8] = (DG_MAT + dt kappa W)^-1, out[9..17] = polar rotation of A,
 // out[18..26] = (DG_MAT + dt rJ W)^-1 (thermal only).  At a relaxed state
 // A = s Rot the distortion source has Jacobian -kappa Pi with
+// The FV cell integrand of one space-time node (centers1/2, fv.cpp:11-109):
+// wgt (S(q) - B_x dq/dx - B_y dq/dy), wgt = dt W_t W_a W_b (TIME) or
+// dt W_a W_b, with dq from the cell's nodes in myQ ([node][V], node =
+// t NS + a N + b).  SRC = false drops S(q).  The FV cell kernel and the
+// predictor's fused epilogue both evaluate it here, so they round alike.
+template <int N, bool TH, int ND, bool TIME, bool SRC, int MK = MK_ANY>
+__device__ __forceinline__ void cell_integrand(const double *q, const double *myQ, int t, int ia, int ib,
+                                               const Mat &m, const Basis &K, double dt, double dx, double dy,
+                                               double *S) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int NS = ND == 1 ? N : N * N;
+  NodeState<double, TH> st;
+  node_state<double, TH, MK>(q, m, st);
+  if (SRC) {
+    source<double, TH, MK>(q, st, m, S);
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) S[v] = 0.0;
+  }
+  double dq[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) acc += K.der[ia][k] * myQ[(t * NS + (ND == 1 ? k : k * N + ib)) * V + v];
+    dq[v] = acc / dx;
+  }
+  bdot_sub(st.v, dq, 0, 1.0, S);
+  if (ND == 2) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += K.der[ib][k] * myQ[(t * NS + ia * N + k) * V + v];
+      dq[v] = acc / dy;
+    }
+    bdot_sub(st.v, dq, 1, 1.0, S);
+  }
+  const double wt = TIME ? dt * K.w[t] : dt;
+  const double wgt = ND == 2 ? wt * K.w[ia] * K.w[ib] : wt * K.w[ia];
+#pragma unroll
+  for (int v = 0; v < V; v++) S[v] = wgt * S[v];
+}
+
+// Face states of the FV corrector's fused form, per extended cell:
+// [side][t][face node][V], sides 0 / 1 the x-faces at x = 0 / 1 of the
+// cell, 2 / 3 the y-faces at y = 0 / 1; face node b on x-sides, a on
+// y-sides.  Each value is sum_k psi_k(end) q(..., k, ...) along the normal,
+// in the order k = 0..N-1 that the face kernel's own extrapolation uses.
+template <int N, int ND, bool TIME = true> struct FaceStates {
+  static constexpr int NSF = ND == 2 ? N : 1;        // face nodes
+  static constexpr int NPS = (TIME ? N : 1) * NSF;  // nodes per side
+  static constexpr int SIDES = 2 * ND;
+};
+
 // Pi(X) = Rot dev(sym(Rot^T X)), kappa = 2 theta1inv cs^2 det(A)^(2/3);
 // the thermal source -(c_alpha^2 T / (kappa rho)) rho J has rate rJ.
 template <int N, bool TH, int MK = MK_ANY, typename R = double>

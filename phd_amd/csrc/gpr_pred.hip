@@ -40,6 +40,9 @@ template <int N, bool TH, int ND, int MK> void pred_launch_mk(hipStream_t s, con
 // pair.  Every other case, and the two-launch FP64 form, uses MK_ANY; the
 // values do not depend on the choice.
 template <int N, bool TH, int ND> hipError_t pred_launch(hipStream_t s, const PredArgs &a, int how) {
+  // the fused FV epilogue exists in the one-launch kernel of one material kind only
+  if (a.fs && !(N == 3 && ND == 2 && !TH && how == PRED_ONE_LAUNCH && material_kind(a.m) == MK_GAS_NEWTONIAN))
+    return hipErrorInvalidValue;
   if constexpr (N == 3 && ND == 2 && !TH) {
     switch (material_kind(a.m)) {
     case MK_GAS_NEWTONIAN: pred_launch_mk<N, TH, ND, MK_GAS_NEWTONIAN>(s, a, how); break;

@@ -74,6 +74,14 @@ struct PredArgs {
   long ncell;
   int dg_it, stiff_it, bailout, mode;  // mode 1: fallback only; 0 or 2: Picard then fallback
   int bail_from;           // > 0: after this many Picard iterations, one growth leaves Picard
+  // fused FV corrector (the one-launch FP64 kernel of the 2D P2 ideal-gas
+  // Newtonian kind, the only one compiled with it): if fs is set, the launch
+  // writes each active cell's face states (FaceStates layout) and each real
+  // cell's FV cell integral (sources, TIME) to cint [fnx][fny][V]; qh may
+  // then be null
+  double *fs;
+  double *cint;
+  int fnx, fny;
   double dg_tol, stiff_tol;
   Mat m;
   // batched launches only
@@ -273,8 +281,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   int phase = active ? ((KIND == KIND_FALLBACK || (FB && a.mode == 1)) ? 1 : 0) : 3;
   int it = 0, pit = 0, grow = 0;
   R prev = R(1e30);
-  // this lane's contributions to a.itc (node 0 of a cell only), added once per wavefront
-  unsigned long long n_pic = 0, n_fb = 0, n_cells = 0, n_fbcells = 0;
+  bool fb_used = phase == 1;  // the cell ran (or runs) the fallback
   const unsigned long long cellbits = (NN == 64 ? ~0ull : ((1ull << NN) - 1)) << (sl * LPC);
   bool need_pre = phase == 1;
   wave_sync();  // sB, sW
@@ -490,8 +497,11 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         ratio = cr;
       }
     }
+    // the next iterate (reset below on a restart); a cell already solved keeps
+    // its result, which the fused epilogue reads
+    if (phase == 0 || phase == 1)
 #pragma unroll
-    for (int v = 0; v < V; v++) q0[v] = rhs[v];  // the next iterate (reset below on a restart)
+      for (int v = 0; v < V; v++) q0[v] = rhs[v];
     [[maybe_unused]] bool push = false;  // KIND_PICARD: append this cell to a.list
     if (phase == 0) {
       it++;
@@ -504,12 +514,11 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       } else if (cnan) {
         to_fallback = true;
       } else {
+        if (a.qh)
 #pragma unroll
-        for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = rhs[v];
+          for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = rhs[v];
         if (node == 0) {
           if (a.iters) a.iters[cell] = it;
-          n_pic += it;
-          n_cells++;
         }
         phase = 2;
       }
@@ -518,30 +527,25 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
           phase = 1;
           pit = 0;
           need_pre = true;
+          fb_used = true;
 #pragma unroll
           for (int v = 0; v < V; v++) q0[v] = myW[s * V + v];
         } else {
           // solved again by the KIND_FALLBACK launch over a.list
-          if (node == 0) {
-            push = true;
-            n_pic += it;
-          }
+          if (node == 0) push = true;
           phase = 3;
         }
       }
     } else if (FB && phase == 1) {
       pit++;
       if (!cmore || pit >= a.stiff_it) {
+        if (a.qh)
 #pragma unroll
-        for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = rhs[v];
+          for (int v = 0; v < V; v++) a.qh[(cell * NN + node) * V + v] = rhs[v];
         bool ok = !cmore && !cnan;
         if (node == 0) {
           if (a.iters) a.iters[cell] = ok ? 1000 + pit : -1;
           if (!ok) atomicAdd(a.fail, 1);
-          n_pic += it;
-          n_fb += pit;
-          n_cells++;
-          n_fbcells++;
         }
         phase = ok ? 2 : 3;
       }
@@ -558,7 +562,73 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     }
     wave_sync();  // sQ, sF and sRat are rewritten next iteration
   }
+  // Fused FV corrector: the final iterate of every node (q0 holds it) gives
+  // the cell's face states and its cell integral (what k_fv_faces extrapolates
+  // and k_fv_cells integrates from qh), so qh need not go through HBM.
+  if constexpr (KIND == KIND_BOTH && !BATCH && sizeof(R) == sizeof(double) && MK == MK_GAS_NEWTONIAN) {
+    if (a.fs) {  // uniform
+      using FS = FaceStates<N, ND>;
+      if (lane_ok)
+#pragma unroll
+        for (int v = 0; v < V; v++) myQ[node * V + v] = q0[v];
+      wave_sync();
+      if (active) {
+        // x-sides from the lanes at a = 0 and a = N-1, y-sides from b = 0 and b = N-1
+#pragma unroll
+        for (int side = 0; side < FS::SIDES; side++) {
+          const bool mine = side < 2 ? ia == (side == 0 ? 0 : N - 1) : ib == (side == 2 ? 0 : N - 1);
+          if (!mine) continue;
+          const int e = side & 1;
+          const int fn = ND == 1 ? 0 : (side < 2 ? ib : ia);
+          double *dst = a.fs + ((cell * FS::SIDES + side) * FS::NPS + t * FS::NSF + fn) * V;
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+              const int nl = ND == 1 ? k : (side < 2 ? k * N + ib : ia * N + k);
+              acc += K.endv[e][k] * (double)myQ[(t * NS + nl) * V + v];
+            }
+            dst[v] = acc;
+          }
+        }
+      }
+      // the cell integral of the real cells (extended cells minus the ghost ring)
+      const long ey = ND == 2 ? a.fny + 2 : 1;
+      const long ci = ND == 2 ? cell / ey - 1 : cell - 1, cj = ND == 2 ? cell % ey - 1 : 0;
+      const bool real = valid && ci >= 0 && ci < a.fnx && cj >= 0 && cj < (ND == 2 ? a.fny : 1);
+      double S[V];
+      cell_integrand<N, TH, ND, true, true, MK>((const double *)q0, (const double *)myQ, t, ia, ib, M, K, dt, dxv,
+                                                a.dy, S);
+      wave_sync();  // every lane has read sQ
+      if (lane_ok)
+#pragma unroll
+        for (int v = 0; v < V; v++) myF[node * V + v] = active ? S[v] : 0.0;
+      wave_sync();
+      if (real) {
+        const long c = ND == 2 ? ci * a.fny + cj : ci;
+        for (int v = node; v < V; v += NN) {
+          double du = 0;
+          for (int k = 0; k < NN; k++) du += (double)myF[k * V + v];
+          a.cint[c * V + v] = du;
+        }
+      }
+    }
+  }
   if (a.itc) {  // every lane is active here
+    // node 0 of each solved cell: its Picard iterations (before the fallback,
+    // if it ran), fallback iterations, and whether it was solved / fell back
+    unsigned long long n_pic = 0, n_fb = 0, n_cells = 0, n_fbcells = 0;
+    if (active && lane_ok && node == 0) {
+      n_pic = (unsigned long long)it;
+      if (fb_used) {
+        n_fb = (unsigned long long)pit;
+        n_cells = 1;
+        n_fbcells = 1;
+      } else if (phase == 2) {
+        n_cells = 1;
+      }
+    }
 #pragma unroll
     for (int o = 1; o < WAVE; o <<= 1) {
       n_pic += __shfl_xor(n_pic, o);
