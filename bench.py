@@ -89,7 +89,7 @@ MI355X_HBM_GBS = 8000.0
 PREDICTOR_TRAFFIC_512 = 1235.2e6
 
 
-def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0, half_step=0):
+def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0, half_step=0, flux=0):
     """The reference's own C++ step loop (oracle/_ref, built from its sources
     by oracle/Makefile) on `nsteps` steps from the GPU's timed starting state,
     OpenMP over `threads` cores; seconds per step as the reference's loop
@@ -106,7 +106,7 @@ def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0
         path = os.path.join(tmp, "state.npz")
         np.savez(path, u=u_start, nX=np.array(nX), dX=np.array(dX), bcs=np.array(bcs), t=t0,
                  count=count0, nsteps=nsteps, V=14, cfl=0.8, par=json.dumps(MP.as_dict()),
-                 split=split, half_step=half_step)
+                 split=split, half_step=half_step, flux=flux)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         p = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "refbind.py"), path],
                            env=env, capture_output=True, text=True, timeout=600)
@@ -429,6 +429,43 @@ def exit_of(args):
     return {"bailout": args.bailout, "bail_from": args.bail_from}
 
 
+def riemann_leg(n, warm, steps, threads, cpu):
+    """The ADER step with each Riemann solver (fluxes.h:7-9: 0 Rusanov, 1
+    Roe, 2 Osher) on the n x n double shear layer through the device loop:
+    `steps` timed steps after `warm` (past the dt ramp).  Roe and Osher
+    decompose a V x V system matrix per face node (Osher at each of the N
+    path nodes).  mu = 1e-2, where the reference's Picard converges in every
+    cell: at 1e-3 its stiff fallback aborts on an Eigen assertion in 2D.
+    With cpu, the reference's own step with the same solver on one step from
+    the GPU's timed starting state."""
+    from phd_amd import problems
+    from phd_amd.runner import DeviceRun
+    u0, MP, _, dX, bcs = problems.double_shear_2d(n, μ=1e-2)
+    out = {"config": f"2D GPR double shear layer {n}x{n} periodic, mu = 1e-2, FP64, P2, CFL 0.8, "
+                     f"steps {warm}-{warm + steps - 1}", "cells": n * n}
+    for flux, name in ((0, "rusanov"), (1, "roe"), (2, "osher")):
+        run = DeviceRun(u0.ravel(), MP, [n, n], dX, bcs, CFL=0.8, flux=flux, **PICARD_EXIT)
+        run.steps(warm)
+        st0 = run.status()
+        u_w = run.host()
+        sec, tm = timed_run(run, steps)
+        K = tm["steps"]
+        r = {"value": n * n * steps / sec, "ms_per_step": 1e3 * sec / steps,
+             "phase_ms_per_step": {k: v / K for k, v in tm["ms"].items()}}
+        if cpu:
+            ref = cpu_reference(u_w, MP, [n, n], dX, bcs, st0["t"], st0["count"], 1, threads, flux=flux)
+            if ref is not None and "error" not in ref:
+                r["cpu_reference"] = {"value": n * n * ref["steps"] / ref["sec"], "cores": threads,
+                                      "sample": "1 step from the GPU's timed starting state, the "
+                                                "reference C++ (oracle/_ref)"}
+                r["gpu_over_cpu"] = r["value"] / r["cpu_reference"]["value"]
+            elif ref is not None:
+                r["cpu_reference_error"] = ref["error"]
+        out[name] = r
+        run.close()
+    return out
+
+
 def iteration_histogram(u, MP, nX, dX, bcs, t, count, exit_opts, mode, precision=0):
     """Per-cell predictor iterations of one step from state u (the timed
     leg's starting state), through the host entry point gpr_predictor with
@@ -545,6 +582,8 @@ def main():
     ap.add_argument("--batch-sample", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-split", action="store_true", help="skip the split-WENO leg")
+    ap.add_argument("--no-riemann", action="store_true", help="skip the Roe / Osher leg")
+    ap.add_argument("--riemann-n", type=int, default=128)
     ap.add_argument("--adj-n", type=int, default=256)
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--adj-k", type=int, default=25)
@@ -656,6 +695,10 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["split"] = split_leg(n, args.steps, args.warmup, threads, not args.no_cpu)
         progress("split leg done")
+    if not args.no_riemann:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["riemann_solvers"] = riemann_leg(args.riemann_n, 6, 5, threads, not args.no_cpu)
+        progress("Roe / Osher leg done")
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline_leg(u_w, MP, nX, dX, bcs, st0, args, threads, value)

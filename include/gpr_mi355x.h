@@ -33,7 +33,7 @@ extern "C" {
 
 #define GPR_OK 0
 #define GPR_E_ARG (-1)          /* bad argument / shape */
-#define GPR_E_UNSUPPORTED (-2)  /* option outside this path (e.g. FLUX != 0) */
+#define GPR_E_UNSUPPORTED (-2)  /* option outside this path (e.g. nmat > 1) */
 #define GPR_E_HIP (-3)          /* HIP runtime error */
 #define GPR_E_EOS (-4)          /* EOS code not recognised (mg.cpp:27) */
 #define GPR_E_PREDICTOR (-5)    /* a cell's predictor converged in neither phase */
@@ -67,6 +67,8 @@ typedef struct gpr_solver_opts {
   int32_t bail_from;  /* 0: off; j > 0: a cell that has run j Picard iterations
                          without converging leaves Picard the first time its
                          stopping statistic grows (in addition to bailout) */
+  int32_t flux;       /* device loop: the FV Riemann solver, 0 Rusanov, 1 Roe,
+                         2 Osher (fluxes.h:7-9); iterator() sets it from FLUX */
 } gpr_solver_opts;
 
 const char *gpr_last_error(void);
@@ -108,6 +110,18 @@ int gpr_d_rus(gpr_ctx *ctx, double *ret, const double *qL, const double *qR, int
               const gpr_par *MP);
 int gpr_bint(gpr_ctx *ctx, double *ret, const double *qL, const double *qR, int64_t count, int d,
              const gpr_par *MP);
+/* solvers.fv.D_ROE (fluxes.cpp:45-64) and solvers.fv.D_OSH (fluxes.cpp:22-43),
+   bindings.cpp:246-247: ret[count][V]; GPR_E_NONFINITE if a QR iteration did
+   not converge (that result is NaN) */
+int gpr_d_roe(gpr_ctx *ctx, double *ret, const double *qL, const double *qR, int64_t count, int d,
+              const gpr_par *MP);
+int gpr_d_osh(gpr_ctx *ctx, double *ret, const double *qL, const double *qR, int64_t count, int d,
+              const gpr_par *MP);
+/* system.system_matrix (equations.cpp:169-176), system.dFdP and system.dPdQ
+   (jacobians.cpp:10-152), bindings.cpp:201,208-209: ret[count][V][V] row-major */
+int gpr_system_matrix(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP);
+int gpr_dfdp(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP);
+int gpr_dpdq(gpr_ctx *ctx, double *ret, const double *Q, int64_t count, const gpr_par *MP);
 
 /* --- grid (boundaries / extend_mask, grid.cpp:147-230, bindings.cpp:66-67) */
 int gpr_boundaries(gpr_ctx *ctx, const double *u, double *ub, const int32_t *nX, const int32_t *bcs);
@@ -125,8 +139,8 @@ int gpr_predictor(gpr_ctx *ctx, double *qh, const double *wh, double dt, const d
                   int64_t ncell, int stiff, int stiff_ig, const gpr_par *MP, const uint8_t *mask,
                   int32_t *iters);
 /* solvers.fv.fv_launcher(u, rec, nX, dt, dX, SOURCES, TIME, FLUX, MP, mask)
- * (fv.h:17-18), FLUX = 0 (Rusanov).  TIME = 1: rec is the predictor's qh
- * ([cells+halo][N time][N^d space][V]); TIME = 0: one time level per cell
+ * (fv.h:17-18), FLUX = 0 Rusanov, 1 Roe, 2 Osher (fluxes.h:7-9).  TIME = 1:
+ * rec is the predictor's qh ([cells+halo][N time][N^d space][V]); TIME = 0: one time level per cell
  * ([cells+halo][N^d][V], the split step's wh), weighted by dt (fv.cpp:46-54). */
 int gpr_fv_launcher(gpr_ctx *ctx, double *u, const double *qh, const int32_t *nX, double dt,
                     const double *dX, int sources, int time, int flux, const gpr_par *MP,
@@ -136,7 +150,7 @@ int gpr_fv_launcher(gpr_ctx *ctx, double *u, const double *qh, const int32_t *nX
 int gpr_fv_centers(gpr_ctx *ctx, double *u, const double *rec, const int32_t *nX, double dt,
                    const double *dX, int sources, int time, const gpr_par *MP, const uint8_t *mask);
 /* solvers.fv.interfs1 / interfs2 (fv.cpp:151-160, 270-279): the face terms of
- * fv_launcher alone, FLUX = 0 */
+ * fv_launcher alone, with the Riemann solver FLUX */
 int gpr_fv_interfs(gpr_ctx *ctx, double *u, const double *rec, const int32_t *nX, double dt,
                    const double *dX, int time, int flux, const gpr_par *MP, const uint8_t *mask);
 /* solvers.timestep (iterator.cpp:54-83) for one material: *dt_out */
@@ -159,7 +173,9 @@ int gpr_ader_stepper(gpr_ctx *ctx, double *u, const double *ub, const int32_t *n
  * converged in neither phase is written to the device int *d_nfail.
  * Every call on a GPU shares that GPU's scratch: calls that overlap on one
  * GPU (contexts on different streams) race and must be serialised by the
- * caller.  Values equal gpr_ader_stepper's for the same inputs. */
+ * caller.  Values equal gpr_ader_stepper's for the same inputs.  FLUX must
+ * be 0: the Roe and Osher solvers keep per-lane matrices in private memory,
+ * which the HIP runtime allocates when such a kernel first runs. */
 #define GPR_NOALLOC_SCRATCH_BYTES (64u << 20)
 int gpr_ader_stepper_noalloc(gpr_ctx *ctx, double *d_u, const double *d_ub, const int32_t *nX,
                              double dt, const double *dX, int stiff, int flux, const gpr_par *MP,
@@ -190,7 +206,7 @@ int gpr_contorted(gpr_ctx *ctx, int32_t *result, const double *u, int64_t ncell,
 int gpr_reset_distortion(gpr_ctx *ctx, double *u, int64_t ncell, const gpr_par *MP);
 
 /* solvers.iterator (iterator.cpp:85-170) for one material (LSET = 0),
- * FLUX = 0; SPLIT = 1 takes the split-WENO step (with HALF_STEP) instead of
+ * FLUX = 0, 1 or 2; SPLIT = 1 takes the split-WENO step (with HALF_STEP) instead of
  * the ADER step.  u is advanced in place to tf.  snapshots
  * ([100][ncell*V], may be null) receives u at each 1% of tf as the reference's
  * returned list; *nsnap the number filled.  On a non-finite state the loop

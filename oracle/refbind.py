@@ -324,7 +324,7 @@ class RefSolver:
 def _time_run_main(argv):
     """python oracle/refbind.py STATE.npz: time the reference's step loop on
     the state in STATE.npz (u, nX, dX, bcs, t, count, nsteps, par as JSON;
-    split and half_step select the split step) and
+    split and half_step select the split step, flux the Riemann solver) and
     print one JSON line {"secs": [...], "status": s}.  Run as a child process
     so that an abort inside the reference (its Eigen assertions are live)
     cannot take the caller down."""
@@ -340,6 +340,8 @@ def _time_run_main(argv):
     R = RefSolver(3, int(d["V"]) == 17)
     split = int(d["split"]) if "split" in d.files else 0
     half = int(d["half_step"]) if "half_step" in d.files else 0
+    if "flux" in d.files:
+        R.set_flux(int(d["flux"]))
     _, _, _, _, secs, status = R.run(d["u"], nX, d["dX"], float(d["cfl"]), d["bcs"], 1e300,
                                      int(d["nsteps"]), MP, t=float(d["t"]),
                                      count=int(d["count"]), split=bool(split),

@@ -35,6 +35,7 @@ SYMBOLS = [
     "gpr_adj_release", "gpr_adj_set_solver", "gpr_adj_stats", "gpr_adj_grad_kappa",
     "gpr_split_stepper", "gpr_ode_launcher", "gpr_analytic_solver", "gpr_midstepper",
     "gpr_fv_centers", "gpr_fv_interfs", "gpr_ader_stepper_noalloc",
+    "gpr_d_roe", "gpr_d_osh", "gpr_system_matrix", "gpr_dfdp", "gpr_dpdq",
 ]
 
 
@@ -50,7 +51,7 @@ class SolverOpts(ctypes.Structure):
                 ("bailout", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("contorted_tol", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("split", ctypes.c_int32), ("half_step", ctypes.c_int32),
-                ("bail_from", ctypes.c_int32)]
+                ("bail_from", ctypes.c_int32), ("flux", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p
@@ -79,6 +80,11 @@ ARGTYPES = {
     "gpr_thermo_acoustic_tensor": [_P, _Dp, _Dp, _l, _i, _M],
     "gpr_d_rus": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
     "gpr_bint": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
+    "gpr_d_roe": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
+    "gpr_d_osh": [_P, _Dp, _Dp, _Dp, _l, _i, _M],
+    "gpr_system_matrix": [_P, _Dp, _Dp, _l, _i, _M],
+    "gpr_dfdp": [_P, _Dp, _Dp, _l, _i, _M],
+    "gpr_dpdq": [_P, _Dp, _Dp, _l, _M],
     "gpr_boundaries": [_P, _Dp, _Dp, _Ip, _Ip],
     "gpr_extend_mask": [_P, _Up, _Up, _Ip],
     "gpr_weno_launcher": [_P, _Dp, _Dp, _Ip, _M],

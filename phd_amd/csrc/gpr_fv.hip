@@ -21,7 +21,10 @@ template <int N, bool TH, int ND, bool TIME, bool SRC> void fv_pair(hipStream_t 
   // k_fv_faces maps block b to work item (b % 8) * M + b / 8 with
   // M = ceil(items / 8): the grid is 8 M blocks, those past the items idle
   const long items = (nf + FPW - 1) / FPW;
-  if (a.parts & 2)
+  if ((a.parts & 2) && a.flux != 0)
+    hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME, false, true>), dim3(eig_face_grid(items)), dim3(WAVE), 0, s,
+                       a);
+  else if (a.parts & 2)
     hipLaunchKernelGGL((k_fv_faces<N, TH, ND, TIME>), dim3(8 * ((items + 7) / 8)), dim3(WAVE), 0, s, a);
   const long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
   constexpr int CPW = CellShape<N, ND, TIME>::CPW;

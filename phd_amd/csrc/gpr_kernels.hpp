@@ -16,6 +16,7 @@
 
 #include <type_traits>
 
+#include "gpr_eigflux.hpp"
 #include "gpr_pointwise.hpp"
 #include "gpr_relax.hpp"
 
@@ -193,6 +194,8 @@ struct FvArgs {
   int sources;           // 1: the cell integral includes S(q)
   int ode;               // with post: the split step's closing half-step ODE first
   int parts;             // 1: cell integrals (centers1/2), 2: face terms (interfs1/2), 3: both
+  int flux;              // Riemann solver (fluxes.h:7-9): 0 Rusanov, 1 Roe, 2 Osher
+  double *eig_ws;        // Roe / Osher: eig_face_grid(items) x WAVE lanes x EIG_WS_DOUBLES(V)
   // fused form (ADER step, TIME and SOURCES): the predictor wrote each active
   // extended cell's face states (FaceStates layout) and each real cell's
   // cell integral; qh is then unused
@@ -217,9 +220,15 @@ template <int N, bool TH, int ND, bool TIME = true> struct FaceShape {
   static constexpr int FPW = WAVE / NT;        // faces per wavefront
 };
 
+// Roe / Osher face launches run a bounded grid (each block loops over its
+// share of the face groups) so that their workspace is bounded too.
+constexpr long EIG_FACE_BLOCKS = 2048;
+inline long eig_face_grid(long items) { return items < EIG_FACE_BLOCKS ? items : EIG_FACE_BLOCKS; }
+
 // FS: the face states come from a.fs (the predictor's fused epilogue)
 // instead of being extrapolated from qh here; the values are the same.
-template <int N, bool TH, int ND, bool TIME = true, bool FS = false>
+// EIG: the Riemann solver is a.flux's Roe (1) or Osher (2) instead of Rusanov.
+template <int N, bool TH, int ND, bool TIME = true, bool FS = false, bool EIG = false>
 __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
@@ -246,100 +255,118 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   const long nitems = (nfx + nfy + FPW - 1) / FPW;
   const long M = (nitems + 7) / 8;
   const long b = blockIdx.x;
-  long item = (b % 8) * M + b / 8;
-  const bool item_ok = item < nitems;
-  const long id0 = item * FPW;
-  const long id = id0 + fl;  // the face in row order
-  const bool ok = item_ok && fl < FPW && id < nfx + nfy;
-  int d = 0;
-  long e0 = 0, e1 = 0, fid = 0;  // fid: the face's slot in fx (d = 0) or fy (d = 1)
-  if (ok) {
-    if (ND == 1) {
-      fid = id;
-      e0 = id;
-      e1 = id + 1;
-    } else {
-      const long per = 2L * ny + 1;  // faces of one row
-      const long i = id / per, k = id - i * per;
-      if (i == nx || k < ny) {  // x-face (i, j)
-        const long j = k;
-        fid = i * ny + j;
-        e0 = i * (ny + 2) + (j + 1);
-        e1 = (i + 1) * (ny + 2) + (j + 1);
-      } else {  // y-face (i, j) of row i
-        d = 1;
-        const long j = k - ny;
-        fid = i * (ny + 1) + j;
-        e0 = (i + 1) * (ny + 2) + j;
-        e1 = e0 + 1;
-      }
-    }
-  }
-  double lo[V], hi[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    lo[v] = 0.0;
-    hi[v] = 0.0;
-  }
-  if (ok && a.mask[e0] && a.mask[e1]) {
-    const double h = d == 0 ? a.dx : a.dy;
-    const double kt = (TIME ? dt * K.w[t] : dt) / (2.0 * h);
-    const double kk = ND == 2 ? kt * K.w[sf] : kt;
-    double qL[V], qR[V];
-    if constexpr (FS) {
-      using F = FaceStates<N, ND, TIME>;
-      const int sl = 2 * d;  // e0's high side is sl + 1, e1's low side sl
-      const double *pl = a.fs + ((e0 * F::SIDES + sl + 1) * F::NPS + t * F::NSF + sf) * V;
-      const double *pr = a.fs + ((e1 * F::SIDES + sl) * F::NPS + t * F::NSF + sf) * V;
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        qL[v] = pl[v];
-        qR[v] = pr[v];
-      }
-    } else {
-      const double *pl = a.qh + (e0 * NN + t * NS) * V;
-      const double *pr = a.qh + (e1 * NN + t * NS) * V;
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        double l = 0, rr = 0;
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-          int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
-          l += K.endv[1][k] * pl[nl * V + v];
-          rr += K.endv[0][k] * pr[nl * V + v];
-        }
-        qL[v] = l;
-        qR[v] = rr;
-      }
-    }
-    // the direction as a compile-time constant in each branch, so that the
-    // physics indexes its small arrays statically (no stack frame)
-    if (d == 0)
-      riemann_accum<N, TH>(qL, qR, 0, a.m, K, kk, lo, hi);
-    else
-      riemann_accum<N, TH>(qL, qR, 1, a.m, K, kk, lo, hi);
-  }
+  // EIG: block b takes items b, b + grid, ...; otherwise its one banded item
+  const long item0 = EIG ? b : (b % 8) * M + b / 8;
+  const long istep = EIG ? (long)gridDim.x : nitems + 1;
+  [[maybe_unused]] const LaneMem ews{EIG ? a.eig_ws + (b * WAVE + lane) : nullptr, (long)gridDim.x * WAVE};
   __shared__ long sOut[FPW];  // each face's output slot: fid in fx, or nfx + fid in fy
-  if (fl < FPW) {
-    double *my = sR + (fl * NT + r) * 2 * V;
+  for (long item = item0; EIG ? item < nitems : item == item0; item += istep) {
+    const bool item_ok = item < nitems;
+    const long id0 = item * FPW;
+    const long id = id0 + fl;  // the face in row order
+    const bool ok = item_ok && fl < FPW && id < nfx + nfy;
+    int d = 0;
+    long e0 = 0, e1 = 0, fid = 0;  // fid: the face's slot in fx (d = 0) or fy (d = 1)
+    if (ok) {
+      if (ND == 1) {
+        fid = id;
+        e0 = id;
+        e1 = id + 1;
+      } else {
+        const long per = 2L * ny + 1;  // faces of one row
+        const long i = id / per, k = id - i * per;
+        if (i == nx || k < ny) {  // x-face (i, j)
+          const long j = k;
+          fid = i * ny + j;
+          e0 = i * (ny + 2) + (j + 1);
+          e1 = (i + 1) * (ny + 2) + (j + 1);
+        } else {  // y-face (i, j) of row i
+          d = 1;
+          const long j = k - ny;
+          fid = i * (ny + 1) + j;
+          e0 = (i + 1) * (ny + 2) + j;
+          e1 = e0 + 1;
+        }
+      }
+    }
+    double lo[V], hi[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      my[v] = lo[v];
-      my[V + v] = hi[v];
+      lo[v] = 0.0;
+      hi[v] = 0.0;
     }
-    if (r == 0) sOut[fl] = ok ? (d == 0 ? fid : nfx + fid) : -1;
-  }
-  __syncthreads();
-  // one lane per (face, output): sum the face's NT lanes in a fixed order
-  for (int o = lane; o < FPW * 2 * V; o += WAVE) {
-    const int f = o / (2 * V), k = o - f * 2 * V;
-    const long slot = sOut[f];
-    if (slot < 0) continue;
-    double acc = 0.0;
+    if (ok && a.mask[e0] && a.mask[e1]) {
+      const double h = d == 0 ? a.dx : a.dy;
+      const double kt = (TIME ? dt * K.w[t] : dt) / (2.0 * h);
+      const double kk = ND == 2 ? kt * K.w[sf] : kt;
+      double qL[V], qR[V];
+      if constexpr (FS) {
+        using F = FaceStates<N, ND, TIME>;
+        const int sl = 2 * d;  // e0's high side is sl + 1, e1's low side sl
+        const double *pl = a.fs + ((e0 * F::SIDES + sl + 1) * F::NPS + t * F::NSF + sf) * V;
+        const double *pr = a.fs + ((e1 * F::SIDES + sl) * F::NPS + t * F::NSF + sf) * V;
 #pragma unroll
-    for (int q = 0; q < NT; q++) acc += sR[(f * NT + q) * 2 * V + k];
-    double *out = slot < nfx ? a.fx + slot * 2 * V : a.fy + (slot - nfx) * 2 * V;
-    out[k] = acc;
+        for (int v = 0; v < V; v++) {
+          qL[v] = pl[v];
+          qR[v] = pr[v];
+        }
+      } else {
+        const double *pl = a.qh + (e0 * NN + t * NS) * V;
+        const double *pr = a.qh + (e1 * NN + t * NS) * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double l = 0, rr = 0;
+#pragma unroll
+          for (int k = 0; k < N; k++) {
+            int nl = ND == 1 ? k : (d == 0 ? k * N + sf : sf * N + k);
+            l += K.endv[1][k] * pl[nl * V + v];
+            rr += K.endv[0][k] * pr[nl * V + v];
+          }
+          qL[v] = l;
+          qR[v] = rr;
+        }
+      }
+      if constexpr (EIG) {
+        // D_ROE / D_OSH (fv.cpp:129-139,202-212), then Bint, weighted as Rusanov's
+        double f[V], b[V];
+        eig_flux<N, TH>(qL, qR, d, a.m, K, a.flux, f, ews);
+        bint<N, TH>(qL, qR, d, K, b);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          lo[v] += kk * (b[v] + f[v]);
+          hi[v] += kk * (b[v] - f[v]);
+        }
+      } else {
+        // the direction as a compile-time constant in each branch, so that the
+        // physics indexes its small arrays statically (no stack frame)
+        if (d == 0)
+          riemann_accum<N, TH>(qL, qR, 0, a.m, K, kk, lo, hi);
+        else
+          riemann_accum<N, TH>(qL, qR, 1, a.m, K, kk, lo, hi);
+      }
+    }
+    if (fl < FPW) {
+      double *my = sR + (fl * NT + r) * 2 * V;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        my[v] = lo[v];
+        my[V + v] = hi[v];
+      }
+      if (r == 0) sOut[fl] = ok ? (d == 0 ? fid : nfx + fid) : -1;
+    }
+    __syncthreads();
+    // one lane per (face, output): sum the face's NT lanes in a fixed order
+    for (int o = lane; o < FPW * 2 * V; o += WAVE) {
+      const int f = o / (2 * V), k = o - f * 2 * V;
+      const long slot = sOut[f];
+      if (slot < 0) continue;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < NT; q++) acc += sR[(f * NT + q) * 2 * V + k];
+      double *out = slot < nfx ? a.fx + slot * 2 * V : a.fy + (slot - nfx) * 2 * V;
+      out[k] = acc;
+    }
+    if constexpr (EIG) __syncthreads();  // sR and sOut are rewritten for the next item
   }
 }
 

@@ -434,8 +434,35 @@ template <int N, bool TH, int ND> struct ResetOp {
 // pointwise operations over a batch of states
 enum PointOp {
   OP_FLUX, OP_SOURCE, OP_BDOT, OP_EIGS, OP_TAT, OP_DRUS, OP_BINT, OP_RENORM, OP_RESET, OP_CONTORTED,
-  OP_ODE_DISTORTION, OP_ODE_THERMAL  // dt in tol
+  OP_ODE_DISTORTION, OP_ODE_THERMAL,  // dt in tol
+  OP_SYSMAT, OP_DFDP, OP_DPDQ, OP_DROE, OP_DOSH
 };
+
+// the system matrix, its factors and the Roe / Osher fluxes, one state (or
+// pair) per thread over a bounded grid; ws: a workspace slice per thread
+template <int N, bool TH>
+__global__ void k_pointwise_eig(int op, double *ret, const double *A, const double *B, long count, int d, Mat m,
+                                Basis K, int *flag, double *ws) {
+  constexpr int V = Dims<TH>::V;
+  constexpr long nn = (long)V * V;
+  const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x, nthr = (long)gridDim.x * blockDim.x;
+  const LaneMem ews{ws + tid, nthr};
+  for (long i = tid; i < count; i += nthr) {
+    double q[V];
+    for (int v = 0; v < V; v++) q[v] = A[i * V + v];
+    switch (op) {
+    case OP_SYSMAT: system_matrix<TH>(q, d, m, LaneMem{ret + i * nn, 1}, ews, ews.at(nn)); break;
+    case OP_DFDP: ef_dFdP<TH>(q, d, m, LaneMem{ret + i * nn, 1}); break;
+    case OP_DPDQ: ef_dPdQ<TH>(q, m, LaneMem{ret + i * nn, 1}); break;
+    default: {  // OP_DROE, OP_DOSH
+      double qR[V], r[V];
+      for (int v = 0; v < V; v++) qR[v] = B[i * V + v];
+      if (eig_flux<N, TH>(q, qR, d, m, K, op == OP_DROE ? 1 : 2, r, ews)) atomicOr(flag, 1);
+      for (int v = 0; v < V; v++) ret[i * V + v] = r[v];
+    }
+    }
+  }
+}
 
 template <int N, bool TH>
 __global__ void k_pointwise(int op, double *ret, const double *A, const double *B, long count, int d, Mat m,
@@ -547,12 +574,24 @@ int pointwise(gpr_ctx *c, int op, double *ret, const double *A, const double *B,
   HIP_TRY(hipMemsetAsync(dflag, 0, sizeof(int), c->stream));
   const Basis &K = basis(c->N);
   int g = grid_for(count, 128);
+  if (op >= OP_SYSMAT) {  // a bounded grid and its workspace
+    g = std::min(g, 64);
+    double *dws = (double *)buf(c, "pws", (size_t)g * 128 * EIG_WS_DOUBLES(c->V) * 8, rc);
+    if (!dws) return rc;
+#define GPR_PWE(n, t)                                                                                      \
+  if (c->N == n && (bool)c->thermal == t)                                                                \
+    hipLaunchKernelGGL((k_pointwise_eig<n, t>), dim3(g), dim3(128), 0, c->stream, op, dR, dA, dB, (long)count, d, \
+                       m, K, dflag, dws);
+    GPR_PWE(3, false) GPR_PWE(3, true) GPR_PWE(2, false) GPR_PWE(2, true)
+#undef GPR_PWE
+  } else {
 #define GPR_PW(n, t)                                                                                          \
   if (c->N == n && (bool)c->thermal == t)                                                                   \
     hipLaunchKernelGGL((k_pointwise<n, t>), dim3(g), dim3(128), 0, c->stream, op, dR, dA, dB, (long)count, d, m, \
                        K, tol, dflag);
-  GPR_PW(3, false) GPR_PW(3, true) GPR_PW(2, false) GPR_PW(2, true)
+    GPR_PW(3, false) GPR_PW(3, true) GPR_PW(2, false) GPR_PW(2, true)
 #undef GPR_PW
+  }
   HIP_TRY(hipGetLastError());
   if (ret_per) HIP_TRY(hipMemcpyAsync(ret, dR, count * ret_per * 8, hipMemcpyDeviceToHost, c->stream));
   if (flag_out) HIP_TRY(hipMemcpyAsync(flag_out, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -644,6 +683,13 @@ int fv_buffers(gpr_ctx *c, FvArgs &a) {
   if (!a.fx) return rc;
   a.fy = (double *)buf(c, "fy", nfy * 2 * c->V * 8, rc);
   if (!a.fy) return rc;
+  if (a.flux != 0) {  // the Roe / Osher face launch's workspace (one slice per lane of its grid)
+    const long lanes_per_face = (long)(a.time ? c->N : 1) * (c->ndim == 2 ? c->N : 1);
+    const long fpw = WAVE / lanes_per_face;
+    const long items = ((c->ndim == 2 ? nfx + nfy : nfx) + fpw - 1) / fpw;
+    a.eig_ws = (double *)buf(c, "eig_ws", eig_face_grid(items) * WAVE * EIG_WS_DOUBLES(c->V) * 8, rc);
+    if (!a.eig_ws) return rc;
+  }
   return GPR_OK;
 }
 
@@ -687,6 +733,7 @@ int gpr_ctx_create(gpr_ctx **ctx, int N, int ndim, int thermal) {
   c->opts.split = 0;
   c->opts.half_step = 0;
   c->opts.bail_from = 0;
+  c->opts.flux = 0;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_st, sizeof(StepState));
@@ -737,7 +784,7 @@ int gpr_ctx_set_opts(gpr_ctx *c, const gpr_solver_opts *o) {
   if (!c || !o) return fail(GPR_E_ARG, "null argument");
   if (o->dg_it < 1 || o->stiff_it < 1 || !(o->dg_tol > 0) || !(o->stiff_tol > 0) || o->bailout < 0 ||
       o->mode < 0 || o->mode > 2 || o->precision < 0 || o->precision > 1 || o->split < 0 || o->split > 1 ||
-      o->half_step < 0 || o->half_step > 1 || o->bail_from < 0)
+      o->half_step < 0 || o->half_step > 1 || o->bail_from < 0 || o->flux < 0 || o->flux > 2)
     return fail(GPR_E_ARG, "invalid solver options");
   c->opts = *o;
   return GPR_OK;
@@ -815,6 +862,34 @@ int gpr_bint(gpr_ctx *c, double *ret, const double *qL, const double *qR, int64_
   if (!c) return fail(GPR_E_ARG, "null context");
   if (!qR) return fail(GPR_E_ARG, "null qR");
   return pointwise(c, OP_BINT, ret, qL, qR, count, d, MP, c->V);
+}
+static int eig_pointwise(gpr_ctx *c, int op, double *ret, const double *qL, const double *qR, int64_t count, int d,
+                         const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  if (!qR) return fail(GPR_E_ARG, "null qR");
+  int32_t bad = 0;
+  int rc = pointwise(c, op, ret, qL, qR, count, d, MP, c->V, 0.0, &bad);
+  if (rc) return rc;
+  if (bad) return fail(GPR_E_NONFINITE, "eigendecomposition: QR did not converge (result NaN)");
+  return GPR_OK;
+}
+int gpr_d_roe(gpr_ctx *c, double *ret, const double *qL, const double *qR, int64_t count, int d, const gpr_par *MP) {
+  return eig_pointwise(c, OP_DROE, ret, qL, qR, count, d, MP);
+}
+int gpr_d_osh(gpr_ctx *c, double *ret, const double *qL, const double *qR, int64_t count, int d, const gpr_par *MP) {
+  return eig_pointwise(c, OP_DOSH, ret, qL, qR, count, d, MP);
+}
+int gpr_system_matrix(gpr_ctx *c, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_SYSMAT, ret, Q, nullptr, count, d, MP, (size_t)c->V * c->V);
+}
+int gpr_dfdp(gpr_ctx *c, double *ret, const double *Q, int64_t count, int d, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_DFDP, ret, Q, nullptr, count, d, MP, (size_t)c->V * c->V);
+}
+int gpr_dpdq(gpr_ctx *c, double *ret, const double *Q, int64_t count, const gpr_par *MP) {
+  if (!c) return fail(GPR_E_ARG, "null context");
+  return pointwise(c, OP_DPDQ, ret, Q, nullptr, count, 0, MP, (size_t)c->V * c->V);
 }
 int gpr_renorm_distortion(gpr_ctx *c, double *u, int64_t ncell, const gpr_par *MP) {
   if (!c) return fail(GPR_E_ARG, "null context");
@@ -962,7 +1037,7 @@ int gpr_fv_interfs(gpr_ctx *c, double *u, const double *rec, const int32_t *nX, 
 static int fv_host(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, double dt, const double *dX,
                    int sources, int time, int flux, const gpr_par *MP, const uint8_t *mask, int parts) {
   if (!c || !u || !qh || !dX || !mask) return fail(GPR_E_ARG, "null argument");
-  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  if (flux < 0 || flux > 2) return fail(GPR_E_ARG, "FLUX must be 0 (Rusanov), 1 (Roe) or 2 (Osher) (fluxes.h:7-9)");
   int rc = check_nX(c, nX);
   if (rc) return rc;
   Mat m;
@@ -987,6 +1062,7 @@ static int fv_host(gpr_ctx *c, double *u, const double *qh, const int32_t *nX, d
   a.time = time != 0;
   a.sources = sources != 0;
   a.parts = parts;
+  a.flux = flux;
   rc = fv_buffers(c, a);
   if (rc) return rc;
   rc = dispatch<FvOp>(c, c, a);
@@ -1028,7 +1104,7 @@ int gpr_timestep(gpr_ctx *c, double *dt_out, const double *u, const int32_t *nX,
 int gpr_ader_stepper(gpr_ctx *c, double *u, const double *ub, const int32_t *nX, double dt, const double *dX,
                      int stiff, int flux, const gpr_par *MP, const uint8_t *mask) {
   if (!c || !u || !ub || !dX || !mask) return fail(GPR_E_ARG, "null argument");
-  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  if (flux < 0 || flux > 2) return fail(GPR_E_ARG, "FLUX must be 0 (Rusanov), 1 (Roe) or 2 (Osher) (fluxes.h:7-9)");
   int rc = check_nX(c, nX);
   if (rc) return rc;
   Mat m;
@@ -1064,6 +1140,7 @@ int gpr_ader_stepper(gpr_ctx *c, double *u, const double *ub, const int32_t *nX,
   rc = dispatch<PredictorOp>(c, c, pa);
   if (rc) return rc;
   FvArgs fa = fv_args(c, m, dt, dX, nX);
+  fa.flux = flux;
   fa.u = du;
   fa.qh = dqh;
   fa.mask = dm;
@@ -1083,7 +1160,8 @@ int gpr_ader_stepper_noalloc(gpr_ctx *c, double *d_u, const double *d_ub, const 
                              const double *dX, int stiff, int flux, const gpr_par *MP, const uint8_t *d_mask,
                              int32_t *d_nfail) {
   if (!c || !d_u || !d_ub || !dX || !d_mask || !d_nfail) return fail(GPR_E_ARG, "null argument");
-  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  if (flux != 0)
+    return fail(GPR_E_UNSUPPORTED, "FLUX = 1, 2 need per-lane scratch that the HIP runtime allocates: use gpr_ader_stepper");
   int rc = check_nX(c, nX);
   if (rc) return rc;
   Mat m;
@@ -1127,6 +1205,7 @@ int gpr_ader_stepper_noalloc(gpr_ctx *c, double *d_u, const double *d_ub, const 
   rc = dispatch<PredictorOp>(c, c, pa);
   if (rc) return rc;
   FvArgs fa = fv_args(c, m, dt, dX, nX);
+  fa.flux = flux;
   fa.u = d_u;
   fa.qh = dqh;
   fa.mask = d_mask;
@@ -1142,7 +1221,7 @@ int gpr_ader_stepper_noalloc(gpr_ctx *c, double *d_u, const double *d_ub, const 
 int gpr_split_stepper(gpr_ctx *c, double *u, double *ub, const int32_t *nX, double dt, const double *dX,
                       int half_step, int flux, const gpr_par *MP, const uint8_t *mask) {
   if (!c || !u || !ub || !dX || !mask) return fail(GPR_E_ARG, "null argument");
-  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  if (flux < 0 || flux > 2) return fail(GPR_E_ARG, "FLUX must be 0 (Rusanov), 1 (Roe) or 2 (Osher) (fluxes.h:7-9)");
   int rc = check_nX(c, nX);
   if (rc) return rc;
   Mat m;
@@ -1172,6 +1251,7 @@ int gpr_split_stepper(gpr_ctx *c, double *u, double *ub, const int32_t *nX, doub
                              c->ndim == 2 ? dX[1] : 1.0, m);
   if (rc) return rc;
   FvArgs fa = fv_args(c, m, dt, dX, nX);
+  fa.flux = flux;
   fa.u = du;
   fa.qh = dwh;
   fa.mask = dm;
@@ -1387,6 +1467,7 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
       if (rc) return rc;
       if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
       FvArgs fs = fv_args(c, c->run_mat, 0, dX, nX);
+      fs.flux = c->opts.flux;
       fs.st = c->d_st;
       fs.u = c->run_u;
       fs.qh = dwh;
@@ -1425,7 +1506,7 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
     // costs more than k_fv_cells (1.85 against 1.31 ms at 1024^2), so that
     // kind and the others keep the qh path.
     const bool fused = !f32 && c->opts.mode == 0 && c->ndim == 2 && c->N == 3 && !c->thermal &&
-                       material_kind(c->run_mat) == MK_GAS_NEWTONIAN;
+                       c->opts.flux == 0 && material_kind(c->run_mat) == MK_GAS_NEWTONIAN;
     if (fused) {
       const long nps = (long)c->N * (c->ndim == 2 ? c->N : 1);
       pa.fs = (double *)buf(c, "fs", ne * 2 * c->ndim * nps * c->V * sizeof(double), rc);
@@ -1445,6 +1526,7 @@ int dev_steps_impl(gpr_ctx *c, int64_t nsteps, const double *dts) {
     if (rc) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[3], c->stream));
     FvArgs fa = fv_args(c, c->run_mat, 0, dX, nX);
+    fa.flux = c->opts.flux;
     fa.st = c->d_st;
     fa.u = c->run_u;
     fa.qh = dqh;
@@ -1531,7 +1613,7 @@ int gpr_iterator(gpr_ctx *c, double *u, double tf, const int32_t *nX, const doub
                  const int32_t *bcs, int split, int half_step, int stiff, int flux, const gpr_par *MPs, int nmat,
                  double contorted_tol, double *snapshots, int32_t *nsnap, int64_t *steps) {
   if (!c || !u || !dX || !MPs) return fail(GPR_E_ARG, "null argument");
-  if (flux != 0) return fail(GPR_E_UNSUPPORTED, "only FLUX = 0 (Rusanov) is supported");
+  if (flux < 0 || flux > 2) return fail(GPR_E_ARG, "FLUX must be 0 (Rusanov), 1 (Roe) or 2 (Osher) (fluxes.h:7-9)");
   if (nmat != 1) return fail(GPR_E_UNSUPPORTED, "one material (LSET = 0) is supported");
   int rc = check_nX(c, nX);
   if (rc) return rc;
@@ -1547,12 +1629,14 @@ int gpr_iterator(gpr_ctx *c, double *u, double tf, const int32_t *nX, const doub
   if (stiff) c->opts.mode = 1;
   c->opts.split = split != 0;
   c->opts.half_step = half_step != 0;
+  c->opts.flux = flux;
   if (split) c->opts.precision = 0;
   rc = gpr_dev_begin(c, du, nX, dX, CFL, bcs, 0.0, tf, 0, MPs);
   if (rc) {
     c->opts = saved;
     return rc;
   }
+
   int push = 0;
   int64_t k = 0;
   int status = GPR_OK;
@@ -1919,6 +2003,7 @@ int gpr_batch_set_block(gpr_ctx *c, int block) {
 
 int gpr_batch_steps(gpr_ctx *c, int64_t nsteps) {
   if (!c || !c->batch || !c->batch->ready) return fail(GPR_E_ARG, "gpr_batch_set_state has not been called");
+  if (c->opts.flux != 0) return fail(GPR_E_UNSUPPORTED, "batches take the Rusanov flux (FLUX = 0)");
   for (int64_t k = 0; k < nsteps; k++) {
     int rc = batch_step(c, c->batch);
     if (rc) return rc;
@@ -1949,6 +2034,7 @@ int gpr_batch_status(gpr_ctx *c, double *t, int64_t *count, double *dt, int32_t 
 int gpr_batch_run(gpr_ctx *c, int steps_per_graph, int64_t max_graphs, int64_t *graphs_run, int32_t *running) {
   if (!c || !c->batch || !c->batch->ready) return fail(GPR_E_ARG, "gpr_batch_set_state has not been called");
   if (steps_per_graph < 1 || max_graphs < 1) return fail(GPR_E_ARG, "steps_per_graph and max_graphs must be >= 1");
+  if (c->opts.flux != 0) return fail(GPR_E_UNSUPPORTED, "batches take the Rusanov flux (FLUX = 0)");
   BatchHost *B = c->batch;
   if (B->exec && B->graph_steps != steps_per_graph) batch_drop_graph(B);
   std::vector<BatchState> st(B->count);
@@ -2184,6 +2270,7 @@ int gpr_adj_forward(gpr_ctx *c, int64_t nsteps, int K, double *dts_out) {
   // in the split or FP32-bulk form would be a different map
   if (c->opts.split != 0) return fail(GPR_E_UNSUPPORTED, "the adjoint covers the ADER step, not the split step");
   if (c->opts.precision != 0) return fail(GPR_E_UNSUPPORTED, "the adjoint covers the FP64 step, not the FP32-bulk form");
+  if (c->opts.flux != 0) return fail(GPR_E_UNSUPPORTED, "the adjoint transposes the Rusanov flux only");
   if (c->run_mat.p.BINGHAM || (c->run_mat.p.POWER_LAW && !c->run_mat.p.SOLID))
     return fail(GPR_E_UNSUPPORTED,
                 "the adjoint covers Newtonian fluids and elastic or power-law solids, not Bingham or power-law fluids");

@@ -92,7 +92,8 @@ class _Module:
         self.system = types.SimpleNamespace(
             flux=self.flux, source=self.source, Bdot=self.Bdot,
             max_abs_eigs=self.max_abs_eigs,
-            thermo_acoustic_tensor=self.thermo_acoustic_tensor)
+            thermo_acoustic_tensor=self.thermo_acoustic_tensor,
+            system_matrix=self.system_matrix, dFdP=self.dFdP, dPdQ=self.dPdQ)
         common = types.SimpleNamespace(
             scaled_nodes=lambda: self._consts()["nodes"],
             scaled_weights=lambda: self._consts()["wghts"],
@@ -103,7 +104,8 @@ class _Module:
             coefficient_matrices=self.coefficient_matrices,
             oscillation_indicator=lambda *a: self._consts()["sig"])
         dg = types.SimpleNamespace(predictor=self.predictor)
-        fv = types.SimpleNamespace(Bint=self.Bint, D_RUS=self.D_RUS, fv_launcher=self.fv_launcher,
+        fv = types.SimpleNamespace(Bint=self.Bint, D_RUS=self.D_RUS, D_ROE=self.D_ROE,
+                                   D_OSH=self.D_OSH, fv_launcher=self.fv_launcher,
                                    centers1=self.centers1, centers2=self.centers2,
                                    interfs1=self.interfs1, interfs2=self.interfs2)
         split = types.SimpleNamespace(
@@ -247,6 +249,38 @@ class _Module:
         check(_lib.load().gpr_d_rus(self.ctx().h, dptr(out), dptr(qL), dptr(qR), n, int(d),
                                     par_ref(MP)))
         return out
+
+    def D_ROE(self, qL, qR, d, MP):
+        """|M(qbar)| (qL - qR) + F(qL) + F(qR) (fluxes.cpp:45-64)."""
+        return self._pair(_lib.load().gpr_d_roe, qL, qR, d, MP)
+
+    def D_OSH(self, qL, qR, d, MP):
+        """sum_i W_i |M(q_i)| (qL - qR) + F(qL) + F(qR) (fluxes.cpp:22-43)."""
+        return self._pair(_lib.load().gpr_d_osh, qL, qR, d, MP)
+
+    def _pair(self, fn, qL, qR, d, MP):
+        qL, n = self._states(qL)
+        qR = _size(_in(qR, np.float64), qL.size, "qR")
+        out = np.zeros_like(qL)
+        check(fn(self.ctx().h, dptr(out), dptr(qL), dptr(qR), n, int(d), par_ref(MP)))
+        return out
+
+    def _matrices(self, fn, Q, *args):
+        Q, n = self._states(Q)
+        V = self._V
+        out = np.zeros(n * V * V)
+        check(fn(self.ctx().h, dptr(out), dptr(Q), n, *args))
+        return out.reshape(V, V) if n == 1 else out.reshape(n, V, V)
+
+    def system_matrix(self, Q, d, MP):
+        """dF_d/dP dP/dQ + B_d (equations.cpp:169-176), V x V."""
+        return self._matrices(_lib.load().gpr_system_matrix, Q, int(d), par_ref(MP))
+
+    def dFdP(self, Q, d, MP):
+        return self._matrices(_lib.load().gpr_dfdp, Q, int(d), par_ref(MP))
+
+    def dPdQ(self, Q, MP):
+        return self._matrices(_lib.load().gpr_dpdq, Q, par_ref(MP))
 
     def Bint(self, qL, qR, d, MP):
         qL, n = self._states(qL)

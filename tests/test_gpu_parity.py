@@ -313,9 +313,14 @@ def test_errors(gp):
     bad.EOS = 9
     with pytest.raises(GprError, match="EOS"):
         G.system.max_abs_eigs(u[:14], 0, bad)
-    with pytest.raises(GprError, match="UNSUPPORTED"):
+    # FLUX outside 0 (Rusanov), 1 (Roe), 2 (Osher) (fluxes.h:7-9)
+    with pytest.raises(GprError, match="GPR_E_ARG"):
         G.solvers.fv.fv_launcher(u, np.zeros(100 * 27 * 14), np.array([8, 8], np.int32), 1e-4,
-                                 dX, True, True, 1, MP, np.ones(100, bool))
+                                 dX, True, True, 3, MP, np.ones(100, bool))
+    # more than one material (LSET > 0) is outside this path
+    with pytest.raises(GprError, match="UNSUPPORTED"):
+        G.solvers.iterator(u.copy(), 1e-3, np.array([8, 8], np.int32), dX, 0.8,
+                           np.array(bcs, np.int32), False, False, False, 0, [MP, MP], 1.0)
     with pytest.raises(TypeError):
         G.solvers.weno.weno_launcher(np.zeros(10, np.float32), u, np.array([8, 8], np.int32), MP)
 
