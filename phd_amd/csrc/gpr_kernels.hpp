@@ -255,12 +255,10 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   const long nitems = (nfx + nfy + FPW - 1) / FPW;
   const long M = (nitems + 7) / 8;
   const long b = blockIdx.x;
-  // EIG: block b takes items b, b + grid, ...; otherwise its one banded item
-  const long item0 = EIG ? b : (b % 8) * M + b / 8;
-  const long istep = EIG ? (long)gridDim.x : nitems + 1;
   [[maybe_unused]] const LaneMem ews{EIG ? a.eig_ws + (b * WAVE + lane) : nullptr, (long)gridDim.x * WAVE};
   __shared__ long sOut[FPW];  // each face's output slot: fid in fx, or nfx + fid in fy
-  for (long item = item0; EIG ? item < nitems : item == item0; item += istep) {
+  // one group of FPW faces (item: its index in row order)
+  auto group = [&](long item) {
     const bool item_ok = item < nitems;
     const long id0 = item * FPW;
     const long id = id0 + fl;  // the face in row order
@@ -366,7 +364,15 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
       double *out = slot < nfx ? a.fx + slot * 2 * V : a.fy + (slot - nfx) * 2 * V;
       out[k] = acc;
     }
-    if constexpr (EIG) __syncthreads();  // sR and sOut are rewritten for the next item
+  };
+  if constexpr (EIG) {
+    // block b takes groups b, b + grid, ... (a bounded grid: its workspace)
+    for (long item = b; item < nitems; item += gridDim.x) {
+      group(item);
+      __syncthreads();  // sR and sOut are rewritten for the next group
+    }
+  } else {
+    group((b % 8) * M + b / 8);  // the XCD-banded group of this block
   }
 }
 
