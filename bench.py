@@ -83,10 +83,12 @@ def predictor_flops(picard_iters, fallback_iters, fallback_cells, nodes=27, spat
 MI355X_FP64_TFLOPS = 78.6   # dense FP64 (vector and matrix) peak
 MI355X_HBM_GBS = 8000.0
 
-# HBM bytes per predictor launch on the 512x512 workload, from two
-# rocprofv3 --pmc runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a
-# plain 512 MiB device copy reads in the same runs); profiles/dsl512_pmc_traffic.md
-PREDICTOR_TRAFFIC_512 = 1235.2e6
+# HBM bytes per predictor launch on the 512x512 workload (the fused form:
+# wh read, face states and cell integrals written), from two rocprofv3 --pmc
+# runs over the driver's window, steps 5-24 (FETCH_SIZE x 2.000, WRITE_SIZE
+# x 1.000, the factors a plain 512 MiB device copy reads in the same runs);
+# profiles/dsl512_pmc_traffic.md.  Algorithmic: 1360.9 MB.
+PREDICTOR_TRAFFIC_512 = 1377.5e6
 
 
 def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0, half_step=0, flux=0):
@@ -638,6 +640,8 @@ def main():
         "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / MI355X_FP64_TFLOPS, 4),
         "traffic": PREDICTOR_TRAFFIC_512 if n == 512 else None,
+        "algorithmic_bytes": ((n + 2) ** 2 * 9 * 14 + (n + 2) ** 2 * 36 * 14 + n * n * 14) * 8
+        if args.mode == 0 else None,
         "flops_per_launch": flops_launch, "avg_launch_ms": pred_ms,
         "picard_iters_per_cell": tm["picard_iters"] / max(tm["cells"], 1),
         "fallback_iters_per_cell": tm["fallback_iters"] / max(tm["cells"], 1),
