@@ -555,15 +555,15 @@ __device__ double max_abs_eig4(const double *a, cplx *lam_out = nullptr) {
   return best;
 }
 
-// Thermo-acoustic tensor Xi = Xi1 Xi2 (eig.cpp:8-64) into X (n x n, n = 3 or 4).
-template <bool TH, typename R = double>
-__device__ __forceinline__ int thermo_acoustic_tensor(const R *q, const Mat &mm, int d, R *X) {
+// Thermo-acoustic tensor Xi = Xi1 Xi2 (eig.cpp:8-64) into X (n x n, n = 3 or 4),
+// from the node state s of q (node_state<R, TH, MK>(q, mm, s)).
+template <bool TH, typename R = double, int MK = MK_ANY>
+__device__ __forceinline__ int thermo_acoustic_tensor_of(const R *q, const NodeState<R, TH> &s, const Mat &mm, int d,
+                                                         R *X) {
   const gpr_par &p = mm.p;
-  NodeState<R, TH> s;
-  node_state<R, TH>(q, mm, s);
   const R *A = q + 5;
   R r = s.r, cs2 = s.cs2;
-  R dcs2 = mm.beta_zero ? R(0.0) : p.beta / r * cs2;
+  R dcs2 = mk_flag<MK>(MatKind<MK>::beta_zero, mm.beta_zero) ? R(0.0) : p.beta / r * cs2;
   R sd[3] = {s.sig[0 * 3 + d], s.sig[1 * 3 + d], s.sig[2 * 3 + d]};
   R fac = 1.0 / r + dcs2 / cs2;
   R dsdr[3] = {fac * sd[0], fac * sd[1], fac * sd[2]};
@@ -580,7 +580,8 @@ __device__ __forceinline__ int thermo_acoustic_tensor(const R *q, const Mat &mm,
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) M[i * 3 + j] = -r * cs2 * T9[j * 3 + i];
   // c0^2 (speeds.cpp:8-18)
-  R Gm = Gamma_MG(r, p), dGm = dGamma_MG(r, p), pr = p_ref(r, p), dpr = dp_ref(r, p), der = de_ref(r, p);
+  R Gm = Gamma_MG<R, MK>(r, p), dGm = dGamma_MG<R, MK>(r, p), pr = p_ref<R, MK>(r, p), dpr = dp_ref<R, MK>(r, p),
+    der = de_ref<R, MK>(r, p);
   R dedr = der - (dpr * r * Gm + (Gm + r * dGm) * (s.p - pr)) / ((r * Gm) * (r * Gm));
   R dC0 = (1 - p.beta / Gm) * dcs2 + p.beta / (Gm * Gm) * dGm * cs2;
   R dEdr = dedr + dC0 / 4 * s.devGsq;
@@ -614,11 +615,19 @@ __device__ __forceinline__ int thermo_acoustic_tensor(const R *q, const Mat &mm,
   return n;
 }
 
-// max |characteristic speed| in direction d (eig.cpp:66-77)
 template <bool TH, typename R = double>
-__device__ __forceinline__ R max_abs_eigs(const R *q, const Mat &mm, int d) {
+__device__ __forceinline__ int thermo_acoustic_tensor(const R *q, const Mat &mm, int d, R *X) {
+  NodeState<R, TH> s;
+  node_state<R, TH>(q, mm, s);
+  return thermo_acoustic_tensor_of<TH, R>(q, s, mm, d, X);
+}
+
+// max |characteristic speed| in direction d (eig.cpp:66-77), from the node
+// state s of q
+template <bool TH, typename R = double, int MK = MK_ANY>
+__device__ __forceinline__ R max_abs_eigs_of(const R *q, const NodeState<R, TH> &s, const Mat &mm, int d) {
   R X[16];
-  thermo_acoustic_tensor<TH, R>(q, mm, d, X);
+  thermo_acoustic_tensor_of<TH, R, MK>(q, s, mm, d, X);
   R lam2;
   if constexpr (TH)
     lam2 = max_abs_eig4(X);
@@ -627,6 +636,12 @@ __device__ __forceinline__ R max_abs_eigs(const R *q, const Mat &mm, int d) {
   R lam = sqrt(lam2);
   R vd = q[2 + d] / q[0];
   return vd > 0 ? vd + lam : lam - vd;
+}
+template <bool TH, typename R = double>
+__device__ __forceinline__ R max_abs_eigs(const R *q, const Mat &mm, int d) {
+  NodeState<R, TH> s;
+  node_state<R, TH>(q, mm, s);
+  return max_abs_eigs_of<TH, R>(q, s, mm, d);
 }
 
 }  // namespace gpr

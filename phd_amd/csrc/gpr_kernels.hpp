@@ -228,7 +228,9 @@ inline long eig_face_grid(long items) { return items < EIG_FACE_BLOCKS ? items :
 // FS: the face states come from a.fs (the predictor's fused epilogue)
 // instead of being extrapolated from qh here; the values are the same.
 // EIG: the Riemann solver is a.flux's Roe (1) or Osher (2) instead of Rusanov.
-template <int N, bool TH, int ND, bool TIME = true, bool FS = false, bool EIG = false>
+// MK: the material kind Rusanov's physics is compiled for (MK_ANY: any
+// material); the values do not depend on it.
+template <int N, bool TH, int ND, bool TIME = true, bool FS = false, bool EIG = false, int MK = MK_ANY>
 __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = ND == 1 ? N : N * N;
@@ -338,9 +340,9 @@ __global__ __launch_bounds__(64) void k_fv_faces(FvArgs a) {
         // the direction as a compile-time constant in each branch, so that the
         // physics indexes its small arrays statically (no stack frame)
         if (d == 0)
-          riemann_accum<N, TH>(qL, qR, 0, a.m, K, kk, lo, hi);
+          riemann_accum<N, TH, double, MK>(qL, qR, 0, a.m, K, kk, lo, hi);
         else
-          riemann_accum<N, TH>(qL, qR, 1, a.m, K, kk, lo, hi);
+          riemann_accum<N, TH, double, MK>(qL, qR, 1, a.m, K, kk, lo, hi);
       }
     }
     if (fl < FPW) {
@@ -387,7 +389,9 @@ template <int N, int ND, bool TIME> struct CellShape {
   static constexpr int CPW = WAVE / NN;           // cells per wavefront
 };
 
-template <int N, bool TH, int ND, bool TIME = true, bool SRC = true>
+// MK: the material kind the physics is compiled for (MK_ANY: any material);
+// the values do not depend on it.
+template <int N, bool TH, int ND, bool TIME = true, bool SRC = true, int MK = MK_ANY>
 __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = CellShape<N, ND, TIME>::NS;
@@ -423,7 +427,7 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   __syncthreads();
   double S[V];
   {
-    cell_integrand<N, TH, ND, TIME, SRC>(q, myQ, t, ia, ib, a.m, K, dt, a.dx, a.dy, S);
+    cell_integrand<N, TH, ND, TIME, SRC, MK>(q, myQ, t, ia, ib, a.m, K, dt, a.dx, a.dy, S);
     const bool centers = a.parts & 1;
 #pragma unroll
     for (int v = 0; v < V; v++) S[v] = active && centers ? S[v] : 0.0;

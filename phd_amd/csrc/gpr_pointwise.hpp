@@ -97,25 +97,30 @@ __device__ __forceinline__ void weno_point(const R *d, const Basis &K, int ncell
 // Rusanov flux D_RUS (fluxes.cpp:66-75) and the path integral Bint
 // (fluxes.cpp:9-20) between two states, accumulated as
 //   lo += k (b + f),  hi += k (b - f).
-template <bool TH, typename R = double>
+// Each side's node state serves both its wave-speed bound and its flux.
+template <bool TH, typename R = double, int MK = MK_ANY>
 __device__ __forceinline__ void rusanov(const R *qL, const R *qR, int d, const Mat &m, R *f) {
   constexpr int V = Dims<TH>::V;
-  R sL = max_abs_eigs<TH, R>(qL, m, d), sR = max_abs_eigs<TH, R>(qR, m, d);
-  R smax = (sL < sR) ? sR : sL;  // std::max(max1, max2): a NaN on the right is dropped
-  R F[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) f[v] = smax * (qL[v] - qR[v]);
+  R FL[V], FR[V], sL, sR;
   {
     NodeState<R, TH> st;
-    node_state<R, TH>(qL, m, st);
-    flux<R, TH>(qL, st, m, d, F);
-#pragma unroll
-    for (int v = 0; v < V; v++) f[v] += F[v];
-    node_state<R, TH>(qR, m, st);
-    flux<R, TH>(qR, st, m, d, F);
-#pragma unroll
-    for (int v = 0; v < V; v++) f[v] += F[v];
+    node_state<R, TH, MK>(qL, m, st);
+    sL = max_abs_eigs_of<TH, R, MK>(qL, st, m, d);
+    flux<R, TH>(qL, st, m, d, FL);
   }
+  {
+    NodeState<R, TH> st;
+    node_state<R, TH, MK>(qR, m, st);
+    sR = max_abs_eigs_of<TH, R, MK>(qR, st, m, d);
+    flux<R, TH>(qR, st, m, d, FR);
+  }
+  R smax = (sL < sR) ? sR : sL;  // std::max(max1, max2): a NaN on the right is dropped
+#pragma unroll
+  for (int v = 0; v < V; v++) f[v] = smax * (qL[v] - qR[v]);
+#pragma unroll
+  for (int v = 0; v < V; v++) f[v] += FL[v];
+#pragma unroll
+  for (int v = 0; v < V; v++) f[v] += FR[v];
 }
 
 template <int N, bool TH, typename R = double>
@@ -137,12 +142,12 @@ __device__ __forceinline__ void bint(const R *qL, const R *qR, int d, const Basi
   }
 }
 
-template <int N, bool TH, typename R = double>
+template <int N, bool TH, typename R = double, int MK = MK_ANY>
 __device__ __forceinline__ void riemann_accum(const R *qL, const R *qR, int d, const Mat &m,
                                               const Basis &K, double k, R *lo, R *hi) {
   constexpr int V = Dims<TH>::V;
   R f[V], b[V];
-  rusanov<TH, R>(qL, qR, d, m, f);
+  rusanov<TH, R, MK>(qL, qR, d, m, f);
   bint<N, TH, R>(qL, qR, d, K, b);
 #pragma unroll
   for (int v = 0; v < V; v++) {
