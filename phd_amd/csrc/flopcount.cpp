@@ -336,7 +336,7 @@ template <int MK> struct Counter {
     for (int k = 0; k < 9; k++) f += u[5 + k] * u[5 + k];
     F d13 = cbrt(fabs(det3(u + 5)));
     (void)(f / (d13 * d13) / F(3.0) - F(1.0));
-    F s = max_abs_eigs<false, F>(u, m, 0) * F(512.0) + max_abs_eigs<false, F>(u, m, 1) * F(512.0);
+    F s = cfl_speed<false, 2, F, MK>(u, m, F(1.0 / 512), F(1.0 / 512));
     (void)s;
     return counted(s0);
   }

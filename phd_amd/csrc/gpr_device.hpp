@@ -644,4 +644,16 @@ __device__ __forceinline__ R max_abs_eigs(const R *q, const Mat &mm, int d) {
   return max_abs_eigs_of<TH, R>(q, s, mm, d);
 }
 
+// The CFL speed of a cell (timestep, iterator.cpp:58-72): the sum over the
+// ND directions of max |characteristic speed| / h, from one node state.
+template <bool TH, int ND, typename R = double, int MK = MK_ANY>
+__device__ __forceinline__ R cfl_speed(const R *q, const Mat &mm, R dx, R dy) {
+  NodeState<R, TH> s;
+  node_state<R, TH, MK>(q, mm, s);
+  R sp = 0;
+#pragma unroll
+  for (int d = 0; d < ND; d++) sp += max_abs_eigs_of<TH, R, MK>(q, s, mm, d) / (d == 0 ? dx : dy);
+  return sp;
+}
+
 }  // namespace gpr

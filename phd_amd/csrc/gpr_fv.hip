@@ -15,7 +15,8 @@
 namespace gpr {
 
 // fv_launcher's faces and cells for one (TIME, SOURCES) pair, Rusanov's
-// physics and the cell integrand compiled for material kind MK
+// physics compiled for material kind MK (the cell kernel measured no faster
+// compiled per kind, and stays general)
 template <int N, bool TH, int ND, bool TIME, bool SRC, int MK = MK_ANY> void fv_pair(hipStream_t s, const FvArgs &a) {
   const long nf = (long)(a.nx + 1) * (ND == 2 ? a.ny : 1) + (ND == 2 ? (long)a.nx * (a.ny + 1) : 0);
   constexpr int FPW = FaceShape<N, TH, ND, TIME>::FPW;
@@ -30,7 +31,7 @@ template <int N, bool TH, int ND, bool TIME, bool SRC, int MK = MK_ANY> void fv_
                        s, a);
   const long nc = (long)a.nx * (ND == 2 ? a.ny : 1);
   constexpr int CPW = CellShape<N, ND, TIME>::CPW;
-  hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC, MK>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, s, a);
+  hipLaunchKernelGGL((k_fv_cells<N, TH, ND, TIME, SRC>), dim3((nc + CPW - 1) / CPW), dim3(WAVE), 0, s, a);
 }
 
 // the fused form: faces from the predictor's face states, then the cell update
@@ -50,8 +51,8 @@ template <int N, bool TH, int ND> hipError_t fv_launch(hipStream_t s, const FvAr
     if (!(N == 3 && ND == 2 && !TH) || material_kind(a.m) != MK_GAS_NEWTONIAN) return hipErrorInvalidValue;
     if constexpr (N == 3 && ND == 2 && !TH) fv_fused<N, TH, ND>(s, a);
   } else if (a.time && a.sources) {
-    // the ADER step's corrector: kernels compiled for the material kind on
-    // the 2D P2 single-material workloads without heat conduction
+    // the ADER step's corrector: the face kernel compiled for the material
+    // kind on the 2D P2 single-material workloads without heat conduction
     int mk = MK_ANY;
     if constexpr (N == 3 && ND == 2 && !TH)
       if (a.flux == 0) mk = material_kind(a.m);

@@ -389,9 +389,7 @@ template <int N, int ND, bool TIME> struct CellShape {
   static constexpr int CPW = WAVE / NN;           // cells per wavefront
 };
 
-// MK: the material kind the physics is compiled for (MK_ANY: any material);
-// the values do not depend on it.
-template <int N, bool TH, int ND, bool TIME = true, bool SRC = true, int MK = MK_ANY>
+template <int N, bool TH, int ND, bool TIME = true, bool SRC = true>
 __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   constexpr int V = Dims<TH>::V;
   constexpr int NS = CellShape<N, ND, TIME>::NS;
@@ -427,7 +425,7 @@ __global__ __launch_bounds__(64) void k_fv_cells(FvArgs a) {
   __syncthreads();
   double S[V];
   {
-    cell_integrand<N, TH, ND, TIME, SRC, MK>(q, myQ, t, ia, ib, a.m, K, dt, a.dx, a.dy, S);
+    cell_integrand<N, TH, ND, TIME, SRC>(q, myQ, t, ia, ib, a.m, K, dt, a.dx, a.dy, S);
     const bool centers = a.parts & 1;
 #pragma unroll
     for (int v = 0; v < V; v++) S[v] = active && centers ? S[v] : 0.0;
@@ -510,9 +508,7 @@ __global__ __launch_bounds__(256) void k_post(double *u, long ncell, double dx, 
 #pragma unroll
     for (int v = 0; v < V; v++)
       if (!(q[v] - q[v] == 0.0)) nonfinite = 1;
-    double sp = 0;
-#pragma unroll
-    for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
+    const double sp = cfl_speed<TH, ND>(q, m, dx, dy);
     if (sp > local_max) local_max = sp;
     if (ODE) {
 #pragma unroll
@@ -541,9 +537,7 @@ __global__ __launch_bounds__(256) void k_cfl(const double *u, long ncell, double
     double q[V];
 #pragma unroll
     for (int v = 0; v < V; v++) q[v] = u[c * V + v];
-    double sp = 0;
-#pragma unroll
-    for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
+    const double sp = cfl_speed<TH, ND>(q, m, dx, dy);
     if (sp > local_max) local_max = sp;
   }
   double mx = wave_max(local_max);
@@ -575,9 +569,7 @@ __global__ __launch_bounds__(256) void k_reset(double *u, long ncell, double dx,
     q[13] = cf;
 #pragma unroll
     for (int v = 0; v < V; v++) u[c * V + v] = q[v];
-    double sp = 0;
-#pragma unroll
-    for (int d = 0; d < ND; d++) sp += max_abs_eigs<TH>(q, m, d) / (d == 0 ? dx : dy);
+    const double sp = cfl_speed<TH, ND>(q, m, dx, dy);
     if (sp > local_max) local_max = sp;
   }
   double mx = wave_max(local_max);
