@@ -180,6 +180,52 @@ __global__ void k_weno_y(const R *__restrict__ ux, R *__restrict__ wh, int nx, i
   }
 }
 
+// 2D x- and y-passes in one launch: block (row i, tile of WENO_JT output
+// columns) reconstructs the x-pass of the columns its y-pass stencils reach
+// into LDS (the values k_weno_x writes to ux), then the y-pass from LDS, so
+// the x-pass output never goes through HBM.  Rows are dealt to the XCDs in
+// contiguous bands (block b on XCD b % 8 takes item (b % 8) M + b / 8), so
+// that the five rows of ub a row's x-pass reads are shared in one L2.
+constexpr int WENO_JT = 64;
+template <typename R, int N, int V>
+__global__ __launch_bounds__(256) void k_weno_xy(const double *__restrict__ ub, R *__restrict__ wh, int nx, int ny,
+                                                 Basis K) {
+  constexpr int NC = WENO_JT + 2 * N - 2;  // x-pass columns a tile's y-pass reads
+  __shared__ R sx[NC * N * V];
+  const int NX = nx + 2, NY = ny + 2, ncol = ny + 2 * N;
+  const int ntile = (NY + WENO_JT - 1) / WENO_JT;
+  const long nitems = (long)NX * ntile;
+  const long M = (nitems + 7) / 8;
+  const long item = (blockIdx.x % 8) * M + blockIdx.x / 8;
+  if (item >= nitems) return;  // uniform over the block
+  const int i = item / ntile, j0 = (item - (long)i * ntile) * WENO_JT;
+  // x-pass of columns j0 .. j0 + NC - 1 of row i (weno.cpp:93-104)
+  for (int k = threadIdx.x; k < NC * V; k += blockDim.x) {
+    const int jj = k / V, v = k - jj * V, j = j0 + jj;
+    if (j >= ncol) continue;
+    const int gap = (j < N || j >= ncol - N) ? N : 0;
+    R d[2 * N - 1], o[N];
+#pragma unroll
+    for (int r = 0; r < 2 * N - 1; r++) d[r] = R(ub[((long)(i + r) * ncol + j) * V + v]);
+    weno_point<R, N>(d, K, NX, gap, i, o);
+#pragma unroll
+    for (int a = 0; a < N; a++) sx[(jj * N + a) * V + v] = o[a];
+  }
+  __syncthreads();
+  // y-pass of output columns j0 .. j0 + WENO_JT - 1 (weno.cpp:109-121)
+  const int gap = (i == 0 || i == NX - 1) ? 1 : 0;
+  for (int k = threadIdx.x; k < WENO_JT * N * V; k += blockDim.x) {
+    const int v = k % V, a = (k / V) % N, jj = k / (V * N), j = j0 + jj;
+    if (j >= NY) continue;
+    R d[2 * N - 1], o[N];
+#pragma unroll
+    for (int r = 0; r < 2 * N - 1; r++) d[r] = sx[((jj + r) * N + a) * V + v];
+    weno_point<R, N>(d, K, NY, gap, j, o);
+#pragma unroll
+    for (int b = 0; b < N; b++) wh[((((long)i * NY + j) * N + a) * N + b) * V + v] = o[b];
+  }
+}
+
 struct FvArgs {
   double *u;             // [cells][V]
   const double *qh;      // [(nx+2)(ny+2)][NN][V]

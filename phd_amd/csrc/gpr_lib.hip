@@ -316,14 +316,21 @@ template <int N, bool TH, int ND> struct BoundariesOp {
 
 // WENO in the type R (double: the plain method; float: the FP32-bulk form),
 // ub read in double, wh written in R.
+// 2D: the fused x/y launch (k_weno_xy), unless the x-pass output is wanted:
+// in ux_given, or (keep_ux) in the context's "ux" buffer, which the adjoint's
+// reverse sweep reads; the values are the same.
 template <typename R, int N, bool TH, int ND>
-int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny, R *ux_given = nullptr) {
+int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny, R *ux_given = nullptr, bool keep_ux = false) {
   constexpr int V = Dims<TH>::V;
   const Basis &K = basis(N);
   if (ND == 1) {
     long total = (long)(nx + 2) * V;
     hipLaunchKernelGGL((k_weno_1d<R, N, V>), dim3(grid_for(total, 256)), dim3(256), 0, c->stream, d_ub, d_wh, nx,
                        K);
+  } else if (!ux_given && !keep_ux) {
+    const long items = (long)(nx + 2) * ((ny + 2 + WENO_JT - 1) / WENO_JT);
+    hipLaunchKernelGGL((k_weno_xy<R, N, V>), dim3(8 * ((items + 7) / 8)), dim3(256), 0, c->stream, d_ub, d_wh, nx,
+                       ny, K);
   } else {
     int rc;
     long nux = (long)(nx + 2) * (ny + 2 * N) * N * V;
@@ -339,8 +346,9 @@ int weno_run(gpr_ctx *c, const double *d_ub, R *d_wh, int nx, int ny, R *ux_give
 }
 
 template <int N, bool TH, int ND> struct WenoOp {
-  static int run(gpr_ctx *c, const double *d_ub, double *d_wh, int nx, int ny, double *ux = nullptr) {
-    return weno_run<double, N, TH, ND>(c, d_ub, d_wh, nx, ny, ux);
+  static int run(gpr_ctx *c, const double *d_ub, double *d_wh, int nx, int ny, double *ux = nullptr,
+                 bool keep_ux = false) {
+    return weno_run<double, N, TH, ND>(c, d_ub, d_wh, nx, ny, ux, keep_ux);
   }
 };
 
@@ -2160,7 +2168,8 @@ int adj_stages(gpr_ctx *c, const double *u_n, double *u_fv, double dt, const dou
     return fail(GPR_E_ARG, "the device run has no ADER stage buffers (gpr_dev_begin in the ADER FP64 form is needed)");
   rc = dispatch<BoundariesOp>(c, c, u_n, dub, c->run_nx, c->run_ny, c->run_bc);
   if (rc) return rc;
-  rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny);
+  // the reverse sweep reads the x-pass output (ux)
+  rc = dispatch<WenoOp>(c, c, (const double *)dub, dwh, c->run_nx, c->run_ny, nullptr, true);
   if (rc) return rc;
   if (!qh_in) {
     PredArgs pa = pred_args(c, c->run_mat, dt, dX);
