@@ -573,21 +573,21 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         for (int v = 0; v < V; v++) myQ[node * V + v] = q0[v];
       wave_sync();
       if (active) {
-        // x-sides from the lanes at a = 0 and a = N-1, y-sides from b = 0 and b = N-1
-#pragma unroll
-        for (int side = 0; side < FS::SIDES; side++) {
-          const bool mine = side < 2 ? ia == (side == 0 ? 0 : N - 1) : ib == (side == 2 ? 0 : N - 1);
-          if (!mine) continue;
+        // the cell's SIDES x NPS face-node states (side, time node tt, face
+        // node fn), dealt over its lanes: the extrapolation to side e of the
+        // N nodes along the side's normal
+        for (int job = node; job < FS::SIDES * FS::NPS; job += NN) {
+          const int side = job / FS::NPS, rem = job - side * FS::NPS;
+          const int tt = rem / FS::NSF, fn = rem - tt * FS::NSF;
           const int e = side & 1;
-          const int fn = ND == 1 ? 0 : (side < 2 ? ib : ia);
-          double *dst = a.fs + ((cell * FS::SIDES + side) * FS::NPS + t * FS::NSF + fn) * V;
+          double *dst = a.fs + ((cell * FS::SIDES + side) * FS::NPS + rem) * V;
 #pragma unroll
           for (int v = 0; v < V; v++) {
             double acc = 0;
 #pragma unroll
             for (int k = 0; k < N; k++) {
-              const int nl = ND == 1 ? k : (side < 2 ? k * N + ib : ia * N + k);
-              acc += K.endv[e][k] * (double)myQ[(t * NS + nl) * V + v];
+              const int nl = ND == 1 ? k : (side < 2 ? k * N + fn : fn * N + k);
+              acc += K.endv[e][k] * (double)myQ[(tt * NS + nl) * V + v];
             }
             dst[v] = acc;
           }
