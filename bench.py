@@ -88,7 +88,7 @@ MI355X_HBM_GBS = 8000.0
 # runs over the driver's window, steps 5-24 (FETCH_SIZE x 2.000, WRITE_SIZE
 # x 1.000, the factors a plain 512 MiB device copy reads in the same runs);
 # profiles/dsl512_pmc_traffic.md.  Algorithmic: 1360.9 MB.
-PREDICTOR_TRAFFIC_512 = 1377.5e6
+PREDICTOR_TRAFFIC_512 = 1377.4e6
 
 
 def cpu_reference(u_start, MP, nX, dX, bcs, t0, count0, nsteps, threads, split=0, half_step=0, flux=0):
