@@ -108,6 +108,26 @@ def test_weno_2d(gp, N):
         assert relerr(wh, O.weno_launcher(data, nX)) < TOL
 
 
+@pytest.mark.parametrize("N", [2, 3])
+@pytest.mark.parametrize("thermal", [False, True])
+def test_weno_2d_row_tiles(gp, N, thermal):
+    """The 2D reconstruction runs its x- and y-passes in one launch over
+    tiles of 64 output columns per row: grids whose ny + 2 columns end in a
+    partial tile, one full tile, and one column past a tile, not square."""
+    from phd_amd import problems
+    G = gp(N, thermal)
+    O = oracle(N, thermal, 2)
+    V = 17 if thermal else 14
+    MP = problems.grf_density_2d(8, thermal=thermal)[1]
+    rng = np.random.default_rng(7)
+    for nx, ny in ((7, 130), (130, 63), (3, 62), (5, 1)):
+        nX = np.array([nx, ny], np.int32)
+        ub = 1.0 + 0.1 * rng.standard_normal((nx + 2 * N) * (ny + 2 * N) * V)
+        wh = np.zeros((nx + 2) * (ny + 2) * N * N * V)
+        G.solvers.weno.weno_launcher(wh, ub, nX, MP)
+        assert relerr(wh, O.weno_launcher(ub, nX)) < TOL, (nx, ny)
+
+
 @pytest.mark.parametrize("thermal", [False, True])
 def test_weno_1d(gp, thermal):
     from phd_amd import problems
