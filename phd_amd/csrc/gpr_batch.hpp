@@ -89,7 +89,9 @@ __global__ void kb_weno(BatchArgs b) {
 }
 
 // interfs1 (fv.cpp:111-160): one lane per (face, time node), as k_fv_faces
-template <int N, bool TH>
+// MK: the material kind Rusanov's physics is compiled for (every instance
+// of the batch of that kind, or MK_ANY); the values do not depend on it.
+template <int N, bool TH, int MK = MK_ANY>
 __global__ __launch_bounds__(64) void kb_faces(BatchArgs b) {
   constexpr int V = Dims<TH>::V;
   constexpr int NT = N, FPW = WAVE / NT, NN = N * N;
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(64) void kb_faces(BatchArgs b) {
       qL[v] = l;
       qR[v] = rr;
     }
-    riemann_accum<N, TH>(qL, qR, 0, I.m, K, kk, lo, hi);
+    riemann_accum<N, TH, double, MK>(qL, qR, 0, I.m, K, kk, lo, hi);
   }
   if (fl < FPW) {
     double *my = sR + (fl * NT + t) * 2 * V;

@@ -1724,6 +1724,7 @@ struct BatchHost {
   bool ready = false;
   int view_count = 0;  // instances the launches cover
   long vnu = 0, vnb = 0, vne = 0;  // their cells, ghost-extended rows, extended cells
+  int mk = MK_ANY;  // the material kind every instance has (MK_ANY if they differ)
 };
 
 namespace {
@@ -1742,9 +1743,22 @@ template <int N, bool TH> struct BatchStep {
     pa.binst = b.inst;
     pa.pblk = (const int *)B->allocs[3];
     pa.bst = b.st;
-    hipLaunchKernelGGL((k_predictor<N, TH, 1, double, double, true>), dim3(B->nblk_p), dim3(WAVE), 0, c->stream,
-                       pa);
-    hipLaunchKernelGGL((kb_faces<N, TH>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
+    // P2 batches of one material kind without heat conduction run the
+    // predictor and the faces compiled for that kind (the same values)
+    bool kinded = false;
+    if constexpr (N == 3 && !TH) {
+      if (B->mk == MK_GAS_NEWTONIAN) {
+        hipLaunchKernelGGL((k_predictor<N, TH, 1, double, double, true, KIND_BOTH, MK_GAS_NEWTONIAN>),
+                           dim3(B->nblk_p), dim3(WAVE), 0, c->stream, pa);
+        hipLaunchKernelGGL((kb_faces<N, TH, MK_GAS_NEWTONIAN>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
+        kinded = true;
+      }
+    }
+    if (!kinded) {
+      hipLaunchKernelGGL((k_predictor<N, TH, 1, double, double, true>), dim3(B->nblk_p), dim3(WAVE), 0, c->stream,
+                         pa);
+      hipLaunchKernelGGL((kb_faces<N, TH>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
+    }
     hipLaunchKernelGGL((kb_cells<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
     hipLaunchKernelGGL((kb_post<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
     hipLaunchKernelGGL(kb_advance, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
@@ -1918,6 +1932,9 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   B->nb = B->b0s[count];
   B->ne = B->e0s[count];
   B->nf = f0s[count];
+  B->mk = material_kind(B->inst[0].m);
+  for (int k = 1; k < count; k++)
+    if (material_kind(B->inst[k].m) != B->mk) B->mk = MK_ANY;
   B->nblk_p = (int)pblk.size();
   B->nblk_c = (int)cblk.size();
   B->nblk_f = (int)fblk.size();
