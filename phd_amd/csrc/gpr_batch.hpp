@@ -24,6 +24,10 @@ struct BatchArgs {
   double *wh;                   // [sum (nx + 2)][N][V]
   double *qh;                   // [sum (nx + 2)][N^2][V]
   double *fx;                   // [sum (nx + 1)][2][V]
+  // fused corrector (the predictor's epilogue): face states per extended
+  // cell (FaceStates<N, 1> layout, in qh's space) and cell integrals [sum nx][V]
+  const double *fs;
+  const double *cint;
   // the view: the instances the element-wise launches cover (all, or the
   // running ones), vinst[j] for j < vcount, with their cumulative cell,
   // ghost-extended row and extended cell counts (vcount + 1 entries each)
@@ -91,7 +95,9 @@ __global__ void kb_weno(BatchArgs b) {
 // interfs1 (fv.cpp:111-160): one lane per (face, time node), as k_fv_faces
 // MK: the material kind Rusanov's physics is compiled for (every instance
 // of the batch of that kind, or MK_ANY); the values do not depend on it.
-template <int N, bool TH, int MK = MK_ANY>
+// FS: the face states come from b.fs (the predictor's fused epilogue)
+// instead of being extrapolated from qh here; the values are the same.
+template <int N, bool TH, int MK = MK_ANY, bool FS = false>
 __global__ __launch_bounds__(64) void kb_faces(BatchArgs b) {
   constexpr int V = Dims<TH>::V;
   constexpr int NT = N, FPW = WAVE / NT, NN = N * N;
@@ -116,18 +122,29 @@ __global__ __launch_bounds__(64) void kb_faces(BatchArgs b) {
     const long e0 = I.e0 + lf, e1 = e0 + 1;
     const double kk = dt * K.w[t] / (2.0 * I.dx);
     double qL[V], qR[V];
-    const double *pl = b.qh + (e0 * NN + t * N) * V;
-    const double *pr = b.qh + (e1 * NN + t * N) * V;
+    if constexpr (FS) {
+      using F = FaceStates<N, 1>;  // e0's right side (1), e1's left side (0)
+      const double *pl = b.fs + ((e0 * F::SIDES + 1) * F::NPS + t) * V;
+      const double *pr = b.fs + ((e1 * F::SIDES + 0) * F::NPS + t) * V;
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      double l = 0, rr = 0;
-#pragma unroll
-      for (int a = 0; a < N; a++) {
-        l += K.endv[1][a] * pl[a * V + v];
-        rr += K.endv[0][a] * pr[a * V + v];
+      for (int v = 0; v < V; v++) {
+        qL[v] = pl[v];
+        qR[v] = pr[v];
       }
-      qL[v] = l;
-      qR[v] = rr;
+    } else {
+      const double *pl = b.qh + (e0 * NN + t * N) * V;
+      const double *pr = b.qh + (e1 * NN + t * N) * V;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double l = 0, rr = 0;
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+          l += K.endv[1][a] * pl[a * V + v];
+          rr += K.endv[0][a] * pr[a * V + v];
+        }
+        qL[v] = l;
+        qR[v] = rr;
+      }
     }
     riemann_accum<N, TH, double, MK>(qL, qR, 0, I.m, K, kk, lo, hi);
   }
@@ -205,6 +222,27 @@ __global__ __launch_bounds__(64) void kb_cells(BatchArgs b) {
   for (int v = node; valid && v < V; v += NN) {
     double du = 0;
     for (int j = 0; j < NN; j++) du += myQ[j * V + v];
+    const double *fl = b.fx + (I.f0 + lc) * 2 * V;      // left face: hi side
+    const double *fr = b.fx + (I.f0 + lc + 1) * 2 * V;  // right face: lo side
+    du -= fl[V + v] + fr[v];
+    b.u[(I.u0 + lc) * V + v] += du;
+  }
+}
+
+// The fused form's cell update: u += cint - the two face terms, per (cell,
+// variable) of the CPW cells of a kb_cells block, as kb_cells ends.
+template <int N, bool TH>
+__global__ __launch_bounds__(64) void kb_update(BatchArgs b) {
+  constexpr int V = Dims<TH>::V;
+  constexpr int CPW = WAVE / (N * N);
+  const int k = b.cblk[blockIdx.x];
+  const BatchInst &I = b.inst[k];
+  if (b.st[k].done) return;  // uniform over the block
+  for (int e = threadIdx.x; e < CPW * V; e += blockDim.x) {
+    const int slot = e / V, v = e - slot * V;
+    const long lc = (long)(blockIdx.x - I.blk_c) * CPW + slot;
+    if (lc >= I.nx) continue;
+    double du = b.cint[(I.u0 + lc) * V + v];
     const double *fl = b.fx + (I.f0 + lc) * 2 * V;      // left face: hi side
     const double *fr = b.fx + (I.f0 + lc + 1) * 2 * V;  // right face: lo side
     du -= fl[V + v] + fr[v];

@@ -1745,12 +1745,20 @@ template <int N, bool TH> struct BatchStep {
     pa.bst = b.st;
     // P2 batches of one material kind without heat conduction run the
     // predictor and the faces compiled for that kind (the same values)
+    // the ideal-gas Newtonian kind also fuses the corrector into the
+    // predictor's epilogue: face states in qh's space and cell integrals,
+    // read by the faces and a light cell update (the same values)
     bool kinded = false;
     if constexpr (N == 3 && !TH) {
       if (B->mk == MK_GAS_NEWTONIAN) {
+        pa.qh = nullptr;
+        pa.fs = (double *)b.fs;
+        pa.cint = (double *)b.cint;
         hipLaunchKernelGGL((k_predictor<N, TH, 1, double, double, true, KIND_BOTH, MK_GAS_NEWTONIAN>),
                            dim3(B->nblk_p), dim3(WAVE), 0, c->stream, pa);
-        hipLaunchKernelGGL((kb_faces<N, TH, MK_GAS_NEWTONIAN>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
+        hipLaunchKernelGGL((kb_faces<N, TH, MK_GAS_NEWTONIAN, true>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream,
+                           b);
+        hipLaunchKernelGGL((kb_update<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
         kinded = true;
       }
     }
@@ -1758,8 +1766,8 @@ template <int N, bool TH> struct BatchStep {
       hipLaunchKernelGGL((k_predictor<N, TH, 1, double, double, true>), dim3(B->nblk_p), dim3(WAVE), 0, c->stream,
                          pa);
       hipLaunchKernelGGL((kb_faces<N, TH>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
+      hipLaunchKernelGGL((kb_cells<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
     }
-    hipLaunchKernelGGL((kb_cells<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
     hipLaunchKernelGGL((kb_post<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
     hipLaunchKernelGGL(kb_advance, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
     hipLaunchKernelGGL((kb_reset<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b);
@@ -1939,9 +1947,11 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   B->nblk_c = (int)cblk.size();
   B->nblk_f = (int)fblk.size();
   B->view_count = count;
-  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 ub, 8 wh, 9 qh,
-  // 10 fx, 11 view instances, 12 view offsets (3 x (count + 1))
-  const size_t sizes[13] = {sizeof(BatchInst) * count,
+  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 ub, 8 wh, 9 qh
+  // (or the fused form's face states: 2N of its N^2 nodes per cell), 10 fx, 11 view instances, 12 view
+  // offsets (3 x (count + 1)), 13 the fused form's cell integrals
+  const bool fused = N == 3 && !c->thermal && B->mk == MK_GAS_NEWTONIAN;
+  const size_t sizes[14] = {sizeof(BatchInst) * count,
                             sizeof(BatchState) * count,
                             sizeof(long) * 3 * (count + 1),
                             sizeof(int) * pblk.size(),
@@ -1953,10 +1963,11 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
                             sizeof(double) * B->ne * NN * V,
                             sizeof(double) * B->nf * 2 * V,
                             sizeof(int) * count,
-                            sizeof(long) * 3 * (count + 1)};
-  B->allocs.assign(13, nullptr);
+                            sizeof(long) * 3 * (count + 1),
+                            fused ? sizeof(double) * B->nu * V : 0};
+  B->allocs.assign(14, nullptr);
   c->batch = B;
-  for (int i = 0; i < 13; i++) HIP_TRY(hipMalloc(&B->allocs[i], std::max<size_t>(sizes[i], 256)));
+  for (int i = 0; i < 14; i++) HIP_TRY(hipMalloc(&B->allocs[i], std::max<size_t>(sizes[i], 256)));
   std::vector<long> offs;
   offs.insert(offs.end(), B->u0s.begin(), B->u0s.end());
   offs.insert(offs.end(), B->b0s.begin(), B->b0s.end());
@@ -1990,6 +2001,8 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   a.wh = (double *)B->allocs[8];
   a.qh = (double *)B->allocs[9];
   a.fx = (double *)B->allocs[10];
+  a.fs = a.qh;
+  a.cint = (const double *)B->allocs[13];
   return batch_build_view(c, B, std::vector<char>(count, 1));
 }
 

@@ -74,11 +74,12 @@ struct PredArgs {
   long ncell;
   int dg_it, stiff_it, bailout, mode;  // mode 1: fallback only; 0 or 2: Picard then fallback
   int bail_from;           // > 0: after this many Picard iterations, one growth leaves Picard
-  // fused FV corrector (the one-launch FP64 kernel of the 2D P2 ideal-gas
-  // Newtonian kind, the only one compiled with it): if fs is set, the launch
-  // writes each active cell's face states (FaceStates layout) and each real
-  // cell's FV cell integral (sources, TIME) to cint [fnx][fny][V]; qh may
-  // then be null
+  // fused FV corrector (the one-launch FP64 kernels of the P2 ideal-gas
+  // Newtonian kind, 2D and batched, the only ones compiled with it): if fs
+  // is set, the launch writes each active cell's face states (FaceStates
+  // layout) and each real cell's FV cell integral (sources, TIME) to cint
+  // ([fnx][fny][V]; batched: the instances' cells concatenated); qh may then
+  // be null
   double *fs;
   double *cint;
   int fnx, fny;
@@ -215,6 +216,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
   long cell;
   bool valid;
   double dt, dxv;
+  [[maybe_unused]] long real_cell = -1;  // batched: the cell's index in u if it is a real cell
   // the material: a reference straight into the kernel argument unless
   // batched (a pointer chosen on a branch would make the compiler copy the
   // whole argument to the stack)
@@ -232,6 +234,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     valid = lane_ok && lc < I.nx + 2 && !a.bst[k].done;
     dt = a.bst[k].dt;
     dxv = I.dx;
+    if (lc >= 1 && lc <= I.nx) real_cell = I.u0 + lc - 1;
   } else if constexpr (KIND == KIND_FALLBACK) {
     const long g0 = (long)blockIdx.x * CPW;
     const long cnt = *a.list_count;
@@ -563,9 +566,11 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
     wave_sync();  // sQ, sF and sRat are rewritten next iteration
   }
   // Fused FV corrector: the final iterate of every node (q0 holds it) gives
-  // the cell's face states and its cell integral (what k_fv_faces extrapolates
-  // and k_fv_cells integrates from qh), so qh need not go through HBM.
-  if constexpr (KIND == KIND_BOTH && !BATCH && sizeof(R) == sizeof(double) && MK == MK_GAS_NEWTONIAN) {
+  // the cell's face states and its cell integral (what k_fv_faces / kb_faces
+  // extrapolate and k_fv_cells / kb_cells integrate from qh), so qh need not
+  // go through HBM.  Batched: cell indexes the instances' concatenated
+  // extended cells and real_cell the concatenated cells.
+  if constexpr (KIND == KIND_BOTH && sizeof(R) == sizeof(double) && MK == MK_GAS_NEWTONIAN) {
     if (a.fs) {  // uniform
       using FS = FaceStates<N, ND>;
       if (lane_ok)
@@ -596,7 +601,8 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
       // the cell integral of the real cells (extended cells minus the ghost ring)
       const long ey = ND == 2 ? a.fny + 2 : 1;
       const long ci = ND == 2 ? cell / ey - 1 : cell - 1, cj = ND == 2 ? cell % ey - 1 : 0;
-      const bool real = valid && ci >= 0 && ci < a.fnx && cj >= 0 && cj < (ND == 2 ? a.fny : 1);
+      const bool real = BATCH ? valid && real_cell >= 0
+                              : valid && ci >= 0 && ci < a.fnx && cj >= 0 && cj < (ND == 2 ? a.fny : 1);
       double S[V];
       cell_integrand<N, TH, ND, true, true, MK>((const double *)q0, (const double *)myQ, t, ia, ib, M, K, dt, dxv,
                                                 a.dy, S);
@@ -606,7 +612,7 @@ __global__ __launch_bounds__(64, KIND == KIND_PICARD && sizeof(R) == sizeof(floa
         for (int v = 0; v < V; v++) myF[node * V + v] = active ? S[v] : 0.0;
       wave_sync();
       if (real) {
-        const long c = ND == 2 ? ci * a.fny + cj : ci;
+        const long c = BATCH ? real_cell : ND == 2 ? ci * a.fny + cj : ci;
         for (int v = node; v < V; v += NN) {
           double du = 0;
           for (int k = 0; k < NN; k++) du += (double)myF[k * V + v];

@@ -46,6 +46,40 @@ def test_batch_matches_each_instance_alone():
     print(f"worst instance difference from its single run: {worst:.2e}")
 
 
+def test_mixed_kind_batch_matches_each_instance_alone():
+    """A batch whose instances are not all of one material kind takes the
+    general kernels (the uniform ideal-gas Newtonian batches above take the
+    kernels compiled for that kind, with the corrector fused into the
+    predictor): one tube here is a fluid whose shear sound speed depends on
+    the density (beta = 0.5).  Each instance still ends as its single run."""
+    from phd_amd import problems
+    from phd_amd.batch import Batch
+    from phd_amd.runner import DeviceRun
+    from phd_amd.params import material_params
+    inst = list(small_batch(seed=11, count=3))
+    nx = 40
+    MP = material_params("sg", 1.0, cv=1.0, γ=1.4, b0=1.0, μ=1e-3)
+    MP.beta = 0.5
+    x = (np.arange(nx) + 0.5) / nx
+    ρ = np.where(x < 0.5, 1.0, 0.125)
+    p = np.where(x < 0.5, 1.0, 0.1)
+    A = (ρ / MP.rho0)[:, None, None] ** (1 / 3) * np.eye(3)
+    u = problems.cvec_field(ρ, p, np.zeros((nx, 3)), MP, A)
+    inst.insert(1, (u, MP, 0.1, [1.0 / nx], [0, 0]))
+    b = Batch(inst, bailout=0, bail_from=6)
+    b.run(16)
+    st = b.status()
+    assert st["done"].all() and not st["nonfinite"].any() and st["pred_fail"] == 0
+    got = b.states()
+    for k, (u0, MP_k, tf, dX, bcs) in enumerate(inst):
+        r = DeviceRun(np.asarray(u0).ravel(), MP_k, [u0.shape[0]], dX, bcs, CFL=0.8, tf=tf,
+                      bailout=0, bail_from=6)
+        while r.status()["t"] < tf:
+            r.steps(1)
+        assert r.status()["count"] == st["count"][k], k
+        assert relerr(got[k], r.host().reshape(-1, 14)) < 1e-12, k
+
+
 def test_batch_against_oracle_steps():
     """Per step: each instance of the batch and the oracle advance the same
     state by one step.  The states are the oracle's own trajectory; the batch
