@@ -229,27 +229,6 @@ __global__ __launch_bounds__(64) void kb_cells(BatchArgs b) {
   }
 }
 
-// The fused form's cell update: u += cint - the two face terms, per (cell,
-// variable) of the CPW cells of a kb_cells block, as kb_cells ends.
-template <int N, bool TH>
-__global__ __launch_bounds__(64) void kb_update(BatchArgs b) {
-  constexpr int V = Dims<TH>::V;
-  constexpr int CPW = WAVE / (N * N);
-  const int k = b.cblk[blockIdx.x];
-  const BatchInst &I = b.inst[k];
-  if (b.st[k].done) return;  // uniform over the block
-  for (int e = threadIdx.x; e < CPW * V; e += blockDim.x) {
-    const int slot = e / V, v = e - slot * V;
-    const long lc = (long)(blockIdx.x - I.blk_c) * CPW + slot;
-    if (lc >= I.nx) continue;
-    double du = b.cint[(I.u0 + lc) * V + v];
-    const double *fl = b.fx + (I.f0 + lc) * 2 * V;      // left face: hi side
-    const double *fr = b.fx + (I.f0 + lc + 1) * 2 * V;  // right face: lo side
-    du -= fl[V + v] + fr[v];
-    b.u[(I.u0 + lc) * V + v] += du;
-  }
-}
-
 // CFL maximum of one value per lane into its instance: one atomic per
 // wavefront when the wavefront's cells share an instance.  sp is never NaN
 // (callers keep a running maximum that NaN cannot enter, as std::max).
@@ -266,7 +245,10 @@ __device__ __forceinline__ void batch_max(BatchState *st, int k, bool work, doub
 // Post-step work, one thread per cell, as k_post: renormalise A, the
 // instance's contortion and non-finite flags, and its CFL maximum for the
 // next dt (cfl_only: the maximum alone, for the first step).
-template <bool TH>
+// UPD: first the fused corrector's cell update, u += cint - the two face
+// terms (the operations with which kb_cells ends), on the state the
+// post-step then works on; the whole updated state is written back.
+template <bool TH, bool UPD = false>
 __global__ void kb_post(BatchArgs b, int cfl_only) {
   constexpr int V = Dims<TH>::V;
   const long total = b.vu0s[b.vcount];
@@ -283,6 +265,17 @@ __global__ void kb_post(BatchArgs b, int cfl_only) {
     double q[V];
 #pragma unroll
     for (int v = 0; v < V; v++) q[v] = b.u[c * V + v];
+    if constexpr (UPD) {
+      const long lc = c - I.u0;
+      const double *fl = b.fx + (I.f0 + lc) * 2 * V;      // left face: hi side
+      const double *fr = b.fx + (I.f0 + lc + 1) * 2 * V;  // right face: lo side
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double du = b.cint[c * V + v];
+        du -= fl[V + v] + fr[v];
+        q[v] += du;
+      }
+    }
     if (!cfl_only) {
       if (q[0] > 0) {
         double cf = cbrt(q[0] / (I.m.p.rho0 * det3(q + 5)));
@@ -299,6 +292,12 @@ __global__ void kb_post(BatchArgs b, int cfl_only) {
       for (int v = 0; v < V; v++)
         if (!(q[v] - q[v] == 0.0)) nonfinite = true;
       if (nonfinite) atomicOr(&S.nonfinite, 1);
+      if constexpr (UPD) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) b.u[c * V + v] = q[v];
+#pragma unroll
+        for (int v = 14; v < V; v++) b.u[c * V + v] = q[v];
+      }
 #pragma unroll
       for (int j = 0; j < 9; j++) b.u[c * V + 5 + j] = q[5 + j];
     }

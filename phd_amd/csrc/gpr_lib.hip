@@ -1747,7 +1747,7 @@ template <int N, bool TH> struct BatchStep {
     // predictor and the faces compiled for that kind (the same values)
     // the ideal-gas Newtonian kind also fuses the corrector into the
     // predictor's epilogue: face states in qh's space and cell integrals,
-    // read by the faces and a light cell update (the same values)
+    // read by the faces and by the post-step's cell update (the same values)
     bool kinded = false;
     if constexpr (N == 3 && !TH) {
       if (B->mk == MK_GAS_NEWTONIAN) {
@@ -1758,7 +1758,6 @@ template <int N, bool TH> struct BatchStep {
                            dim3(B->nblk_p), dim3(WAVE), 0, c->stream, pa);
         hipLaunchKernelGGL((kb_faces<N, TH, MK_GAS_NEWTONIAN, true>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream,
                            b);
-        hipLaunchKernelGGL((kb_update<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
         kinded = true;
       }
     }
@@ -1768,7 +1767,10 @@ template <int N, bool TH> struct BatchStep {
       hipLaunchKernelGGL((kb_faces<N, TH>), dim3(B->nblk_f), dim3(WAVE), 0, c->stream, b);
       hipLaunchKernelGGL((kb_cells<N, TH>), dim3(B->nblk_c), dim3(WAVE), 0, c->stream, b);
     }
-    hipLaunchKernelGGL((kb_post<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
+    if (kinded)  // the fused corrector's cell update, then the post-step
+      hipLaunchKernelGGL((kb_post<TH, true>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
+    else
+      hipLaunchKernelGGL((kb_post<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b, 0);
     hipLaunchKernelGGL(kb_advance, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
     hipLaunchKernelGGL((kb_reset<TH>), dim3((B->vnu + blk - 1) / blk), dim3(blk), 0, c->stream, b);
     hipLaunchKernelGGL(kb_finalize, dim3((B->count + 63) / 64), dim3(64), 0, c->stream, b);
