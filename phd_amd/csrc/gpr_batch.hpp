@@ -20,7 +20,6 @@ struct BatchArgs {
   const long *u0s, *b0s, *e0s;  // instance offsets (count + 1 entries, last = total)
   const int *cblk, *fblk;       // cell-update and face block -> instance
   double *u;                    // [sum nx][V]
-  double *ub;                   // [sum (nx + 2N)][V]
   double *wh;                   // [sum (nx + 2)][N][V]
   double *qh;                   // [sum (nx + 2)][N^2][V]
   double *fx;                   // [sum (nx + 1)][2][V]
@@ -49,27 +48,11 @@ __device__ __forceinline__ int find_inst(const long *starts, int count, long x) 
   return lo;
 }
 
-// boundaries1 (grid.cpp:12-145) per instance: one thread per (row of ub, variable)
-template <int N, int V>
-__global__ void kb_boundaries(BatchArgs b) {
-  const long total = b.vb0s[b.vcount] * V;
-  for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
-    const int v = id % V;
-    const long vrow = id / V;
-    const int j = find_inst(b.vb0s, b.vcount, vrow);
-    const int k = b.vinst[j];
-    const BatchInst &I = b.inst[k];
-    if (b.st[k].done) continue;
-    const int Irow = (int)(vrow - b.vb0s[j]);
-    bool refl;
-    const int code = Irow < N ? I.bc0 : I.bc1;
-    const int i = bc_src(Irow, I.nx, N, code, refl);
-    const double *src = b.u + (I.u0 + i) * V;
-    b.ub[(I.b0 + Irow) * V + v] = refl ? bc_value(v, code, 0, [&](int m) { return src[m]; }) : src[v];
-  }
-}
-
-// weno_launcher in 1D (weno.cpp:145-147): one thread per (extended cell, variable)
+// weno_launcher in 1D (weno.cpp:145-147): one thread per (extended cell,
+// variable).  The stencil reads the ghost-extended row I of the instance
+// straight from u through boundaries1's map (grid.cpp:12-145: the source
+// cell, and the boundary code's reflection), so no ghost-extended copy of u
+// is written.
 template <int N, int V>
 __global__ void kb_weno(BatchArgs b) {
   const Basis &K = basis_dev<N>();
@@ -85,7 +68,13 @@ __global__ void kb_weno(BatchArgs b) {
     const long e = I.e0 + i;
     double d[2 * N - 1], o[N];
 #pragma unroll
-    for (int r = 0; r < 2 * N - 1; r++) d[r] = b.ub[(I.b0 + i + r) * V + v];
+    for (int r = 0; r < 2 * N - 1; r++) {
+      const int row = i + r;  // row of the ghost-extended instance
+      bool refl;
+      const int code = row < N ? I.bc0 : I.bc1;
+      const double *src = b.u + (I.u0 + bc_src(row, I.nx, N, code, refl)) * V;
+      d[r] = refl ? bc_value(v, code, 0, [&](int m) { return src[m]; }) : src[v];
+    }
     weno_point<double, N>(d, K, I.nx + 2, 0, i, o);
 #pragma unroll
     for (int a = 0; a < N; a++) b.wh[(e * N + a) * V + v] = o[a];

@@ -1734,7 +1734,6 @@ template <int N, bool TH> struct BatchStep {
     constexpr int V = Dims<TH>::V;
     const BatchArgs &b = B->args;
     const int blk = B->block;
-    hipLaunchKernelGGL((kb_boundaries<N, V>), dim3(grid_for(B->vnb * V, blk)), dim3(blk), 0, c->stream, b);
     hipLaunchKernelGGL((kb_weno<N, V>), dim3(grid_for(B->vne * V, blk)), dim3(blk), 0, c->stream, b);
     PredArgs pa = pred_args(c, B->inst[0].m, 0.0, &B->inst[0].dx);
     pa.qh = b.qh;
@@ -1949,7 +1948,8 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   B->nblk_c = (int)cblk.size();
   B->nblk_f = (int)fblk.size();
   B->view_count = count;
-  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 ub, 8 wh, 9 qh
+  // allocations: 0 inst, 1 st, 2 offsets (3 x (count + 1)), 3 pblk, 4 cblk, 5 fblk, 6 u, 7 (unused: WENO
+  // reads the ghost cells from u), 8 wh, 9 qh
   // (or the fused form's face states: 2N of its N^2 nodes per cell), 10 fx, 11 view instances, 12 view
   // offsets (3 x (count + 1)), 13 the fused form's cell integrals
   const bool fused = N == 3 && !c->thermal && B->mk == MK_GAS_NEWTONIAN;
@@ -1960,7 +1960,7 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
                             sizeof(int) * cblk.size(),
                             sizeof(int) * fblk.size(),
                             sizeof(double) * B->nu * V,
-                            sizeof(double) * B->nb * V,
+                            0,
                             sizeof(double) * B->ne * N * V,
                             sizeof(double) * B->ne * NN * V,
                             sizeof(double) * B->nf * 2 * V,
@@ -1999,7 +1999,6 @@ int gpr_batch_create(gpr_ctx *c, int count, const int32_t *nx, const double *dx,
   a.cblk = (const int *)B->allocs[4];
   a.fblk = (const int *)B->allocs[5];
   a.u = (double *)B->allocs[6];
-  a.ub = (double *)B->allocs[7];
   a.wh = (double *)B->allocs[8];
   a.qh = (double *)B->allocs[9];
   a.fx = (double *)B->allocs[10];
