@@ -471,6 +471,44 @@ __device__ __forceinline__ void speed_grad_th(const double *q, const Mat &m, dou
   }
 }
 
+// d s / d q_j, j < 14, of the wave speed s = max_abs_eigs<false>(q, m, D)
+// without differentiating the root iteration: at the root r of
+// p(x) = x^3 - c2 x^2 + c1 x - c0 that max_abs_eig3 settles on,
+// dr = (r^2 dc2 - r dc1 + dc0) / p'(r), the coefficients' derivatives taken
+// from the thermo-acoustic tensor in dual numbers (one seed per input); a
+// complex pair's modulus sqrt(|c0 / r|) follows from r's.
+template <int D>
+__device__ __forceinline__ void speed_grad(const double *q, const Mat &m, double *out) {
+  constexpr int V = 14;
+  double X[9];
+  thermo_acoustic_tensor<false, double>(q, m, D, X);
+  Eig3Root e;
+  const double lam2 = max_abs_eig3<double>(X, &e);
+  const double r = e.root, dp = (3 * r - 2 * e.c2) * r + e.c1;
+  const double mod2 = e.kind ? e.c0 / r : 0.0;
+  const double sq = sqrt(lam2), vd = q[2 + D] / q[0];
+  for (int j = 0; j < V; j++) {
+    Dual x[V], Xd[9];
+#pragma unroll
+    for (int v = 0; v < V; v++) x[v] = Dual(q[v], v == j ? 1.0 : 0.0);
+    thermo_acoustic_tensor<false, Dual>(x, m, D, Xd);
+    const Dual c2 = Xd[0] + Xd[4] + Xd[8];
+    const Dual c1 = Xd[0] * Xd[4] - Xd[1] * Xd[3] + Xd[0] * Xd[8] - Xd[2] * Xd[6] + Xd[4] * Xd[8] - Xd[5] * Xd[7];
+    const Dual c0 = Xd[0] * (Xd[4] * Xd[8] - Xd[5] * Xd[7]) - Xd[1] * (Xd[3] * Xd[8] - Xd[5] * Xd[6]) +
+                    Xd[2] * (Xd[3] * Xd[7] - Xd[4] * Xd[6]);
+    const double dr = (r * r * c2.d - r * c1.d + c0.d) / dp;
+    double dlam2;
+    if (e.kind == 0) {
+      dlam2 = r >= 0 ? dr : -dr;
+    } else {
+      const double dmod2 = (c0.d - mod2 * dr) / r;
+      dlam2 = (mod2 >= 0 ? dmod2 : -dmod2) / (2.0 * sqrt(fabs(mod2)));
+    }
+    const double dv = (j == 2 + D ? 1.0 / q[0] : 0.0) - (j == 0 ? vd / q[0] : 0.0);
+    out[j] = dlam2 / (2.0 * sq) + (vd > 0 ? dv : -dv);
+  }
+}
+
 // The transposed Riemann terms of one face node in direction D.  With
 // f = smax (qL - qR) + F(qL) + F(qR), smax = max(s(qL), s(qR)) (rusanov) and
 // b = Bint(qL, qR), out[j] = sum_v ls[v] db_v/dx_j + ld[v] df_v/dx_j for the
@@ -493,9 +531,11 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
     const bool with_speed = side == 0 ? Lmax : !Lmax;
     double fl[V];  // ld . dF(qs)/dq_j
     if constexpr (!DUAL) node_vjp<TH, false, D == 0, D == 1>(qs, nullptr, nullptr, m, nullptr, ld, ld, fl);
-    double sg[V];  // the wave speed's gradient, thermal case
+    double sg[V];  // the wave speed's gradient (the dual-number form differentiates it directly)
     if constexpr (TH) {
       if (with_speed) speed_grad_th<D>(qs, m, sg);
+    } else if constexpr (!DUAL) {
+      if (with_speed) speed_grad<D>(qs, m, sg);
     }
     for (int j = 0; j < V; j++) {
       Dual x[V];
@@ -513,7 +553,7 @@ __device__ __forceinline__ void riemann_T(const double *qL, const double *qR, co
         g += fl[j];
       }
       if (with_speed) {
-        if constexpr (TH)
+        if constexpr (TH || !DUAL)
           g += dd * sg[j];
         else
           g += dd * max_abs_eigs<false, Dual>(x, m, D).d;

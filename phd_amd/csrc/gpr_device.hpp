@@ -11,6 +11,8 @@
 // reference does.  State layout per cell: [ρ, ρE, ρv(3), A row-major (9), ρJ(3)].
 #pragma once
 
+#include <type_traits>
+
 #ifdef GPR_HOST_ONLY
 // host compile of the pointwise physics (flopcount.cpp): the templated
 // functions below are instantiated with a counting number type
@@ -405,10 +407,20 @@ __device__ __forceinline__ void bdot(const R *v, const R *x, int d, R *out) {
 }
 
 // ------------------------------------------------------------------ eigenvalues
+// Which root max_abs_eig3 settles on: kind 0 the real root `root` (the value
+// is |root|), kind 1 the complex pair beside the real root `root` (the value
+// is sqrt(|c0 / root|)); c2, c1, c0 the characteristic polynomial's
+// coefficients.
+struct Eig3Root {
+  int kind;
+  double root, c2, c1, c0;
+};
+
 // Largest |eigenvalue| of a real 3x3 matrix: roots of the characteristic
 // polynomial (trigonometric form when all are real, Cardano otherwise), the
-// dominant real root refined by Newton steps on the polynomial.
-template <typename R> __device__ __forceinline__ R max_abs_eig3(const R *a) {
+// dominant real root refined by Newton steps on the polynomial.  info (R =
+// double only) reports the root taken.
+template <typename R> __device__ __forceinline__ R max_abs_eig3(const R *a, Eig3Root *info = nullptr) {
   R c2 = a[0] + a[4] + a[8];
   R c1 = a[0] * a[4] - a[1] * a[3] + a[0] * a[8] - a[2] * a[6] + a[4] * a[8] - a[5] * a[7];
   R c0 = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
@@ -431,6 +443,8 @@ template <typename R> __device__ __forceinline__ R max_abs_eig3(const R *a) {
       if (!(nx < x)) break;
       x = nx;
     }
+    if constexpr (std::is_same<R, double>::value)
+      if (info) *info = {0, x, c2, c1, c0};
     return fabs(x);
   }
   if (disc <= 0.0) {
@@ -452,6 +466,8 @@ template <typename R> __device__ __forceinline__ R max_abs_eig3(const R *a) {
       if (!(fabs(nb - best) > 0)) { best = nb; break; }
       best = nb;
     }
+    if constexpr (std::is_same<R, double>::value)
+      if (info) *info = {0, best, c2, c1, c0};
     return fabs(best);
   }
   R sq = sqrt(disc);
@@ -467,6 +483,8 @@ template <typename R> __device__ __forceinline__ R max_abs_eig3(const R *a) {
   // complex pair: product of all roots = c0
   R mod2 = (lr != 0.0) ? c0 / lr : R(0.0);
   R mc = sqrt(fabs(mod2));
+  if constexpr (std::is_same<R, double>::value)
+    if (info) *info = {fabs(lr) >= mc ? 0 : 1, lr, c2, c1, c0};
   return fmax(fabs(lr), mc);
 }
 

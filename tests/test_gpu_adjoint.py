@@ -335,7 +335,11 @@ def test_sigmaY_gradient_against_fd():
 
 def test_plastic_reverse_mode_equals_dual_numbers():
     """The power-law branch of theta1inv in the hand-written reverse mode
-    against the dual numbers, on the plastic pulse."""
+    against the dual numbers, on the plastic pulse.  The hand-written form
+    takes the wave speed's derivative from the characteristic polynomial at
+    the root the eigenvalue iteration settles on (exact there), the dual
+    numbers through that iteration; on the solid the two sweeps then differ
+    by ~2e-11 relative (on the fluid, within 1e-12), hence 1e-10 here."""
     from phd_amd.adjoint import AdjointRun
     u0, MP, nX, dX, bcs = plastic_case()
     out = {}
@@ -350,8 +354,8 @@ def test_plastic_reverse_mode_equals_dual_numbers():
     (l0, g0), (l1, g1) = out[False], out[True]
     err = np.max(np.abs(l1 - l0)) / np.max(np.abs(l1))
     print(f"plastic: max |lam0 difference| / max |lam0| {err:.2e}, dL/dsigmaY {g0:.12e} vs {g1:.12e}")
-    assert err <= 1e-12
-    assert abs(g1 - g0) <= 1e-12 * abs(g1)
+    assert err <= 1e-10
+    assert abs(g1 - g0) <= 1e-10 * abs(g1)
 
 
 def thermal_case(seed=1, noise=1e-3):
