@@ -238,7 +238,8 @@ int gpr_dev_status(gpr_ctx *ctx, double *t, int64_t *count, double *dt, int32_t 
  * enable = 1 clears the totals and starts recording; 0 stops. */
 int gpr_dev_set_timing(gpr_ctx *ctx, int enable);
 /* Totals since timing was enabled (synchronises): ms[5] = boundaries, WENO,
- * predictor, FV (faces + cells with the fused post-step), reset + dt;
+ * predictor, FV (faces and cells, or faces and the cell update), post-step +
+ * reset + dt;
  * *steps = steps timed; iters[3] = Picard iterations, fallback iterations and
  * fallback cells summed over the predicted cells; cells = predicted cells
  * summed over the steps. */
@@ -300,8 +301,11 @@ int gpr_adj_release(gpr_ctx *ctx);
  * of the iterations).  tol: stopping tolerance on the change of the iterate,
  * relative to 1 + |iterate| (default 1e-13).  dual_local = 1 takes the local
  * derivatives of the node physics by forward-mode dual numbers, one seed per
- * input, instead of the hand-written reverse mode (default 0; same values to
- * round-off, several times slower). */
+ * input, instead of the hand-written reverse mode (default 0; several times
+ * slower).  The two agree to round-off: within 1e-12 relative on fluids, 2e-11
+ * on a power-law solid, where the reverse mode takes the wave speed's
+ * derivative at the computed eigenvalue and the dual numbers carry it through
+ * the eigenvalue iteration. */
 int gpr_adj_set_solver(gpr_ctx *ctx, int start_preconditioned, double tol, int dual_local);
 /* d loss / d kappa (the heat conductivity) of the last gpr_adj_backward; 0 for
  * a context without heat conduction */
