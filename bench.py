@@ -53,12 +53,12 @@ PICARD_EXIT = {"bailout": 0, "bail_from": 6}
 FLOPS = {
     "gas_newtonian_2d_p2": {"picard_node": 788, "fallback_node": 1083, "fallback_setup_spatial": 611,
                             "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
-                            "fv_face_lane": 1811, "fv_cell_node": 408, "fv_cell": 43614,
-                            "post_cell": 1075},
+                            "fv_face_lane": 1671, "fv_cell_node": 408, "fv_cell": 41094,
+                            "post_cell": 987},
     "gr_plastic_2d_p2": {"picard_node": 824, "fallback_node": 1119, "fallback_setup_spatial": 647,
                          "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
-                         "fv_face_lane": 1727, "fv_cell_node": 444, "fv_cell": 43074,
-                         "post_cell": 1122},
+                         "fv_face_lane": 1695, "fv_cell_node": 444, "fv_cell": 42498,
+                         "post_cell": 1054},
 }
 
 
