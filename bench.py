@@ -55,10 +55,10 @@ FLOPS = {
                             "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
                             "fv_face_lane": 1671, "fv_cell_node": 408, "fv_cell": 41094,
                             "post_cell": 987},
-    "gr_plastic_2d_p2": {"picard_node": 824, "fallback_node": 1119, "fallback_setup_spatial": 647,
+    "gr_plastic_2d_p2": {"picard_node": 829, "fallback_node": 1124, "fallback_setup_spatial": 652,
                          "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
-                         "fv_face_lane": 1695, "fv_cell_node": 444, "fv_cell": 42498,
-                         "post_cell": 1054},
+                         "fv_face_lane": 1697, "fv_cell_node": 449, "fv_cell": 42669,
+                         "post_cell": 1055},
 }
 
 

@@ -55,6 +55,7 @@ inline F fmax(F a, F b) { return F(std::fmax(a.v, b.v)); }
 inline F sqrt(F a) { g.special++; return F(std::sqrt(a.v)); }
 inline F cbrt(F a) { g.special++; return F(std::cbrt(a.v)); }
 inline F pow(F a, F b) { g.special++; return F(std::pow(a.v, b.v)); }
+inline double exponent_value(F b) { return b.v; }
 inline F exp(F a) { g.special++; return F(std::exp(a.v)); }
 inline F cos(F a) { g.special++; return F(std::cos(a.v)); }
 inline F acos(F a) { g.special++; return F(std::acos(a.v)); }

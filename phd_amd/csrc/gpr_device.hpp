@@ -108,7 +108,30 @@ inline int material_kind(const Mat &m) {
 }
 
 // ------------------------------------------------------------------ EOS
-template <typename R> __device__ __forceinline__ R dpow(R a, R b) { return pow(a, b); }
+// The value of an exponent, which decides how dpow evaluates (the dual-number
+// and counting types overload it beside their own definitions).
+__device__ __forceinline__ double exponent_value(double b) { return b; }
+__device__ __forceinline__ double exponent_value(float b) { return b; }
+
+// a^b.  A whole exponent from 2 to 16, such as a Godunov-Romenski shear
+// exponent beta = 3 or a power-law exponent n = 10, is evaluated by
+// repeated squaring (a^10 = a^2 a^8: four products), which costs a fraction
+// of pow's logarithm and exponential and differs from it by a few ulps.
+// The exponents are material constants, so every lane takes the same branch.
+template <typename R> __device__ __forceinline__ R dpow(R a, R b) {
+  const double e = exponent_value(b);
+  if (e >= 2.0 && e <= 16.0 && e == double(int(e))) {
+    int k = int(e) - 1;  // out = a^(1 + k)
+    R base = a, out = a;
+    while (k) {
+      if (k & 1) out = out * base;
+      k >>= 1;
+      if (k) base = base * base;
+    }
+    return out;
+  }
+  return pow(a, b);
+}
 
 template <typename R, int MK = MK_ANY> __device__ __forceinline__ R Gamma_MG(R r, const gpr_par &p) {
   switch (eos_code<MK>(p)) {

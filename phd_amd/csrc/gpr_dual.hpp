@@ -75,6 +75,7 @@ __device__ __forceinline__ Dual exp(const Dual &a) {
 __device__ __forceinline__ Dual log(const Dual &a) { return Dual(::log(a.v), a.d / a.v); }
 // a^b; the exponent's derivative term only where it is non-zero (a constant
 // exponent of a negative base stays finite)
+__device__ __forceinline__ double exponent_value(const Dual &b) { return b.v; }
 __device__ __forceinline__ Dual pow(const Dual &a, const Dual &b) {
   const double p = ::pow(a.v, b.v);
   double d = b.v * ::pow(a.v, b.v - 1.0) * a.d;
