@@ -53,12 +53,12 @@ PICARD_EXIT = {"bailout": 0, "bail_from": 6}
 FLOPS = {
     "gas_newtonian_2d_p2": {"picard_node": 788, "fallback_node": 1083, "fallback_setup_spatial": 611,
                             "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
-                            "fv_face_lane": 1671, "fv_cell_node": 408, "fv_cell": 41094,
-                            "post_cell": 987},
+                            "fv_face_lane": 1673, "fv_cell_node": 408, "fv_cell": 41130,
+                            "post_cell": 989},
     "gr_plastic_2d_p2": {"picard_node": 829, "fallback_node": 1124, "fallback_setup_spatial": 652,
                          "polar_iterations": 4, "weno_line": 182, "weno_cell": 10192,
-                         "fv_face_lane": 1697, "fv_cell_node": 449, "fv_cell": 42669,
-                         "post_cell": 1055},
+                         "fv_face_lane": 1699, "fv_cell_node": 449, "fv_cell": 42705,
+                         "post_cell": 1057},
 }
 
 

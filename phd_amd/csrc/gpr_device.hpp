@@ -613,14 +613,17 @@ __device__ __forceinline__ int thermo_acoustic_tensor_of(const R *q, const NodeS
   R r = s.r, cs2 = s.cs2;
   R dcs2 = mk_flag<MK>(MatKind<MK>::beta_zero, mm.beta_zero) ? R(0.0) : p.beta / r * cs2;
   R sd[3] = {s.sig[0 * 3 + d], s.sig[1 * 3 + d], s.sig[2 * 3 + d]};
-  R fac = 1.0 / r + dcs2 / cs2;
+  // one reciprocal of cs2 for the ten quotients below (within an ulp of
+  // dividing each)
+  const R ics2 = R(1) / cs2;
+  R fac = 1.0 / r + dcs2 * ics2;
   R dsdr[3] = {fac * sd[0], fac * sd[1], fac * sd[2]};
   // dsigma/dA (state.cpp:56-69): M = -rho cs2 (AdevG with col d doubled + A_d G_d/3 + G_dd A)^T
   R G[9];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) G[i * 3 + j] = A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j];
   R T9[9];
-  for (int i = 0; i < 9; i++) T9[i] = s.EA[i] / cs2;  // A dev G
+  for (int i = 0; i < 9; i++) T9[i] = s.EA[i] * ics2;  // A dev G
   for (int i = 0; i < 3; i++) T9[i * 3 + d] *= 2.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) T9[i * 3 + j] += 1.0 / 3.0 * A[i * 3 + d] * G[d * 3 + j] + G[d * 3 + d] * A[i * 3 + j];
